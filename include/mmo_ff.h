@@ -1,0 +1,187 @@
+/*
+ * mmo_ff.h -- C ABI of the MI355X (gfx950) scoring library libmmo_ff.so.
+ *
+ * Drop-in boundary for MMO's UFF nonbonded scoring path (UnixJunkie/MMO).  The
+ * reference is OCaml: its scoring functions are pure functions over Mol.t SoA
+ * arrays (src/mol.ml:17-35), consumed as closures `ene_inter : Mol.t -> float` and
+ * `ene_intra : Mol.t -> float` (src/lds.ml:702-739, :1952-1983).  Each entry point
+ * below names the reference function it replaces.  An OCaml binding is a thin
+ * `external` stub per entry point.
+ *
+ * Conventions
+ *   - All functions return MMO_OK (0) or a negative MMO_ERR_*; mmo_last_error()
+ *     gives a message.  Scoring itself never fails on finite input.
+ *   - Host pointers are caller-owned and only read/written during the call.
+ *     `_dev` entry points take device pointers and a HIP stream (`void*`, NULL =
+ *     the null stream); they enqueue work and return without synchronising.
+ *   - Coordinates are FP64 in the caller's frame.  The library re-origins them (in
+ *     FP64) and keeps receptor/ligand positions as 32-bit fixed point, so pair
+ *     differences are exact; pair arithmetic is FP32, per-pose sums are FP64.
+ *   - A context belongs to the process that created it: calls from a forked child
+ *     fail with MMO_ERR_FORK (HIP state does not survive fork()).
+ *   - A context must be used by one thread at a time.
+ */
+#ifndef MMO_FF_H
+#define MMO_FF_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mmo_ctx mmo_ctx;
+typedef struct mmo_batch mmo_batch;
+
+enum {
+  MMO_OK = 0,
+  MMO_ERR_BAD_ARG = -1,      /* NULL pointer, bad size, unsupported element ...      */
+  MMO_ERR_DEVICE = -2,       /* a HIP call failed                                     */
+  MMO_ERR_NOMEM = -3,        /* device or host allocation failed                      */
+  MMO_ERR_OUT_OF_GRID = -4,  /* an atom's interpolation cell is outside the grid      */
+  MMO_ERR_NO_RECEPTOR = -5,  /* mmo_set_receptor() has not been called                */
+  MMO_ERR_NO_GRID = -6,      /* mmo_set_grids() has not been called                   */
+  MMO_ERR_FORK = -7,         /* context used from a process other than its creator    */
+  MMO_ERR_RANGE = -8         /* an atom lies outside the fixed-point range (GLOBAL)   */
+};
+
+/* energy forms */
+enum {
+  MMO_GLOBAL = 0,    /* Mol.ene_inter_UFF_global_brute  src/mol.ml:796-818 (BrG)          */
+  MMO_SHIFTED = 1,   /* Mol.ene_inter_UFF_shifted_brute src/mol.ml:822-849 (BrL), and the
+                        BST form it equals, src/mol.ml:928-960 (Bst, the default -ff)     */
+  MMO_INTERP = 2     /* Mol.ene_inter_UFF_interp        src/mol.ml:1012-1020              */
+};
+
+/* ---------------------------------------------------------------- context */
+int  mmo_ctx_create(int device, mmo_ctx** out);
+void mmo_ctx_destroy(mmo_ctx* ctx);
+const char* mmo_last_error(const mmo_ctx* ctx);   /* ctx may be NULL: last global error */
+/* library version string, e.g. "mmo_ff 1.0 gfx950" */
+const char* mmo_version(void);
+
+/* UFF per-element nonbond parameters.  Replaces UFF.anum_xi_Di (src/UFF.ml:30-42);
+ * pair parameters follow the same geometric-mean rule (src/UFF.ml:62-71).  The
+ * default table (set at creation) is the reference's: anum 0 (virtual), H, C, N, O,
+ * F, Mg, P, S, Cl, Br, I.  Elements outside the table are rejected (the reference
+ * reads NaN parameters for them, src/UFF.ml:57). */
+int mmo_set_uff_table(mmo_ctx* ctx, const int32_t* anum, const double* x, const double* d, int n);
+
+/* Receptor (the `prot` Mol.t of src/mol.ml:796-960).  Uploaded once; replaces
+ * the previous receptor.  origin (nullable): fixed-point frame centre; default is
+ * the centre of the receptor's bounding box. */
+int mmo_set_receptor(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
+                     const double* q, const int32_t* anum, int n, const double* origin);
+
+/* Per-type FP32 grids (G3D.t array, src/G3D.ml:11; the `ff_comps` of
+ * src/mol.ml:1012).  grids[t] has nx*ny*nz floats, index i + j*nx + k*nx*ny
+ * (src/grid.ml:98-99); origin is the position of node (0,0,0) (the reference's
+ * grid starts at (0,0,0), src/grid.ml:9). */
+int mmo_set_grids(mmo_ctx* ctx, const float* const* grids, int ntypes, double step,
+                  int nx, int ny, int nz, const double* origin);
+
+/* ------------------------------------------------- plain forms (one ligand) */
+/* Mol.ene_inter_UFF_global_brute / _shifted_brute / _shifted_bst_components
+ * (src/mol.ml:796-960).  e_elec/e_vdw nullable; e_elec already includes the
+ * 332.0637/4 factor (src/mol.ml:955). */
+int mmo_ene_inter(mmo_ctx* ctx, int mode, const double* xs, const double* ys, const double* zs,
+                  const double* q, const int32_t* anum, int n,
+                  double* e, double* e_elec, double* e_vdw);
+
+/* Mol.ene_inter_UFF_interp (src/mol.ml:1012-1020) with G3D.trilin (src/G3D.ml:97-157);
+ * type[j] selects the grid (Mol.t.t_a, src/mol.ml:455-461). */
+int mmo_ene_inter_interp(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
+                         const int32_t* type, int n, double* e);
+
+/* Mol.ene_intra_UFFNB_brute (src/mol.ml:881-903); dists is the topological distance
+ * matrix as stored in Mol.t.dists_a: dists[i + j*n] (src/mol.ml:151-152). */
+int mmo_ene_intra(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
+                  const double* q, const int32_t* anum, const int32_t* dists, int n, double* e);
+
+/* Gradient forms of the three plain functions: dE/dxyz for each ligand atom
+ * (the reference only has central finite differences, src/lds.ml:1118-1136). */
+int mmo_ene_inter_grad(mmo_ctx* ctx, int mode, const double* xs, const double* ys,
+                       const double* zs, const double* q, const int32_t* anum, int n,
+                       double* e, double* gx, double* gy, double* gz);
+int mmo_ene_intra_grad(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
+                       const double* q, const int32_t* anum, const int32_t* dists, int n,
+                       double* e, double* gx, double* gy, double* gz);
+int mmo_ene_inter_interp_grad(mmo_ctx* ctx, const double* xs, const double* ys,
+                              const double* zs, const int32_t* type, int n,
+                              double* e, double* gx, double* gy, double* gz);
+
+/* --------------------------------------------------------- ligand registry */
+/* Registers a ligand topology for the batched, DOF, scan and MC forms: the
+ * centred conformer (Lds.preprocess_ligand, src/lds.ml:45-51), charges, atomic
+ * numbers, the dists matrix (dists[i + j*n]), rotatable bonds with the axis
+ * left->right and rotation groups rg_atoms[rg_off[k]..rg_off[k+1]) excluding
+ * `right` (src/pqrs.ml:204-250), and grid types (nullable).  Returns an id >= 0. */
+int mmo_ligand_add(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
+                   const double* q, const int32_t* anum, const int32_t* dists, int n,
+                   const int32_t* rb_left, const int32_t* rb_right, const int32_t* rg_off,
+                   const int32_t* rg_atoms, int nrb, const int32_t* types, int* lig_id);
+int mmo_ligand_count(const mmo_ctx* ctx);
+int mmo_ligand_natoms(const mmo_ctx* ctx, int lig_id);
+
+/* ------------------------------------------------------------ batched forms */
+/* A batch is a fixed list of poses (pose p is a conformation of ligand
+ * pose_lig[p]); its atoms are stored pose-major: atom j of pose p is row
+ * atom_off[p] + j of an (n_atoms_total x 3) FP64 array.  Planning (grouping poses
+ * of one ligand into wavefront tiles, tile shape from the autotuner) happens here. */
+int mmo_batch_create(mmo_ctx* ctx, const int32_t* pose_lig, int npose, mmo_batch** out);
+void mmo_batch_destroy(mmo_batch* b);
+int mmo_batch_natoms(const mmo_batch* b);           /* total atoms (rows)      */
+int mmo_batch_atom_offsets(const mmo_batch* b, int32_t* off /* npose+1 */);
+
+/* flags for batched scoring */
+enum {
+  MMO_F_INTER_SHIFTED = 1,   /* e_out[p*4+0] += shifted direct E_inter                 */
+  MMO_F_INTER_GLOBAL = 2,    /* e_out[p*4+0] += global direct E_inter                  */
+  MMO_F_INTER_INTERP = 4,    /* e_out[p*4+0] += interpolated E_inter                   */
+  MMO_F_INTRA = 8,           /* e_out[p*4+1]  = E_intra                                */
+  MMO_F_GRAD = 16,           /* also dE/dxyz (inter + intra) into grad (n_atoms x 3)   */
+  MMO_F_COMPONENTS = 32      /* e_out[p*4+2], [p*4+3] = elec, vdW parts of direct E_inter */
+};
+
+/* Scores every pose of the batch from coordinates.  xyz: device FP64
+ * (n_atoms_total x 3); e_out: device FP64 (npose x 4: inter, intra, elec, vdW);
+ * grad: device FP64 (n_atoms_total x 3) or NULL.  Out-of-grid poses (INTERP) get
+ * NaN in e_out[p*4+0]; poses with atoms outside the fixed-point range in GLOBAL mode
+ * get NaN as well. */
+int mmo_batch_score_dev(mmo_batch* b, int flags, const double* xyz, double* e_out,
+                        double* grad, void* stream);
+/* Host-pointer variant (copies in and out; synchronous). */
+int mmo_batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out, double* grad);
+
+/* DOF form.  Optim.apply_config (src/optim.ml:64-80): for pose p the DOF vector is
+ * dofs[dof_off[p] .. dof_off[p+1]) = [x, y, z, alpha, beta, gamma, tau_0..tau_{R-1}]
+ * (src/optim.ml:19-22), torsions applied in rbond order to the registered centred
+ * conformer (Mol.rotate_bond, src/mol.ml:610-631), then Rot.r_xyz and translation.
+ * Writes coordinates to xyz_out (n_atoms_total x 3, nullable), energies to e_out,
+ * and with MMO_F_GRAD dE/dDOF to gdof (same layout as dofs).  radius_out (nullable,
+ * npose) gets the enclosing radius after the torsions (Mol.check_elongation_exn uses
+ * > 12 A as Too_long, src/optim.ml:75).  All pointers are device pointers. */
+int mmo_batch_dof_offsets(const mmo_batch* b, int32_t* dof_off /* npose+1 */);
+int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double* xyz_out,
+                             double* e_out, double* gdof, double* radius_out, void* stream);
+
+/* --------------------------------------------------------------- rigid scan */
+/* Exhaustive rigid scan (Lds.exhaustive_rigid_ligand_docking, src/lds.ml:1040-1114,
+ * without its clash prefilters): pose (r, t) places the registered centred ligand at
+ * R_r c + T_t (Mol.centered_rotate_copy + translate_copy_to, src/lds.ml:1081-1094);
+ * e_out[r * ntrans + t] = E_inter of that pose in `mode` (MMO_SHIFTED, MMO_GLOBAL or
+ * MMO_INTERP).  rots: host (nrot x 9, row major); trans: host (ntrans x 3);
+ * e_out: device (nrot * ntrans).  Enqueued on `stream`. */
+int mmo_scan_dev(mmo_ctx* ctx, int lig_id, int mode, const double* rots, int nrot,
+                 const double* trans, int ntrans, double* e_out, void* stream);
+
+/* ------------------------------------------------------------------ tuning */
+/* Kernel time of the last launch of `kernel` ("inter", "interp", "intra") measured
+ * with HIP events on the launch stream, in milliseconds (-1 if none). */
+double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
+/* Fixed-point scale exponent k (coordinates are stored as round(x * 2^k)). */
+int mmo_fixed_point_bits(const mmo_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

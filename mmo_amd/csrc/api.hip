@@ -1,0 +1,999 @@
+// api.hip -- the C ABI of libmmo_ff.so (declared in include/mmo_ff.h).
+//
+// Host responsibilities: validating inputs, the UFF per-element table, uploading the
+// receptor as fixed-point positions + factorised LJ parameters, the ligand registry
+// (topology, intra pair lists, rotation groups), planning pose tiles for batches, and
+// launching the kernels of ff_kernels.hip / pose_kernels.hip on a HIP stream.
+#include <hip/hip_runtime.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "mmo_ff.h"
+
+using namespace mmo;
+
+namespace {
+
+std::string g_last_error;
+
+struct Uff { double x, d; };
+
+// src/UFF.ml:30-42
+const std::map<int, Uff> kDefaultUff = {
+    {0, {0.0, 0.0}},     {1, {2.886, 0.044}},  {6, {3.851, 0.105}},  {7, {3.660, 0.069}},
+    {8, {3.500, 0.060}}, {9, {3.364, 0.050}},  {12, {3.021, 0.111}}, {15, {4.147, 0.305}},
+    {16, {4.035, 0.274}}, {17, {3.947, 0.227}}, {35, {4.189, 0.251}}, {53, {4.500, 0.339}}};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    const size_t c = std::max<size_t>(count, 1);
+    hipError_t e = hipMalloc(&p, c * sizeof(T));
+    if (e == hipSuccess) n = c;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct LigHost {
+  int n;
+  std::vector<double> xyz;   // centred conformer, rows
+  std::vector<double> q;
+  std::vector<int> anum, type;
+  std::vector<uint32_t> pair_ij;
+  std::vector<int> rb_left, rb_right, rg_off, rg_atoms;
+};
+
+}  // namespace
+
+struct mmo_ctx {
+  int device = 0;
+  pid_t pid = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int, Uff> uff = kDefaultUff;
+  // receptor
+  bool has_rec = false;
+  int nrec = 0;
+  Frame fr{};
+  int fp_bits = 24;
+  DevBuf<RecPos> rpos;
+  DevBuf<RecAB> rab;
+  // grids
+  bool has_grid = false;
+  int ntypes = 0;
+  GridDev g{};
+  DevBuf<float> grid;
+  // ligand registry
+  std::vector<LigHost> ligs;
+  bool lig_dirty = true;
+  int max_atoms = 0;
+  DevBuf<LigInfo> d_ligs;
+  DevBuf<LigAtom> d_latoms;
+  DevBuf<double> d_lig_xyz;
+  DevBuf<uint32_t> d_pair_ij;
+  DevBuf<double4> d_pair_par;
+  DevBuf<int> d_rb_left, d_rb_right, d_rg_off, d_rg_atoms;
+  // scratch for the plain (one ligand) forms
+  DevBuf<double> s_xyz, s_e, s_g, s_part;
+  DevBuf<LigInfo> s_lig;
+  DevBuf<LigAtom> s_latom;
+  DevBuf<uint32_t> s_pij;
+  DevBuf<double4> s_ppar;
+  DevBuf<int> s_int;
+  DevBuf<Tile> s_tile;
+  DevBuf<double> s_scan;   // rotations + translations of the last scan
+  // tuning
+  int inter_k = 4, inter_waves = 4, inter_rtile = 2048;
+  std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::map<std::string, float> last_ms;   // -2: pending
+};
+
+struct mmo_batch {
+  mmo_ctx* ctx = nullptr;
+  int npose = 0, natoms = 0, ndof = 0, ntiles = 0, max_atoms = 0;
+  std::vector<int> pose_lig, atom_off, dof_off, perm;
+  std::vector<Tile> tiles;
+  DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm;
+  DevBuf<Tile> d_tiles;
+  DevBuf<double> d_xyz, d_e, d_g, d_part;
+};
+
+namespace {
+
+int fail(mmo_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, expr)                                                                     \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return fail((c), _e == hipErrorOutOfMemory ? MMO_ERR_NOMEM : MMO_ERR_DEVICE,         \
+                  "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__);     \
+  } while (0)
+
+int check_ctx(mmo_ctx* c) {
+  if (!c) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL context");
+  if (getpid() != c->pid)
+    return fail(c, MMO_ERR_FORK, "context created by pid %d used from pid %d", (int)c->pid,
+                (int)getpid());
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return fail(c, MMO_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+  return MMO_OK;
+}
+
+// A = sqrt(D) x^6, B = sqrt(D) x^3 (factorised geometric-mean pair parameters)
+int atom_ab(mmo_ctx* c, int anum, double* A, double* B) {
+  auto it = c->uff.find(anum);
+  if (it == c->uff.end()) return fail(c, MMO_ERR_BAD_ARG, "unsupported atomic number %d", anum);
+  const double x = it->second.x, sd = std::sqrt(it->second.d);
+  const double x3 = x * x * x;
+  *A = sd * x3 * x3;
+  *B = sd * x3;
+  return MMO_OK;
+}
+
+template <class T>
+int upload(mmo_ctx* c, DevBuf<T>& b, const T* src, size_t n, hipStream_t s) {
+  HIPCHK(c, b.ensure(n));
+  if (n) HIPCHK(c, hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return MMO_OK;
+}
+
+std::vector<uint32_t> intra_pairs(const int32_t* dists, int n) {
+  // src/mol.ml:203-208 (interacting = dists[i + j*n] >= 3), pairs i < j (src/mol.ml:885-890)
+  std::vector<uint32_t> out;
+  for (int i = 0; i < n - 1; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (dists[i + (size_t)j * n] >= 3) out.push_back((uint32_t)i | ((uint32_t)j << 16));
+  return out;
+}
+
+int pair_params(mmo_ctx* c, const std::vector<uint32_t>& pij, const double* q,
+                const int32_t* anum, std::vector<double4>& out) {
+  out.resize(pij.size());
+  for (size_t p = 0; p < pij.size(); ++p) {
+    const int i = pij[p] & 0xffff, j = pij[p] >> 16;
+    double Ai, Bi, Aj, Bj;
+    int r = atom_ab(c, anum[i], &Ai, &Bi);
+    if (r) return r;
+    r = atom_ab(c, anum[j], &Aj, &Bj);
+    if (r) return r;
+    out[p] = make_double4(kElecWeight * q[i] * q[j], Ai * Aj, 2.0 * Bi * Bj, 0.0);
+  }
+  return MMO_OK;
+}
+
+int sync_ligands(mmo_ctx* c) {
+  if (!c->lig_dirty) return MMO_OK;
+  std::vector<LigInfo> info;
+  std::vector<LigAtom> atoms;
+  std::vector<double> xyz;
+  std::vector<uint32_t> pij;
+  std::vector<double4> ppar;
+  std::vector<int> rbl, rbr, rgo, rga;
+  c->max_atoms = 0;
+  for (const LigHost& h : c->ligs) {
+    LigInfo li{};
+    li.n_atoms = h.n;
+    li.atom_base = (int)atoms.size();
+    li.pair_base = (int)pij.size();
+    li.n_pairs = (int)h.pair_ij.size();
+    li.nrb = (int)h.rb_left.size();
+    li.rb_base = (int)rbl.size();
+    li.rg_base = (int)rgo.size();
+    li.dof_n = 6 + li.nrb;
+    for (int j = 0; j < h.n; ++j) {
+      double A, B;
+      int r = atom_ab(c, h.anum[j], &A, &B);
+      if (r) return r;
+      LigAtom la{kElecWeight * h.q[j], A, B, h.type[j], h.anum[j]};
+      atoms.push_back(la);
+    }
+    xyz.insert(xyz.end(), h.xyz.begin(), h.xyz.end());
+    std::vector<double4> pp;
+    int r = pair_params(c, h.pair_ij, h.q.data(), h.anum.data(), pp);
+    if (r) return r;
+    pij.insert(pij.end(), h.pair_ij.begin(), h.pair_ij.end());
+    ppar.insert(ppar.end(), pp.begin(), pp.end());
+    rbl.insert(rbl.end(), h.rb_left.begin(), h.rb_left.end());
+    rbr.insert(rbr.end(), h.rb_right.begin(), h.rb_right.end());
+    const int ga = (int)rga.size();
+    for (int v : h.rg_off) rgo.push_back(v + ga);
+    rga.insert(rga.end(), h.rg_atoms.begin(), h.rg_atoms.end());
+    info.push_back(li);
+    c->max_atoms = std::max(c->max_atoms, h.n);
+  }
+  hipStream_t s = c->stream;
+  int r;
+  if ((r = upload(c, c->d_ligs, info.data(), info.size(), s))) return r;
+  if ((r = upload(c, c->d_latoms, atoms.data(), atoms.size(), s))) return r;
+  if ((r = upload(c, c->d_lig_xyz, xyz.data(), xyz.size(), s))) return r;
+  if ((r = upload(c, c->d_pair_ij, pij.data(), pij.size(), s))) return r;
+  if ((r = upload(c, c->d_pair_par, ppar.data(), ppar.size(), s))) return r;
+  if ((r = upload(c, c->d_rb_left, rbl.data(), rbl.size(), s))) return r;
+  if ((r = upload(c, c->d_rb_right, rbr.data(), rbr.size(), s))) return r;
+  if ((r = upload(c, c->d_rg_off, rgo.data(), rgo.size(), s))) return r;
+  if ((r = upload(c, c->d_rg_atoms, rga.data(), rga.size(), s))) return r;
+  HIPCHK(c, hipStreamSynchronize(s));
+  c->lig_dirty = false;
+  return MMO_OK;
+}
+
+InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAtom* latoms,
+                         const double* lig_xyz) {
+  InterLaunch L;
+  L.mode = mode;
+  L.k = c->inter_k;
+  L.waves = c->inter_waves;
+  L.rtile = std::min(c->inter_rtile, std::max(c->nrec, 1));
+  L.rslice = c->nrec;
+  L.fr = c->fr;
+  L.rpos = c->rpos.p;
+  L.rab = c->rab.p;
+  L.nrec = c->nrec;
+  L.ligs = ligs;
+  L.latoms = latoms;
+  L.lig_xyz = lig_xyz;
+  return L;
+}
+
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Pose tiles: poses grouped by ligand; `split` lanes per pose so that a wavefront
+// holds min(64, #poses) poses of one ligand (split = 64 / poses per tile).
+void plan_tiles(const std::vector<int>& pose_lig, const std::vector<LigHost>* ligs,
+                const std::vector<int>* natoms, int k, std::vector<int>& perm,
+                std::vector<Tile>& tiles) {
+  const int npose = (int)pose_lig.size();
+  perm.resize(npose);
+  for (int p = 0; p < npose; ++p) perm[p] = p;
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int a, int b) { return pose_lig[a] < pose_lig[b]; });
+  tiles.clear();
+  int p = 0;
+  while (p < npose) {
+    int q = p;
+    while (q < npose && pose_lig[perm[q]] == pose_lig[perm[p]]) ++q;
+    const int lig = pose_lig[perm[p]];
+    const int m = q - p;
+    const int n_at = ligs ? (*ligs)[lig].n : (*natoms)[lig];
+    (void)n_at;
+    (void)k;
+    const int per = std::min(kWave, next_pow2(m));   // poses per tile
+    const int split = kWave / per;                   // lanes per pose
+    for (int s = p; s < q; s += per) tiles.push_back(Tile{lig, s, std::min(per, q - s), split});
+    p = q;
+  }
+}
+
+// HIP events bracketing the last launch of each kernel family, on its stream
+void timer_begin(mmo_ctx* c, const char* name, hipStream_t s) {
+  (void)hipEventRecord(c->ev[name].first, s);
+}
+void timer_end(mmo_ctx* c, const char* name, hipStream_t s) {
+  (void)hipEventRecord(c->ev[name].second, s);
+  c->last_ms[name] = -2.0f;   // resolved by mmo_last_kernel_ms
+}
+
+// plain one-ligand helper: uploads coordinates and a one-entry ligand table
+int plain_setup(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                const double* q, const int32_t* anum, int n, bool need_params) {
+  std::vector<double> xyz(3 * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    xyz[3 * j] = xs[j]; xyz[3 * j + 1] = ys[j]; xyz[3 * j + 2] = zs[j];
+  }
+  std::vector<LigAtom> la(n);
+  for (int j = 0; j < n; ++j) {
+    double A = 0.0, B = 0.0;
+    if (need_params) {
+      int r = atom_ab(c, anum[j], &A, &B);
+      if (r) return r;
+    }
+    la[j] = LigAtom{need_params ? kElecWeight * q[j] : 0.0, A, B, 0, need_params ? anum[j] : 0};
+  }
+  LigInfo li{};
+  li.n_atoms = n;
+  li.dof_n = 6;
+  int r;
+  hipStream_t s = c->stream;
+  if ((r = upload(c, c->s_xyz, xyz.data(), xyz.size(), s))) return r;
+  if ((r = upload(c, c->s_latom, la.data(), la.size(), s))) return r;
+  if ((r = upload(c, c->s_lig, &li, 1, s))) return r;
+  return MMO_OK;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+const char* mmo_version(void) { return "mmo_ff 1.0 gfx950"; }
+
+const char* mmo_last_error(const mmo_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+int mmo_ctx_create(int device, mmo_ctx** out) {
+  if (!out) return fail(nullptr, MMO_ERR_BAD_ARG, "out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, MMO_ERR_DEVICE, "no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev)
+    return fail(nullptr, MMO_ERR_BAD_ARG, "device %d out of range (%d devices)", device, ndev);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(nullptr, MMO_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+  mmo_ctx* c = new mmo_ctx();
+  c->device = device;
+  c->pid = getpid();
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return fail(nullptr, MMO_ERR_DEVICE, "stream creation failed");
+  }
+  for (const char* k : {"inter", "interp", "intra"}) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      delete c;
+      return fail(nullptr, MMO_ERR_DEVICE, "event creation failed");
+    }
+    c->ev[k] = {a, b};
+  }
+  *out = c;
+  return MMO_OK;
+}
+
+void mmo_ctx_destroy(mmo_ctx* c) {
+  if (!c) return;
+  if (getpid() == c->pid) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->rpos.release(); c->rab.release(); c->grid.release();
+    c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
+    c->d_pair_ij.release(); c->d_pair_par.release();
+    c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
+    c->s_xyz.release(); c->s_e.release(); c->s_g.release(); c->s_part.release();
+    c->s_lig.release(); c->s_latom.release(); c->s_pij.release(); c->s_ppar.release();
+    c->s_int.release(); c->s_tile.release(); c->s_scan.release();
+    for (auto& kv : c->ev) {
+      (void)hipEventDestroy(kv.second.first);
+      (void)hipEventDestroy(kv.second.second);
+    }
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
+
+int mmo_set_uff_table(mmo_ctx* c, const int32_t* anum, const double* x, const double* d, int n) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!anum || !x || !d || n <= 0) return fail(c, MMO_ERR_BAD_ARG, "bad UFF table arguments");
+  std::map<int, Uff> t;
+  for (int i = 0; i < n; ++i) {
+    if (!(x[i] >= 0.0) || !(d[i] >= 0.0) || !std::isfinite(x[i]) || !std::isfinite(d[i]))
+      return fail(c, MMO_ERR_BAD_ARG, "UFF entry %d: x and D must be finite and >= 0", i);
+    t[anum[i]] = Uff{x[i], d[i]};
+  }
+  c->uff = t;
+  c->lig_dirty = true;
+  c->has_rec = false;   // receptor parameters depend on the table: upload again
+  return MMO_OK;
+}
+
+int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                     const double* q, const int32_t* anum, int n, const double* origin) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xs || !ys || !zs || !q || !anum || n <= 0)
+    return fail(c, MMO_ERR_BAD_ARG, "bad receptor arguments (n=%d)", n);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n; ++i) {
+    const double v[3] = {xs[i], ys[i], zs[i]};
+    for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(v[a])) return fail(c, MMO_ERR_BAD_ARG, "receptor atom %d not finite", i);
+      lo[a] = std::min(lo[a], v[a]);
+      hi[a] = std::max(hi[a], v[a]);
+    }
+    if (!std::isfinite(q[i])) return fail(c, MMO_ERR_BAD_ARG, "receptor charge %d not finite", i);
+  }
+  Frame fr{};
+  if (origin) { fr.ox = origin[0]; fr.oy = origin[1]; fr.oz = origin[2]; }
+  else { fr.ox = 0.5 * (lo[0] + hi[0]); fr.oy = 0.5 * (lo[1] + hi[1]); fr.oz = 0.5 * (lo[2] + hi[2]); }
+  double ext = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double o = a == 0 ? fr.ox : a == 1 ? fr.oy : fr.oz;
+    ext = std::max(ext, std::max(std::fabs(hi[a] - o), std::fabs(lo[a] - o)));
+  }
+  // fixed point: |x| <= 2^(30-k) must cover the receptor plus 16 A, so that ligand
+  // atoms saturated at the range boundary are beyond the 12 A cutoff
+  int k = 26;
+  while (k > 8 && std::ldexp(1.0, 30 - k) < ext + 16.0) --k;
+  fr.scale = std::ldexp(1.0, k);
+  fr.inv_s = (float)std::ldexp(1.0, -k);
+  fr.inv_s2 = (float)std::ldexp(1.0, -2 * k);
+  fr.lim = (1 << 30) - 1;
+  std::vector<RecPos> pos(n);
+  std::vector<RecAB> ab(n);
+  for (int i = 0; i < n; ++i) {
+    double A, B;
+    r = atom_ab(c, anum[i], &A, &B);
+    if (r) return r;
+    pos[i].x = (int)std::llrint((xs[i] - fr.ox) * fr.scale);
+    pos[i].y = (int)std::llrint((ys[i] - fr.oy) * fr.scale);
+    pos[i].z = (int)std::llrint((zs[i] - fr.oz) * fr.scale);
+    pos[i].q = (float)q[i];
+    ab[i].a = (float)A;
+    ab[i].b = (float)B;
+  }
+  if ((r = upload(c, c->rpos, pos.data(), pos.size(), c->stream))) return r;
+  if ((r = upload(c, c->rab, ab.data(), ab.size(), c->stream))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->fr = fr;
+  c->fp_bits = k;
+  c->nrec = n;
+  c->has_rec = true;
+  return MMO_OK;
+}
+
+int mmo_set_grids(mmo_ctx* c, const float* const* grids, int ntypes, double step, int nx, int ny,
+                  int nz, const double* origin) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!grids || ntypes <= 0 || !(step > 0.0) || nx < 2 || ny < 2 || nz < 2)
+    return fail(c, MMO_ERR_BAD_ARG, "bad grid arguments");
+  const size_t nvox = (size_t)nx * ny * nz;
+  HIPCHK(c, c->grid.ensure(nvox * ntypes));
+  for (int t = 0; t < ntypes; ++t) {
+    if (!grids[t]) return fail(c, MMO_ERR_BAD_ARG, "grid %d is NULL", t);
+    HIPCHK(c, hipMemcpyAsync(c->grid.p + t * nvox, grids[t], nvox * sizeof(float),
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  GridDev g{};
+  g.data = c->grid.p;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nvox = nvox;
+  g.step = step;
+  g.ods = 1.0 / step;   // Grid.one_div_step, src/grid.ml:41
+  g.ox = origin ? origin[0] : 0.0;
+  g.oy = origin ? origin[1] : 0.0;
+  g.oz = origin ? origin[2] : 0.0;
+  c->g = g;
+  c->ntypes = ntypes;
+  c->has_grid = true;
+  return MMO_OK;
+}
+
+// ------------------------------------------------------------------ plain forms
+static int plain_inter(mmo_ctx* c, int mode, const double* xs, const double* ys, const double* zs,
+                       const double* q, const int32_t* anum, int n, double* e, double* e_elec,
+                       double* e_vdw, double* gx, double* gy, double* gz) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (mode != MMO_GLOBAL && mode != MMO_SHIFTED) return fail(c, MMO_ERR_BAD_ARG, "bad mode %d", mode);
+  if (!xs || !ys || !zs || !q || !anum || n <= 0 || n > 65535 || !e)
+    return fail(c, MMO_ERR_BAD_ARG, "bad ligand arguments (n=%d)", n);
+  if ((r = plain_setup(c, xs, ys, zs, q, anum, n, true))) return r;
+  hipStream_t s = c->stream;
+  const bool grad = gx != nullptr;
+  // one pose: split the receptor over block rows so the whole chip works on it
+  InterLaunch L = inter_launch(c, mode, c->s_lig.p, c->s_latom.p, nullptr);
+  L.rslice = std::max(64, std::min(c->nrec, 256));
+  L.rtile = std::min(L.rtile, L.rslice);
+  L.waves = 1;
+  const int nslice = (c->nrec + L.rslice - 1) / L.rslice;
+  int zero = 0;
+  Tile t{0, 0, 1, kWave};
+  if ((r = upload(c, c->s_int, &zero, 1, s))) return r;    // atom_off = perm = {0}
+  if ((r = upload(c, c->s_tile, &t, 1, s))) return r;
+  HIPCHK(c, c->s_e.ensure(4));
+  HIPCHK(c, c->s_part.ensure((size_t)nslice * 4));
+  if (grad) {
+    HIPCHK(c, c->s_g.ensure(3 * (size_t)n));
+    HIPCHK(c, launch_fill(c->s_g.p, 3 * (size_t)n, 0.0, s));
+  }
+  HIPCHK(c, launch_inter_coords(L, c->s_tile.p, c->s_int.p, 1, 1, c->s_xyz.p, c->s_int.p,
+                                c->s_e.p, 4, !grad, grad ? c->s_g.p : nullptr, 1,
+                                c->s_part.p, s));
+  double h[4];
+  HIPCHK(c, hipMemcpyAsync(h, c->s_e.p, sizeof h, hipMemcpyDeviceToHost, s));
+  std::vector<double> g;
+  if (grad) {
+    g.resize(3 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->s_g.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (std::isnan(h[0])) return fail(c, MMO_ERR_RANGE, "ligand atom outside the fixed-point range");
+  *e = h[0];
+  if (e_elec) *e_elec = h[2];
+  if (e_vdw) *e_vdw = h[3];
+  if (grad)
+    for (int j = 0; j < n; ++j) { gx[j] = g[3 * j]; gy[j] = g[3 * j + 1]; gz[j] = g[3 * j + 2]; }
+  return MMO_OK;
+}
+
+int mmo_ene_inter(mmo_ctx* c, int mode, const double* xs, const double* ys, const double* zs,
+                  const double* q, const int32_t* anum, int n, double* e, double* e_elec,
+                  double* e_vdw) {
+  return plain_inter(c, mode, xs, ys, zs, q, anum, n, e, e_elec, e_vdw, nullptr, nullptr, nullptr);
+}
+
+int mmo_ene_inter_grad(mmo_ctx* c, int mode, const double* xs, const double* ys, const double* zs,
+                       const double* q, const int32_t* anum, int n, double* e, double* gx,
+                       double* gy, double* gz) {
+  if (!gx || !gy || !gz) return fail(c, MMO_ERR_BAD_ARG, "gradient outputs are NULL");
+  return plain_inter(c, mode, xs, ys, zs, q, anum, n, e, nullptr, nullptr, gx, gy, gz);
+}
+
+static int plain_interp(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                        const int32_t* type, int n, double* e, double* gx, double* gy, double* gz) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_grid) return fail(c, MMO_ERR_NO_GRID, "no grids set");
+  if (!xs || !ys || !zs || !type || n <= 0 || !e) return fail(c, MMO_ERR_BAD_ARG, "bad arguments");
+  for (int j = 0; j < n; ++j)
+    if (type[j] < 0 || type[j] >= c->ntypes)
+      return fail(c, MMO_ERR_BAD_ARG, "atom %d: type %d not in [0, %d)", j, type[j], c->ntypes);
+  if ((r = plain_setup(c, xs, ys, zs, nullptr, nullptr, n, false))) return r;
+  hipStream_t s = c->stream;
+  std::vector<int> ints(2 + n);
+  ints[0] = 0; ints[1] = 0;
+  for (int j = 0; j < n; ++j) ints[2 + j] = type[j];
+  Tile t{0, 0, 1, kWave};
+  if ((r = upload(c, c->s_int, ints.data(), ints.size(), s))) return r;
+  if ((r = upload(c, c->s_tile, &t, 1, s))) return r;
+  HIPCHK(c, c->s_e.ensure(4));
+  const bool grad = gx != nullptr;
+  if (grad) HIPCHK(c, c->s_g.ensure(3 * (size_t)n));
+  InterpLaunch L{c->g, c->s_lig.p, c->s_latom.p, nullptr};
+  HIPCHK(c, launch_interp_coords(L, c->s_tile.p, c->s_int.p, 1, c->s_xyz.p, c->s_int.p + 1,
+                                 c->s_int.p + 2, c->s_e.p, 4, grad ? c->s_g.p : nullptr, 0, s));
+  double h = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->s_e.p, sizeof h, hipMemcpyDeviceToHost, s));
+  std::vector<double> g;
+  if (grad) {
+    g.resize(3 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->s_g.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (std::isnan(h)) return fail(c, MMO_ERR_OUT_OF_GRID, "an atom is outside the grid");
+  *e = h;
+  if (grad)
+    for (int j = 0; j < n; ++j) { gx[j] = g[3 * j]; gy[j] = g[3 * j + 1]; gz[j] = g[3 * j + 2]; }
+  return MMO_OK;
+}
+
+int mmo_ene_inter_interp(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                         const int32_t* type, int n, double* e) {
+  return plain_interp(c, xs, ys, zs, type, n, e, nullptr, nullptr, nullptr);
+}
+
+int mmo_ene_inter_interp_grad(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                              const int32_t* type, int n, double* e, double* gx, double* gy,
+                              double* gz) {
+  if (!gx || !gy || !gz) return fail(c, MMO_ERR_BAD_ARG, "gradient outputs are NULL");
+  return plain_interp(c, xs, ys, zs, type, n, e, gx, gy, gz);
+}
+
+static int plain_intra(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                       const double* q, const int32_t* anum, const int32_t* dists, int n, double* e,
+                       double* gx, double* gy, double* gz) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xs || !ys || !zs || !q || !anum || !dists || n <= 0 || n > 65535 || !e)
+    return fail(c, MMO_ERR_BAD_ARG, "bad ligand arguments (n=%d)", n);
+  if ((r = plain_setup(c, xs, ys, zs, q, anum, n, true))) return r;
+  std::vector<uint32_t> pij = intra_pairs(dists, n);
+  std::vector<double4> ppar;
+  if ((r = pair_params(c, pij, q, anum, ppar))) return r;
+  LigInfo li{};
+  li.n_atoms = n;
+  li.n_pairs = (int)pij.size();
+  hipStream_t s = c->stream;
+  if ((r = upload(c, c->s_lig, &li, 1, s))) return r;
+  if ((r = upload(c, c->s_pij, pij.data(), pij.size(), s))) return r;
+  if ((r = upload(c, c->s_ppar, ppar.data(), ppar.size(), s))) return r;
+  int ints[2] = {0, 0};   // atom_off, pose_lig
+  if ((r = upload(c, c->s_int, ints, 2, s))) return r;
+  HIPCHK(c, c->s_e.ensure(4));
+  const bool grad = gx != nullptr;
+  if (grad) HIPCHK(c, c->s_g.ensure(3 * (size_t)n));
+  IntraArgs a{};
+  a.ligs = c->s_lig.p; a.pair_ij = c->s_pij.p; a.pair_par = c->s_ppar.p; a.xyz = c->s_xyz.p;
+  a.atom_off = c->s_int.p; a.pose_lig = c->s_int.p + 1; a.npose = 1; a.max_atoms = n;
+  a.e_out = c->s_e.p; a.e_stride = 1; a.grad = grad ? c->s_g.p : nullptr; a.grad_accumulate = 0;
+  HIPCHK(c, launch_intra(a, grad, s));
+  double h = 0.0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->s_e.p, sizeof h, hipMemcpyDeviceToHost, s));
+  std::vector<double> g;
+  if (grad) {
+    g.resize(3 * (size_t)n);
+    HIPCHK(c, hipMemcpyAsync(g.data(), c->s_g.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  *e = h;
+  if (grad)
+    for (int j = 0; j < n; ++j) { gx[j] = g[3 * j]; gy[j] = g[3 * j + 1]; gz[j] = g[3 * j + 2]; }
+  return MMO_OK;
+}
+
+int mmo_ene_intra(mmo_ctx* c, const double* xs, const double* ys, const double* zs, const double* q,
+                  const int32_t* anum, const int32_t* dists, int n, double* e) {
+  return plain_intra(c, xs, ys, zs, q, anum, dists, n, e, nullptr, nullptr, nullptr);
+}
+
+int mmo_ene_intra_grad(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                       const double* q, const int32_t* anum, const int32_t* dists, int n, double* e,
+                       double* gx, double* gy, double* gz) {
+  if (!gx || !gy || !gz) return fail(c, MMO_ERR_BAD_ARG, "gradient outputs are NULL");
+  return plain_intra(c, xs, ys, zs, q, anum, dists, n, e, gx, gy, gz);
+}
+
+// ------------------------------------------------------------------ ligand registry
+int mmo_ligand_add(mmo_ctx* c, const double* xs, const double* ys, const double* zs,
+                   const double* q, const int32_t* anum, const int32_t* dists, int n,
+                   const int32_t* rb_left, const int32_t* rb_right, const int32_t* rg_off,
+                   const int32_t* rg_atoms, int nrb, const int32_t* types, int* lig_id) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xs || !ys || !zs || !q || !anum || !dists || n <= 0 || n > 65535 || nrb < 0 || !lig_id)
+    return fail(c, MMO_ERR_BAD_ARG, "bad ligand arguments (n=%d, nrb=%d)", n, nrb);
+  if (nrb > 0 && (!rb_left || !rb_right || !rg_off || !rg_atoms))
+    return fail(c, MMO_ERR_BAD_ARG, "rotatable bonds given without their arrays");
+  LigHost h;
+  h.n = n;
+  h.xyz.resize(3 * (size_t)n);
+  for (int j = 0; j < n; ++j) {
+    h.xyz[3 * j] = xs[j]; h.xyz[3 * j + 1] = ys[j]; h.xyz[3 * j + 2] = zs[j];
+    double A, B;
+    if ((r = atom_ab(c, anum[j], &A, &B))) return r;
+  }
+  h.q.assign(q, q + n);
+  h.anum.assign(anum, anum + n);
+  h.type.assign(n, 0);
+  if (types) {
+    for (int j = 0; j < n; ++j)
+      if (types[j] < 0) return fail(c, MMO_ERR_BAD_ARG, "atom %d: negative grid type", j);
+    h.type.assign(types, types + n);
+  }
+  h.pair_ij = intra_pairs(dists, n);
+  for (int k = 0; k < nrb; ++k) {
+    if (rb_left[k] < 0 || rb_left[k] >= n || rb_right[k] < 0 || rb_right[k] >= n ||
+        rb_left[k] == rb_right[k])
+      return fail(c, MMO_ERR_BAD_ARG, "rbond %d: bad atoms %d-%d", k, rb_left[k], rb_right[k]);
+    if (rg_off[k] < 0 || rg_off[k + 1] < rg_off[k])
+      return fail(c, MMO_ERR_BAD_ARG, "rbond %d: bad rotation group offsets", k);
+    for (int m = rg_off[k]; m < rg_off[k + 1]; ++m)
+      if (rg_atoms[m] < 0 || rg_atoms[m] >= n || rg_atoms[m] == rb_left[k] ||
+          rg_atoms[m] == rb_right[k])
+        return fail(c, MMO_ERR_BAD_ARG, "rbond %d: rotation group holds atom %d", k, rg_atoms[m]);
+  }
+  if (nrb > 0) {
+    h.rb_left.assign(rb_left, rb_left + nrb);
+    h.rb_right.assign(rb_right, rb_right + nrb);
+    h.rg_off.assign(rg_off, rg_off + nrb + 1);
+    for (int& v : h.rg_off) v -= rg_off[0];
+    h.rg_atoms.assign(rg_atoms + rg_off[0], rg_atoms + rg_off[nrb]);
+  } else {
+    h.rg_off.assign(1, 0);
+  }
+  c->ligs.push_back(std::move(h));
+  c->lig_dirty = true;
+  *lig_id = (int)c->ligs.size() - 1;
+  return MMO_OK;
+}
+
+int mmo_ligand_count(const mmo_ctx* c) { return c ? (int)c->ligs.size() : 0; }
+int mmo_ligand_natoms(const mmo_ctx* c, int id) {
+  if (!c || id < 0 || id >= (int)c->ligs.size()) return MMO_ERR_BAD_ARG;
+  return c->ligs[id].n;
+}
+
+// ------------------------------------------------------------------ batches
+int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch** out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!pose_lig || npose <= 0 || !out) return fail(c, MMO_ERR_BAD_ARG, "bad batch arguments");
+  *out = nullptr;
+  mmo_batch* b = new mmo_batch();
+  b->ctx = c;
+  b->npose = npose;
+  b->pose_lig.assign(pose_lig, pose_lig + npose);
+  b->atom_off.resize(npose + 1);
+  b->dof_off.resize(npose + 1);
+  int na = 0, nd = 0;
+  for (int p = 0; p < npose; ++p) {
+    const int l = pose_lig[p];
+    if (l < 0 || l >= (int)c->ligs.size()) {
+      delete b;
+      return fail(c, MMO_ERR_BAD_ARG, "pose %d: unknown ligand %d", p, l);
+    }
+    b->atom_off[p] = na;
+    b->dof_off[p] = nd;
+    na += c->ligs[l].n;
+    nd += 6 + (int)c->ligs[l].rb_left.size();
+    b->max_atoms = std::max(b->max_atoms, c->ligs[l].n);
+  }
+  b->atom_off[npose] = na;
+  b->dof_off[npose] = nd;
+  b->natoms = na;
+  b->ndof = nd;
+  plan_tiles(b->pose_lig, &c->ligs, nullptr, c->inter_k, b->perm, b->tiles);
+  b->ntiles = (int)b->tiles.size();
+  hipStream_t s = c->stream;
+  if ((r = upload(c, b->d_pose_lig, b->pose_lig.data(), npose, s)) ||
+      (r = upload(c, b->d_atom_off, b->atom_off.data(), npose + 1, s)) ||
+      (r = upload(c, b->d_dof_off, b->dof_off.data(), npose + 1, s)) ||
+      (r = upload(c, b->d_perm, b->perm.data(), npose, s)) ||
+      (r = upload(c, b->d_tiles, b->tiles.data(), b->tiles.size(), s))) {
+    delete b;
+    return r;
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete b;
+    return fail(c, MMO_ERR_DEVICE, "batch upload: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return MMO_OK;
+}
+
+void mmo_batch_destroy(mmo_batch* b) {
+  if (!b) return;
+  if (getpid() == b->ctx->pid) {
+    (void)hipSetDevice(b->ctx->device);
+    b->d_pose_lig.release(); b->d_atom_off.release(); b->d_dof_off.release(); b->d_perm.release();
+    b->d_tiles.release(); b->d_xyz.release(); b->d_e.release(); b->d_g.release(); b->d_part.release();
+  }
+  delete b;
+}
+
+int mmo_batch_natoms(const mmo_batch* b) { return b ? b->natoms : MMO_ERR_BAD_ARG; }
+
+int mmo_batch_atom_offsets(const mmo_batch* b, int32_t* off) {
+  if (!b || !off) return MMO_ERR_BAD_ARG;
+  std::copy(b->atom_off.begin(), b->atom_off.end(), off);
+  return MMO_OK;
+}
+
+int mmo_batch_dof_offsets(const mmo_batch* b, int32_t* off) {
+  if (!b || !off) return MMO_ERR_BAD_ARG;
+  std::copy(b->dof_off.begin(), b->dof_off.end(), off);
+  return MMO_OK;
+}
+
+static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out, double* grad,
+                       hipStream_t s) {
+  mmo_ctx* c = b->ctx;
+  const int inter = flags & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL | MMO_F_INTER_INTERP);
+  if (inter && (inter & (inter - 1)))
+    return fail(c, MMO_ERR_BAD_ARG, "at most one E_inter form per call");
+  const bool want_grad = (flags & MMO_F_GRAD) != 0;
+  if (!xyz || !e_out || (want_grad && !grad)) return fail(c, MMO_ERR_BAD_ARG, "NULL buffer");
+  if ((inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) && !c->has_rec)
+    return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if ((inter & MMO_F_INTER_INTERP) && !c->has_grid) return fail(c, MMO_ERR_NO_GRID, "no grids set");
+  int r = sync_ligands(c);
+  if (r) return r;
+  if (inter & MMO_F_INTER_INTERP) {
+    for (const LigHost& h : c->ligs)
+      for (int t : h.type)
+        if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
+  }
+  // gradient buffer starts at zero; every producer adds
+  if (want_grad) HIPCHK(c, launch_fill(grad, 3 * (size_t)b->natoms, 0.0, s));
+  HIPCHK(c, launch_fill(e_out, 4 * (size_t)b->npose, 0.0, s));
+  if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
+    const int mode = (inter & MMO_F_INTER_SHIFTED) ? MMO_SHIFTED : MMO_GLOBAL;
+    InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    // few tiles: split the receptor over block rows so that the chip fills up
+    const int blocks = (b->ntiles + L.waves - 1) / L.waves;
+    int nslice = 1;
+    while (blocks * nslice < 512 && c->nrec / (nslice * 2) >= 128) nslice *= 2;
+    L.rslice = (c->nrec + nslice - 1) / nslice;
+    L.rtile = std::min(L.rtile, L.rslice);
+    nslice = (c->nrec + L.rslice - 1) / L.rslice;
+    if (nslice > 1) HIPCHK(c, b->d_part.ensure((size_t)nslice * b->npose * 4));
+    const bool comp = (flags & MMO_F_COMPONENTS) != 0;
+    timer_begin(c, "inter", s);
+    if (!want_grad || comp)
+      HIPCHK(c, launch_inter_coords(L, b->d_tiles.p, b->d_perm.p, b->ntiles, b->npose, xyz,
+                                    b->d_atom_off.p, e_out, 4, comp, nullptr, 1, b->d_part.p, s));
+    if (want_grad)
+      HIPCHK(c, launch_inter_coords(L, b->d_tiles.p, b->d_perm.p, b->ntiles, b->npose, xyz,
+                                    b->d_atom_off.p, e_out, 4, false, grad, comp ? 0 : 1,
+                                    b->d_part.p, s));
+    timer_end(c, "inter", s);
+  }
+  if (inter & MMO_F_INTER_INTERP) {
+    InterpLaunch L{c->g, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p};
+    timer_begin(c, "interp", s);
+    HIPCHK(c, launch_interp_coords(L, b->d_tiles.p, b->d_perm.p, b->ntiles, xyz, b->d_atom_off.p,
+                                   nullptr, e_out, 4, want_grad ? grad : nullptr, 1, s));
+    timer_end(c, "interp", s);
+  }
+  if (flags & MMO_F_INTRA) {
+    IntraArgs a{};
+    a.ligs = c->d_ligs.p; a.pair_ij = c->d_pair_ij.p; a.pair_par = c->d_pair_par.p; a.xyz = xyz;
+    a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
+    a.max_atoms = b->max_atoms; a.e_out = e_out + 1; a.e_stride = 4;
+    a.grad = want_grad ? grad : nullptr; a.grad_accumulate = 1;
+    timer_begin(c, "intra", s);
+    HIPCHK(c, launch_intra(a, want_grad, s));
+    timer_end(c, "intra", s);
+  }
+  return MMO_OK;
+}
+
+int mmo_batch_score_dev(mmo_batch* b, int flags, const double* xyz, double* e_out, double* grad,
+                        void* stream) {
+  if (!b) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL batch");
+  int r = check_ctx(b->ctx);
+  if (r) return r;
+  return batch_score(b, flags, xyz, e_out, grad, (hipStream_t)stream);
+}
+
+int mmo_batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out, double* grad) {
+  if (!b) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL batch");
+  mmo_ctx* c = b->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!xyz || !e_out) return fail(c, MMO_ERR_BAD_ARG, "NULL buffer");
+  hipStream_t s = c->stream;
+  HIPCHK(c, b->d_xyz.ensure(3 * (size_t)b->natoms));
+  HIPCHK(c, b->d_e.ensure(4 * (size_t)b->npose));
+  const bool g = (flags & MMO_F_GRAD) != 0;
+  if (g) {
+    if (!grad) return fail(c, MMO_ERR_BAD_ARG, "NULL gradient buffer");
+    HIPCHK(c, b->d_g.ensure(3 * (size_t)b->natoms));
+  }
+  HIPCHK(c, hipMemcpyAsync(b->d_xyz.p, xyz, 3 * (size_t)b->natoms * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  r = batch_score(b, flags, b->d_xyz.p, b->d_e.p, g ? b->d_g.p : nullptr, s);
+  if (r) return r;
+  HIPCHK(c, hipMemcpyAsync(e_out, b->d_e.p, 4 * (size_t)b->npose * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  if (g)
+    HIPCHK(c, hipMemcpyAsync(grad, b->d_g.p, 3 * (size_t)b->natoms * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return MMO_OK;
+}
+
+int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double* xyz_out,
+                             double* e_out, double* gdof, double* radius_out, void* stream) {
+  if (!b) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL batch");
+  mmo_ctx* c = b->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!dofs) return fail(c, MMO_ERR_BAD_ARG, "NULL dofs");
+  const bool g = (flags & MMO_F_GRAD) != 0;
+  if (g && !gdof) return fail(c, MMO_ERR_BAD_ARG, "NULL dof gradient");
+  if ((r = sync_ligands(c))) return r;
+  hipStream_t s = (hipStream_t)stream;
+  double* xyz = xyz_out;
+  if (!xyz) {
+    HIPCHK(c, b->d_xyz.ensure(3 * (size_t)b->natoms));
+    xyz = b->d_xyz.p;
+  }
+  DofArgs a{};
+  a.ligs = c->d_ligs.p; a.lig_xyz = c->d_lig_xyz.p; a.rb_left = c->d_rb_left.p;
+  a.rb_right = c->d_rb_right.p; a.rg_off = c->d_rg_off.p; a.rg_atoms = c->d_rg_atoms.p;
+  a.pose_lig = b->d_pose_lig.p; a.atom_off = b->d_atom_off.p; a.dof_off = b->d_dof_off.p;
+  a.npose = b->npose; a.max_atoms = b->max_atoms; a.dofs = dofs; a.xyz_out = xyz;
+  a.radius_out = radius_out;
+  HIPCHK(c, launch_apply_config(a, s));
+  const int score_flags = flags & ~MMO_F_COMPONENTS;
+  if (score_flags & ~MMO_F_GRAD) {
+    if (!e_out) return fail(c, MMO_ERR_BAD_ARG, "NULL energy buffer");
+    double* gx = nullptr;
+    if (g) {
+      HIPCHK(c, b->d_g.ensure(3 * (size_t)b->natoms));
+      gx = b->d_g.p;
+    }
+    if ((r = batch_score(b, score_flags, xyz, e_out, gx, s))) return r;
+    if (g) {
+      a.grad_xyz = gx;
+      a.gdof = gdof;
+      HIPCHK(c, launch_dof_grad(a, s));
+    }
+  }
+  return MMO_OK;
+}
+
+// ------------------------------------------------------------------ rigid scan
+int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
+                 const double* trans, int ntrans, double* e_out, void* stream) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (lig_id < 0 || lig_id >= (int)c->ligs.size())
+    return fail(c, MMO_ERR_BAD_ARG, "unknown ligand %d", lig_id);
+  if (!rots || !trans || nrot <= 0 || ntrans <= 0 || !e_out)
+    return fail(c, MMO_ERR_BAD_ARG, "bad scan arguments");
+  if ((long long)nrot * ntrans > 2147483647LL)
+    return fail(c, MMO_ERR_BAD_ARG, "too many poses (%d x %d)", nrot, ntrans);
+  if ((mode == MMO_SHIFTED || mode == MMO_GLOBAL) && !c->has_rec)
+    return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (mode == MMO_INTERP && !c->has_grid) return fail(c, MMO_ERR_NO_GRID, "no grids set");
+  if (mode != MMO_SHIFTED && mode != MMO_GLOBAL && mode != MMO_INTERP)
+    return fail(c, MMO_ERR_BAD_ARG, "bad mode %d", mode);
+  if ((r = sync_ligands(c))) return r;
+  if (mode == MMO_INTERP)
+    for (int t : c->ligs[lig_id].type)
+      if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p + 9 * (size_t)nrot, trans, 3 * (size_t)ntrans * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  const double* d_rot = c->s_scan.p;
+  const double* d_trans = c->s_scan.p + 9 * (size_t)nrot;
+  if (mode == MMO_INTERP) {
+    InterpLaunch L{c->g, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p};
+    timer_begin(c, "interp", s);
+    HIPCHK(c, launch_interp_scan(L, lig_id, d_rot, nrot, d_trans, ntrans, e_out, s));
+    timer_end(c, "interp", s);
+  } else {
+    InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    timer_begin(c, "inter", s);
+    HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, ntrans, e_out, s));
+    timer_end(c, "inter", s);
+  }
+  // the host arrays were copied asynchronously from pageable memory; wait so that the
+  // caller may reuse them
+  HIPCHK(c, hipStreamSynchronize(s));
+  return MMO_OK;
+}
+
+double mmo_last_kernel_ms(mmo_ctx* c, const char* kernel) {
+  if (!c || !kernel) return -1.0;
+  auto it = c->last_ms.find(kernel);
+  if (it == c->last_ms.end()) return -1.0;
+  if (it->second == -2.0f) {
+    auto& e = c->ev[kernel];
+    if (hipEventSynchronize(e.second) != hipSuccess) return -1.0;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, e.first, e.second) != hipSuccess) return -1.0;
+    it->second = ms;
+  }
+  return it->second;
+}
+
+int mmo_fixed_point_bits(const mmo_ctx* c) { return c ? c->fp_bits : MMO_ERR_BAD_ARG; }
+
+}  // extern "C"

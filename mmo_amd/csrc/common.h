@@ -1,0 +1,91 @@
+// common.h -- device data layout and helpers shared by the gfx950 kernels.
+//
+// HBM layout (built by api.hip):
+//   receptor  RecPos[P]  {x, y, z fixed point (int32), q (float bits)}   16 B
+//             RecAB[P]   {A = sqrt(D) x^6, B = sqrt(D) x^3}               8 B
+//   ligands   LigInfo[nlig], LigAtom[sum n] {ke*q, A, B, grid type} (FP64),
+//             LigXYZ[sum n] centred conformer (FP64), intra pair list per ligand
+//             (packed i | j << 16) with FP64 pair products.
+//   poses     FP64 (x, y, z) rows, pose-major, atom_off[p] = first row of pose p.
+//
+// UFF pair parameters factorise: x_ij = sqrt(x_i x_j), D_ij = sqrt(D_i D_j)
+// (src/UFF.ml:62-71), so D_ij (x_ij^12/r^12 - 2 x_ij^6/r^6) = A_i A_j u^6 - 2 B_i B_j u^3
+// with u = 1/r^2, A = sqrt(D) x^6, B = sqrt(D) x^3.  A lane owning ligand atom j can
+// therefore accumulate receptor-side sums  Sq = sum w q_i / r, S12 = sum w A_i u^6,
+// S6 = sum w B_i u^3  and apply j's parameters once per chunk:
+//   E_j = ke q_j Sq + A_j S12 - 2 B_j S6.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace mmo {
+
+constexpr int kWave = 64;
+constexpr double kElecWeight = 332.0637 / 4.0;   // src/UFF.ml:45
+constexpr float kCutoff2 = 144.0f;                // src/mol.ml:836
+constexpr float kMinR2 = 1.0e-4f;                 // Math.non_zero_dist 0.01 A, src/math.ml:58-62
+
+struct RecPos { int x, y, z; float q; };
+struct RecAB { float a, b; };
+
+struct LigInfo {
+  int n_atoms;
+  int atom_base;   // into LigAtom / LigXYZ
+  int pair_base;   // into the intra pair list
+  int n_pairs;
+  int nrb;         // rotatable bonds
+  int rb_base;     // into rb_left/rb_right/rg_off (rg_off has nrb+1 entries at rb_base + lig)
+  int rg_base;     // first rg_atoms entry of this ligand
+  int dof_n;       // 6 + nrb
+};
+
+struct LigAtom { double keq, a, b; int type; int anum; };
+
+// fixed-point frame: ix = rint((x - o) * 2^k), saturated to +-lim
+struct Frame {
+  double ox, oy, oz;
+  double scale;      // 2^k
+  float inv_s2;      // 2^-2k
+  float inv_s;       // 2^-k
+  int lim;           // 2^30 - 1
+};
+
+// A pose tile: npose poses of ligand `lig` starting at pose0 (poses of one ligand are
+// contiguous after planning); each pose is handled by `split` lanes.
+struct Tile { int lig, pose0, npose, split; };
+
+__device__ __forceinline__ int to_fixed(double v, double o, double s, int lim, int* sat) {
+  double t = (v - o) * s;
+  const double dl = (double)lim;
+  if (!(t >= -dl && t <= dl)) {   // also catches NaN
+    *sat = 1;
+    t = (t != t) ? 0.0 : fmin(fmax(t, -dl), dl);
+  }
+  return __double2int_rn(t);
+}
+
+// wave-wide sums over groups of `split` consecutive lanes (split a power of two)
+__device__ __forceinline__ double group_sum(double v, int split) {
+  for (int off = 1; off < split; off <<= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ int group_or(int v, int split) {
+  for (int off = 1; off < split; off <<= 1) v |= __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 1; off < kWave; off <<= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// t = clamp(1 - r2/144, 0, 1): the cutoff test r2 < 144 and the shift
+// S(r) = (1 - (r/12)^2)^2 (src/FF.ml:17-20) in one clamped FMA.
+__device__ __forceinline__ float shift_t(float r2c) {
+  return __builtin_amdgcn_fmed3f(fmaf(r2c, -1.0f / 144.0f, 1.0f), 0.0f, 1.0f);
+}
+
+}  // namespace mmo

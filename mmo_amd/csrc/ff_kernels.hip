@@ -1,0 +1,553 @@
+// ff_kernels.hip -- gfx950 kernels for the UFF nonbonded scoring path.
+//
+//   k_inter   ligand-receptor LJ 12-6 + Coulomb, direct sum over the receptor
+//             (MODE 1: shifted, src/mol.ml:822-849 / :928-960; MODE 0: global,
+//             src/mol.ml:796-818).  The receptor is staged through LDS in tiles and
+//             read by broadcast; ligand atoms of a wavefront's pose tile sit in VGPRs
+//             (K per lane) as 32-bit fixed point, so x_rec - x_lig is exact in integer
+//             arithmetic.  Pair math is FP32, chunks of kChunk receptor atoms are
+//             folded into FP64 per-pose sums.
+//   k_intra   intra-ligand nonbonded energy, src/mol.ml:881-903, FP64, one wavefront
+//             per pose over the precomputed list of pairs with topo distance >= 3.
+//   k_interp  trilinear interpolation in per-type FP32 grids, src/G3D.ml:97-157 and
+//             src/mol.ml:1012-1020, FP64 weights.
+// Every kernel has an adjoint (GRAD) variant giving dE/dxyz of the ligand atoms.
+#include "common.h"
+#include "kernels.h"
+
+namespace mmo {
+
+constexpr int kChunk = 64;   // receptor atoms per FP32 partial sum
+
+// ------------------------------------------------------------------ pose sources
+struct CoordSrc {
+  const double* xyz;      // rows (x, y, z), pose-major
+  const int* atom_off;    // first row of each pose
+  const Tile* tiles;
+  const int* perm;        // tile slot -> pose index (poses grouped by ligand)
+  __device__ Tile tile(int t) const { return tiles[t]; }
+  __device__ int pose_id(int slot) const { return perm[slot]; }
+  __device__ void pos(int pose, int /*pslot*/, int j, const LigInfo&, const LigAtom*,
+                      const double* /*lig_xyz*/, double& x, double& y, double& z) const {
+    const double* r = xyz + 3 * (size_t)(atom_off[pose] + j);
+    x = r[0]; y = r[1]; z = r[2];
+  }
+  __device__ int row(int pose, int j) const { return atom_off[pose] + j; }
+};
+
+// rigid scan: pose = rot * ntrans + trans, x = R c + T (src/lds.ml:1081-1094)
+struct ScanSrc {
+  const double* rots;     // nrot x 9
+  const double* trans;    // ntrans x 3
+  int ntrans, tiles_per_rot, lig;
+  __device__ Tile tile(int t) const {
+    const int r = t / tiles_per_rot, tb = t - r * tiles_per_rot;
+    Tile o;
+    o.lig = lig;
+    o.pose0 = r * ntrans + tb * kWave;
+    o.npose = min(kWave, ntrans - tb * kWave);
+    o.split = 1;
+    return o;
+  }
+  __device__ int pose_id(int slot) const { return slot; }
+  __device__ void pos(int pose, int, int j, const LigInfo& li, const LigAtom*,
+                      const double* lig_xyz, double& x, double& y, double& z) const {
+    const int r = pose / ntrans, t = pose - r * ntrans;
+    const double* R = rots + 9 * r;
+    const double* T = trans + 3 * t;
+    const double* c = lig_xyz + 3 * (size_t)(li.atom_base + j);
+    // Rot.rotate (src/rot.ml:97-100) then translate
+    x = R[0] * c[0] + R[1] * c[1] + R[2] * c[2] + T[0];
+    y = R[3] * c[0] + R[4] * c[1] + R[5] * c[2] + T[1];
+    z = R[6] * c[0] + R[7] * c[1] + R[8] * c[2] + T[2];
+  }
+  __device__ int row(int, int) const { return 0; }
+};
+
+template <class Src>
+struct InterArgs {
+  Src src;
+  int ntiles;
+  Frame fr;
+  const RecPos* rpos;
+  const RecAB* rab;
+  int nrec, rtile;
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  const double* lig_xyz;
+  double* e_out;      // e_out[pose * e_stride + 0] = E_inter
+  int e_stride;
+  double* e_comp;     // nullable: [pose*e_stride + 2] elec, [+3] vdW (same array as e_out)
+  double* grad;       // GRAD: rows (x, y, z)
+  int write_e;        // GRAD kernel: also write E
+  int rslice;         // receptor atoms per gridDim.y slice
+  int npose_total;    // for partial outputs when gridDim.y > 1
+};
+
+// --------------------------------------------------------------- pair arithmetic
+template <int MODE>
+__device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, int lx, int ly,
+                                            int lz, float inv_s2, float& sq, float& s12,
+                                            float& s6) {
+  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
+  const float r2c = fmaxf(r2, kMinR2);
+  const float ir = __builtin_amdgcn_rsqf(r2c);
+  const float u = ir * ir;
+  const float u3 = u * u * u;
+  const float u6 = u3 * u3;
+  if (MODE == MMO_SHIFTED) {
+    const float t = shift_t(r2c);
+    const float w = t * t;
+    sq = fmaf(w * p.q, ir, sq);
+    s12 = fmaf(w * ab.a, u6, s12);
+    s6 = fmaf(w * ab.b, u3, s6);
+  } else {
+    sq = fmaf(p.q, ir, sq);
+    s12 = fmaf(ab.a, u6, s12);
+    s6 = fmaf(ab.b, u3, s6);
+  }
+}
+
+// energy + dE/dr2 contribution; gradient accumulated as coef * (x_rec - x_lig) in
+// fixed-point units (caller scales by -2 * 2^-k).
+template <int MODE>
+__device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, int lx, int ly,
+                                          int lz, float inv_s2, float c1, float aj, float bj,
+                                          float& e, float& gx, float& gy, float& gz) {
+  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
+  const float r2c = fmaxf(r2, kMinR2);
+  const float ir = __builtin_amdgcn_rsqf(r2c);
+  const float u = ir * ir;
+  const float u3 = u * u * u;
+  const float u6 = u3 * u3;
+  const float eq = p.q * ir, e12 = ab.a * u6, e6 = ab.b * u3;
+  // F = ke qi qj / r + Aij u^6 - 2 Bij u^3 ;  dF/dr2 = -u (ke qi qj/(2r) + 6 Aij u^6 - 6 Bij u^3)
+  const float F = fmaf(c1, eq, fmaf(aj, e12, -2.0f * bj * e6));
+  const float G = fmaf(0.5f * c1, eq, 6.0f * fmaf(aj, e12, -bj * e6));
+  float w, coef;
+  if (MODE == MMO_SHIFTED) {
+    const float t = shift_t(r2c);
+    w = t * t;
+    coef = fmaf(t * F, -1.0f / 72.0f, -(w * u) * G);   // dw/dr2 = -t/72
+  } else {
+    w = 1.0f;
+    coef = -u * G;
+  }
+  coef = (r2 >= kMinR2) ? coef : 0.0f;   // clamped distance: flat
+  e = fmaf(w, F, e);
+  gx = fmaf(coef, dx, gx);
+  gy = fmaf(coef, dy, gy);
+  gz = fmaf(coef, dz, gz);
+}
+
+// ------------------------------------------------------------------ inter kernel
+template <int MODE, int K, bool GRAD, class Src>
+__global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
+  extern __shared__ __align__(16) char smem[];
+  RecPos* s_pos = reinterpret_cast<RecPos*>(smem);
+  RecAB* s_ab = reinterpret_cast<RecAB*>(smem + (size_t)a.rtile * sizeof(RecPos));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nw = blockDim.x / kWave;
+  const int tidx = blockIdx.x * nw + (threadIdx.x / kWave);
+  const bool tile_ok = tidx < a.ntiles;
+  Tile t;
+  if (tile_ok) t = a.src.tile(tidx);
+  else { t.lig = 0; t.pose0 = 0; t.npose = 0; t.split = kWave; }
+  const int split = t.split;
+  const int pslot = lane / split, aslot = lane - pslot * split;
+  const bool pose_ok = tile_ok && pslot < t.npose;
+  const int pose = pose_ok ? a.src.pose_id(t.pose0 + pslot) : 0;
+  const LigInfo li = a.ligs[t.lig];
+  const int per_lane = (li.n_atoms + split - 1) / split;
+  const int nblk = (per_lane + K - 1) / K;
+  const Frame fr = a.fr;
+  double e_el = 0.0, e_vd = 0.0;
+  int sat = 0;
+  const int r_begin = blockIdx.y * a.rslice;
+  const int r_end = min(a.nrec, r_begin + a.rslice);
+  const bool split_rec = gridDim.y > 1;
+
+  for (int rb = r_begin; rb < r_end; rb += a.rtile) {
+    const int nt = min(a.rtile, r_end - rb);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+      s_pos[i] = a.rpos[rb + i];
+      s_ab[i] = a.rab[rb + i];
+    }
+    __syncthreads();
+    if (!tile_ok) continue;
+    for (int blk = 0; blk < nblk; ++blk) {
+      int lx[K], ly[K], lz[K];
+      double cq[K], ca[K], cb[K];
+      float c1[K], fa[K], fb[K];
+      int row[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int j = aslot + split * (blk * K + k);
+        if (pose_ok && j < li.n_atoms) {
+          double x, y, z;
+          a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
+          lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
+          ly[k] = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
+          lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+          const LigAtom la = a.latoms[li.atom_base + j];
+          cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
+          row[k] = GRAD ? a.src.row(pose, j) : 0;
+        } else {
+          lx[k] = ly[k] = lz[k] = 0;
+          cq[k] = ca[k] = cb[k] = 0.0;
+          row[k] = -1;
+        }
+        c1[k] = (float)cq[k]; fa[k] = (float)ca[k]; fb[k] = (float)cb[k];
+      }
+      if (!GRAD) {
+        for (int c0 = 0; c0 < nt; c0 += kChunk) {
+          const int c1e = min(nt, c0 + kChunk);
+          float sq[K], s12[K], s6[K];
+#pragma unroll
+          for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
+#pragma unroll 2
+          for (int i = c0; i < c1e; ++i) {
+            const RecPos p = s_pos[i];
+            const RecAB ab = s_ab[i];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              pair_energy<MODE>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, sq[k], s12[k], s6[k]);
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            e_el = fma((double)sq[k], cq[k], e_el);
+            e_vd = fma((double)s12[k], ca[k], e_vd);
+            e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
+          }
+        }
+      } else {
+        double gx[K], gy[K], gz[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) gx[k] = gy[k] = gz[k] = 0.0;
+        for (int c0 = 0; c0 < nt; c0 += kChunk) {
+          const int c1e = min(nt, c0 + kChunk);
+          float fe[K], fx[K], fy[K], fz[K];
+#pragma unroll
+          for (int k = 0; k < K; ++k) fe[k] = fx[k] = fy[k] = fz[k] = 0.0f;
+          for (int i = c0; i < c1e; ++i) {
+            const RecPos p = s_pos[i];
+            const RecAB ab = s_ab[i];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              pair_grad<MODE>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, c1[k], fa[k], fb[k], fe[k],
+                              fx[k], fy[k], fz[k]);
+          }
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            e_vd += (double)fe[k];
+            gx[k] += (double)fx[k]; gy[k] += (double)fy[k]; gz[k] += (double)fz[k];
+          }
+        }
+        // dE/dx_lig = sum coef * d(r2)/dx_lig = -2 * sum coef * (x_rec - x_lig)
+        const double gs = -2.0 * (double)fr.inv_s;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (row[k] >= 0) {
+            double* g = a.grad + 3 * (size_t)row[k];
+            if (split_rec) {   // grad pre-zeroed, slices add concurrently
+              unsafeAtomicAdd(&g[0], gs * gx[k]);
+              unsafeAtomicAdd(&g[1], gs * gy[k]);
+              unsafeAtomicAdd(&g[2], gs * gz[k]);
+            } else if (rb == r_begin) {
+              g[0] = gs * gx[k]; g[1] = gs * gy[k]; g[2] = gs * gz[k];
+            } else {
+              g[0] += gs * gx[k]; g[1] += gs * gy[k]; g[2] += gs * gz[k];
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!tile_ok) return;
+  e_el = group_sum(e_el, split);
+  e_vd = group_sum(e_vd, split);
+  sat = group_or(sat, split);
+  if (pose_ok && aslot == 0) {
+    double e = e_el + e_vd;
+    // out of the fixed-point range: exact for the shifted form (saturated atoms stay
+    // beyond the cutoff), undefined for the global form
+    if (MODE == MMO_GLOBAL && sat) e = __builtin_nan("");
+    // receptor slices write partial sums to slice-major buffers reduced afterwards
+    const size_t o = (size_t)blockIdx.y * a.npose_total * a.e_stride + (size_t)pose * a.e_stride;
+    if (!GRAD || a.write_e) a.e_out[o] = e;
+    if (!GRAD && a.e_comp) {
+      a.e_comp[o + 2] = e_el;
+      a.e_comp[o + 3] = e_vd;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ intra kernel
+// one wavefront per pose; pairs of the ligand's list are spread over lanes
+template <bool GRAD>
+__global__ void __launch_bounds__(64) k_intra(IntraArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  double* sx = reinterpret_cast<double*>(smem);
+  const int pose = blockIdx.x;
+  if (pose >= a.npose) return;
+  const int lane = threadIdx.x;
+  const LigInfo li = a.ligs[a.pose_lig[pose]];
+  const int L = li.n_atoms;
+  double* sy = sx + L;
+  double* sz = sy + L;
+  double* gx = sz + L;
+  double* gy = gx + L;
+  double* gz = gy + L;
+  const int base = a.atom_off[pose];
+  for (int j = lane; j < L; j += kWave) {
+    const double* r = a.xyz + 3 * (size_t)(base + j);
+    sx[j] = r[0]; sy[j] = r[1]; sz[j] = r[2];
+    if (GRAD) { gx[j] = 0.0; gy[j] = 0.0; gz[j] = 0.0; }
+  }
+  __syncthreads();
+  double e = 0.0;
+  for (int p = lane; p < li.n_pairs; p += kWave) {
+    const uint32_t ij = a.pair_ij[li.pair_base + p];
+    const int i = ij & 0xffff, j = ij >> 16;
+    const double4 pp = a.pair_par[li.pair_base + p];   // ke qi qj, Ai Aj, 2 Bi Bj
+    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = rr < 0.01 ? 0.01 : rr;   // Math.non_zero_dist
+    const double ir = 1.0 / r;
+    const double u = ir * ir;
+    const double u3 = u * u * u;
+    const double u6 = u3 * u3;
+    e += pp.x * ir + pp.y * u6 - pp.z * u3;
+    if (GRAD && rr >= 0.01) {
+      // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
+      const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
+      atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
+      atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
+    }
+  }
+  e = wave_sum(e);
+  if (lane == 0) a.e_out[(size_t)pose * a.e_stride] = e;
+  if (GRAD) {
+    __syncthreads();
+    for (int j = lane; j < L; j += kWave) {
+      double* g = a.grad + 3 * (size_t)(base + j);
+      if (a.grad_accumulate) { g[0] += gx[j]; g[1] += gy[j]; g[2] += gz[j]; }
+      else { g[0] = gx[j]; g[1] = gy[j]; g[2] = gz[j]; }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- interp kernel
+template <class Src>
+struct InterpArgs {
+  Src src;
+  int ntiles;
+  GridDev g;
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  const double* lig_xyz;
+  const int* type_override;   // nullable: per-atom types of the plain form
+  double* e_out;
+  int e_stride;
+  double* grad;
+  int grad_accumulate;
+};
+
+template <bool GRAD>
+__device__ __forceinline__ double trilin(const GridDev& g, const float* arr, double px, double py,
+                                         double pz, bool& ok, double* dg) {
+  // src/G3D.ml:97-157: i0 = int_of_float (x * one_div_step) (truncation)
+  const double fx = px * g.ods, fy = py * g.ods, fz = pz * g.ods;
+  const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+  if (!(px >= 0.0 && py >= 0.0 && pz >= 0.0 && i0 + 1 < g.nx && j0 + 1 < g.ny &&
+        k0 + 1 < g.nz)) {
+    ok = false;
+    return 0.0;
+  }
+  const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
+  const size_t b = i0 + j0 * nx + k0 * nxy;
+  const double A = arr[b], B = arr[b + 1], C = arr[b + 1 + nx], D = arr[b + nx];
+  const double E = arr[b + nxy], F = arr[b + 1 + nxy], G = arr[b + 1 + nx + nxy],
+               H = arr[b + nx + nxy];
+  const double wlx = (px - i0 * g.step) * g.ods, wly = (py - j0 * g.step) * g.ods,
+               wlz = (pz - k0 * g.step) * g.ods;
+  const double whx = 1.0 - wlx, why = 1.0 - wly, whz = 1.0 - wlz;
+  if (GRAD) {
+    dg[0] = g.ods * ((B - A) * why * whz + (C - D) * wly * whz + (F - E) * why * wlz + (G - H) * wly * wlz);
+    dg[1] = g.ods * ((D - A) * whx * whz + (C - B) * wlx * whz + (H - E) * whx * wlz + (G - F) * wlx * wlz);
+    dg[2] = g.ods * ((E - A) * whx * why + (F - B) * wlx * why + (G - C) * wlx * wly + (H - D) * whx * wly);
+  }
+  // corner order a..h, src/G3D.ml:144-151
+  return A * (whx * why * whz) + B * (wlx * why * whz) + C * (wlx * wly * whz) +
+         D * (whx * wly * whz) + E * (whx * why * wlz) + F * (wlx * why * wlz) +
+         G * (wlx * wly * wlz) + H * (whx * wly * wlz);
+}
+
+template <bool GRAD, class Src>
+__global__ void __launch_bounds__(256) k_interp(InterpArgs<Src> a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int tidx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (tidx >= a.ntiles) return;
+  const Tile t = a.src.tile(tidx);
+  const int split = t.split;
+  const int pslot = lane / split, aslot = lane - pslot * split;
+  const bool pose_ok = pslot < t.npose;
+  const int pose = pose_ok ? a.src.pose_id(t.pose0 + pslot) : 0;
+  const LigInfo li = a.ligs[t.lig];
+  double e = 0.0;
+  int bad = 0;
+  if (pose_ok) {
+    for (int j = aslot; j < li.n_atoms; j += split) {
+      double x, y, z;
+      a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
+      const int ty = a.type_override ? a.type_override[j] : a.latoms[li.atom_base + j].type;
+      const float* arr = a.g.data + (size_t)ty * a.g.nvox;
+      bool ok = true;
+      double dg[3] = {0.0, 0.0, 0.0};
+      e += trilin<GRAD>(a.g, arr, x - a.g.ox, y - a.g.oy, z - a.g.oz, ok, dg);
+      if (!ok) bad = 1;
+      if (GRAD) {
+        double* gr = a.grad + 3 * (size_t)a.src.row(pose, j);
+        if (!ok) dg[0] = dg[1] = dg[2] = __builtin_nan("");
+        if (a.grad_accumulate) { gr[0] += dg[0]; gr[1] += dg[1]; gr[2] += dg[2]; }
+        else { gr[0] = dg[0]; gr[1] = dg[1]; gr[2] = dg[2]; }
+      }
+    }
+  }
+  e = group_sum(e, split);
+  bad = group_or(bad, split);
+  if (pose_ok && aslot == 0) a.e_out[(size_t)pose * a.e_stride] = bad ? __builtin_nan("") : e;
+}
+
+__global__ void k_fill(double* p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------------------- launchers
+template <int MODE, int K, bool GRAD, class Src>
+static hipError_t launch_inter_t(const InterArgs<Src>& a, int waves, hipStream_t s) {
+  const int blocks = (a.ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  const int ny = (a.nrec + a.rslice - 1) / a.rslice;
+  const size_t lds = (size_t)a.rtile * (sizeof(RecPos) + sizeof(RecAB));
+  hipLaunchKernelGGL((k_inter<MODE, K, GRAD, Src>), dim3(blocks, ny), dim3(waves * kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+// sums the gridDim.y partial outputs (slice-major) into out, in slice order
+__global__ void k_reduce_slices(const double* part, int nslice, size_t n, int stride, int ncomp,
+                                double* out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < ncomp; ++c) {
+      if (ncomp == 1 && c > 0) break;
+      const int cc = (ncomp == 1) ? 0 : (c == 0 ? 0 : c + 1);   // 0, 2, 3
+      double acc = 0.0;
+      for (int y = 0; y < nslice; ++y) acc += part[(size_t)y * n * stride + i * stride + cc];
+      out[i * stride + cc] = acc;
+    }
+  }
+}
+
+template <class Src>
+static hipError_t launch_inter_any(int mode, int k, bool grad, const InterArgs<Src>& a, int waves,
+                                   hipStream_t s) {
+#define MMO_K(KK)                                                                       \
+  if (k == KK) {                                                                        \
+    if (grad) return mode == MMO_SHIFTED ? launch_inter_t<MMO_SHIFTED, KK, true>(a, waves, s) \
+                                         : launch_inter_t<MMO_GLOBAL, KK, true>(a, waves, s);  \
+    return mode == MMO_SHIFTED ? launch_inter_t<MMO_SHIFTED, KK, false>(a, waves, s)           \
+                               : launch_inter_t<MMO_GLOBAL, KK, false>(a, waves, s);           \
+  }
+  MMO_K(1) MMO_K(2) MMO_K(4) MMO_K(8)
+#undef MMO_K
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const int* perm,
+                               int ntiles, int npose, const double* xyz, const int* atom_off,
+                               double* e_out, int e_stride, bool comp, double* grad, int write_e,
+                               double* part, hipStream_t s) {
+  InterArgs<CoordSrc> a;
+  a.src.xyz = xyz; a.src.atom_off = atom_off; a.src.tiles = tiles; a.src.perm = perm;
+  a.ntiles = ntiles; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab; a.nrec = L.nrec;
+  a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms; a.lig_xyz = L.lig_xyz;
+  a.e_stride = e_stride; a.grad = grad; a.write_e = write_e;
+  a.rslice = L.rslice > 0 ? L.rslice : L.nrec;
+  a.npose_total = npose;
+  const int ny = (L.nrec + a.rslice - 1) / a.rslice;
+  a.e_out = ny > 1 ? part : e_out;
+  a.e_comp = comp ? a.e_out : nullptr;
+  hipError_t err = launch_inter_any(L.mode, L.k, grad != nullptr, a, L.waves, s);
+  if (err != hipSuccess || ny == 1 || (grad && !write_e)) return err;
+  const int blocks = std::min(4096, (npose + 255) / 256);
+  hipLaunchKernelGGL(k_reduce_slices, dim3(blocks), dim3(256), 0, s, part, ny, (size_t)npose,
+                     e_stride, comp ? 3 : 1, e_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, int nrot,
+                             const double* trans, int ntrans, double* e_out, hipStream_t s) {
+  InterArgs<ScanSrc> a;
+  a.src.rots = rots; a.src.trans = trans; a.src.ntrans = ntrans;
+  a.src.tiles_per_rot = (ntrans + kWave - 1) / kWave; a.src.lig = lig;
+  a.ntiles = nrot * a.src.tiles_per_rot; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab;
+  a.nrec = L.nrec; a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms;
+  a.lig_xyz = L.lig_xyz; a.e_out = e_out; a.e_stride = 1; a.e_comp = nullptr; a.grad = nullptr;
+  a.write_e = 1; a.rslice = L.nrec; a.npose_total = nrot * ntrans;
+  return launch_inter_any(L.mode, L.k, false, a, L.waves, s);
+}
+
+hipError_t launch_intra(const IntraArgs& a, bool grad, hipStream_t s) {
+  if (a.npose == 0) return hipSuccess;
+  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
+  if (grad) hipLaunchKernelGGL(k_intra<true>, dim3(a.npose), dim3(kWave), lds, s, a);
+  else hipLaunchKernelGGL(k_intra<false>, dim3(a.npose), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const int* perm,
+                                int ntiles, const double* xyz, const int* atom_off,
+                                const int* type_override, double* e_out, int e_stride,
+                                double* grad, int grad_accumulate, hipStream_t s) {
+  InterpArgs<CoordSrc> a;
+  a.src.xyz = xyz; a.src.atom_off = atom_off; a.src.tiles = tiles; a.src.perm = perm;
+  a.ntiles = ntiles; a.g = L.g; a.ligs = L.ligs; a.latoms = L.latoms; a.lig_xyz = L.lig_xyz;
+  a.type_override = type_override; a.e_out = e_out; a.e_stride = e_stride; a.grad = grad;
+  a.grad_accumulate = grad_accumulate;
+  const int waves = 4;
+  const int blocks = (ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  if (grad) hipLaunchKernelGGL((k_interp<true, CoordSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  else hipLaunchKernelGGL((k_interp<false, CoordSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots, int nrot,
+                              const double* trans, int ntrans, double* e_out, hipStream_t s) {
+  InterpArgs<ScanSrc> a;
+  a.src.rots = rots; a.src.trans = trans; a.src.ntrans = ntrans;
+  a.src.tiles_per_rot = (ntrans + kWave - 1) / kWave; a.src.lig = lig;
+  a.ntiles = nrot * a.src.tiles_per_rot; a.g = L.g; a.ligs = L.ligs; a.latoms = L.latoms;
+  a.lig_xyz = L.lig_xyz; a.type_override = nullptr; a.e_out = e_out; a.e_stride = 1;
+  a.grad = nullptr; a.grad_accumulate = 0;
+  const int waves = 4;
+  const int blocks = (a.ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_interp<false, ScanSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace mmo

@@ -1,0 +1,90 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (implemented in *.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.h"
+#include "mmo_ff.h"
+
+namespace mmo {
+
+struct InterLaunch {
+  int mode;            // MMO_SHIFTED / MMO_GLOBAL
+  int k;               // ligand atoms per lane (1, 2, 4, 8)
+  int waves;           // wavefronts per block
+  int rtile;           // receptor atoms per LDS tile
+  int rslice;          // receptor atoms per block row (gridDim.y > 1 splits the receptor)
+  Frame fr;
+  const RecPos* rpos;
+  const RecAB* rab;
+  int nrec;
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  const double* lig_xyz;
+};
+
+struct GridDev {
+  const float* data;   // ntypes * nvox
+  int nx, ny, nz;
+  size_t nvox;
+  double step, ods;    // step and 1/step
+  double ox, oy, oz;   // position of node (0,0,0)
+};
+
+struct InterpLaunch {
+  GridDev g;
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  const double* lig_xyz;
+};
+
+struct IntraArgs {
+  const LigInfo* ligs;
+  const uint32_t* pair_ij;
+  const double4* pair_par;
+  const double* xyz;
+  const int* atom_off;
+  const int* pose_lig;
+  int npose;
+  int max_atoms;
+  double* e_out;
+  int e_stride;
+  double* grad;
+  int grad_accumulate;
+};
+
+struct DofArgs {
+  const LigInfo* ligs;
+  const double* lig_xyz;        // centred conformers
+  const int* rb_left;           // per ligand at rb_base
+  const int* rb_right;
+  const int* rg_off;            // per ligand at rb_base + lig (nrb+1 entries)
+  const int* rg_atoms;
+  const int* pose_lig;
+  const int* atom_off;
+  const int* dof_off;
+  int npose, max_atoms;
+  const double* dofs;
+  double* xyz_out;
+  double* radius_out;           // nullable
+  const double* grad_xyz;       // dE/dxyz (for the adjoint)
+  double* gdof;                 // dE/dDOF
+};
+
+// part: scratch of (nslices * npose * e_stride) doubles when the receptor is split
+hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const int* perm,
+                               int ntiles, int npose, const double* xyz, const int* atom_off,
+                               double* e_out, int e_stride, bool comp, double* grad, int write_e,
+                               double* part, hipStream_t s);
+hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, int nrot,
+                             const double* trans, int ntrans, double* e_out, hipStream_t s);
+hipError_t launch_intra(const IntraArgs& a, bool grad, hipStream_t s);
+hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const int* perm,
+                                int ntiles, const double* xyz, const int* atom_off,
+                                const int* type_override, double* e_out, int e_stride,
+                                double* grad, int grad_accumulate, hipStream_t s);
+hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots, int nrot,
+                              const double* trans, int ntrans, double* e_out, hipStream_t s);
+hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
+hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
+
+}  // namespace mmo

@@ -1,0 +1,236 @@
+// pose_kernels.hip -- DOFs -> coordinates (Optim.apply_config, src/optim.ml:64-80) and
+// its adjoint (dE/dxyz -> dE/dDOF), one wavefront per pose, FP64 throughout.
+//
+// Forward: the registered centred conformer c0; for k = 0..R-1 the atoms of rotation
+// group k turn by tau_k about the axis right_k - left_k through right_k
+// (Mol.rotate_bond, src/mol.ml:610-631), in rbond order, each about the current
+// positions; then x = r_xyz(alpha, beta, gamma) c + t (Rot.r_xyz src/rot.ml:52-66,
+// Mol.rotate_then_translate_copy src/mol.ml:669-672).
+//
+// Adjoint: reverse pass over k.  Neither `left` nor `right` of rbond k moves under
+// rotation k (the group excludes both ends, src/pqrs.ml:243-250), so c^k is recovered
+// from c^{k+1} by the inverse rotation and no intermediate conformers are stored.
+// For y_i = p + R(a, tau)(x_i - p), with upstream adjoint h_i:
+//   dE/dtau  = sum_i h_i . (a x (y_i - p))
+//   x_i_bar  = R^T h_i
+//   p_bar    = sum_i (h_i - R^T h_i)                       (pivot = right)
+//   a_bar    = sum_i (z_i x h_i) sin + h_i (a.z_i)(1-cos) + z_i (a.h_i)(1-cos),  z_i = x_i - p
+//   v_bar    = (I - a a^T) a_bar / |v|,  v = x_right - x_left
+//   right_bar += v_bar + p_bar,  left_bar -= v_bar.
+#include "common.h"
+#include "kernels.h"
+
+namespace mmo {
+
+struct Rod { double m[9]; };
+
+// Rot.of_axis_angle, src/rot.ml:121-146
+__device__ __forceinline__ Rod rodrigues(double x, double y, double z, double theta) {
+  double s, c;
+  sincos(theta, &s, &c);
+  const double o = 1.0 - c;
+  Rod r;
+  r.m[0] = c + x * x * o;     r.m[1] = x * y * o - z * s; r.m[2] = x * z * o + y * s;
+  r.m[3] = x * y * o + z * s; r.m[4] = c + y * y * o;     r.m[5] = y * z * o - x * s;
+  r.m[6] = x * z * o - y * s; r.m[7] = y * z * o + x * s; r.m[8] = c + z * z * o;
+  return r;
+}
+
+// Rot.r_xyz, src/rot.ml:52-66, and its partial derivatives
+__device__ __forceinline__ void rot_xyz(double al, double be, double ga, double* m, double* da,
+                                        double* db, double* dg) {
+  double as, ac, bs, bc, gs, gc;
+  sincos(al, &as, &ac);
+  sincos(be, &bs, &bc);
+  sincos(ga, &gs, &gc);
+  m[0] = bc * gc;  m[1] = gc * as * bs - ac * gs;  m[2] = as * gs + ac * gc * bs;
+  m[3] = bc * gs;  m[4] = ac * gc + as * bs * gs;  m[5] = ac * bs * gs - gc * as;
+  m[6] = -bs;      m[7] = bc * as;                 m[8] = ac * bc;
+  if (da) {
+    da[0] = 0.0;      da[1] = gc * ac * bs + as * gs; da[2] = ac * gs - as * gc * bs;
+    da[3] = 0.0;      da[4] = -as * gc + ac * bs * gs; da[5] = -as * bs * gs - gc * ac;
+    da[6] = 0.0;      da[7] = bc * ac;               da[8] = -as * bc;
+    db[0] = -bs * gc; db[1] = gc * as * bc;          db[2] = ac * gc * bc;
+    db[3] = -bs * gs; db[4] = as * bc * gs;          db[5] = ac * bc * gs;
+    db[6] = -bc;      db[7] = -bs * as;              db[8] = -ac * bs;
+    dg[0] = -bc * gs; dg[1] = -gs * as * bs - ac * gc; dg[2] = as * gc - ac * gs * bs;
+    dg[3] = bc * gc;  dg[4] = -ac * gs + as * bs * gc; dg[5] = ac * bs * gc + gs * as;
+    dg[6] = 0.0;      dg[7] = 0.0;                   dg[8] = 0.0;
+  }
+}
+
+// torsions in rbond order on the conformer held in LDS (all 64 lanes participate)
+__device__ void forward_torsions(const DofArgs& a, const LigInfo& li, const double* d,
+                                 double* sx, double* sy, double* sz) {
+  const int lane = threadIdx.x;
+  for (int k = 0; k < li.nrb; ++k) {
+    __syncthreads();
+    const int left = a.rb_left[li.rb_base + k], right = a.rb_right[li.rb_base + k];
+    const double px = sx[right], py = sy[right], pz = sz[right];
+    double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
+    const double mag = sqrt(vx * vx + vy * vy + vz * vz);   // V3.normalize
+    const Rod R = rodrigues(vx / mag, vy / mag, vz / mag, d[6 + k]);
+    const int g0 = a.rg_off[li.rg_base + k], g1 = a.rg_off[li.rg_base + k + 1];
+    __syncthreads();
+    for (int m = g0 + lane; m < g1; m += kWave) {
+      const int i = a.rg_atoms[m];
+      const double zx = sx[i] - px, zy = sy[i] - py, zz = sz[i] - pz;
+      sx[i] = (R.m[0] * zx + R.m[1] * zy + R.m[2] * zz) + px;
+      sy[i] = (R.m[3] * zx + R.m[4] * zy + R.m[5] * zz) + py;
+      sz[i] = (R.m[6] * zx + R.m[7] * zy + R.m[8] * zz) + pz;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void load_conformer(const DofArgs& a, const LigInfo& li, double* sx, double* sy,
+                               double* sz) {
+  for (int j = threadIdx.x; j < li.n_atoms; j += kWave) {
+    const double* c = a.lig_xyz + 3 * (size_t)(li.atom_base + j);
+    sx[j] = c[0]; sy[j] = c[1]; sz[j] = c[2];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_apply_config(DofArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const int pose = blockIdx.x;
+  if (pose >= a.npose) return;
+  const LigInfo li = a.ligs[a.pose_lig[pose]];
+  const int L = li.n_atoms, lane = threadIdx.x;
+  double* sx = reinterpret_cast<double*>(smem);
+  double* sy = sx + L;
+  double* sz = sy + L;
+  const double* d = a.dofs + a.dof_off[pose];
+  load_conformer(a, li, sx, sy, sz);
+  forward_torsions(a, li, d, sx, sy, sz);
+  // centre: the registered conformer's (origin) until a torsion updates it to the mean
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  if (li.nrb > 0) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int j = lane; j < L; j += kWave) { s0 += sx[j]; s1 += sy[j]; s2 += sz[j]; }
+    cx = wave_sum(s0) / L; cy = wave_sum(s1) / L; cz = wave_sum(s2) / L;
+  }
+  double rmax = 0.0;   // Mol.radius, src/mol.ml:576-583
+  for (int j = lane; j < L; j += kWave) {
+    const double dx = cx - sx[j], dy = cy - sy[j], dz = cz - sz[j];
+    rmax = fmax(rmax, 0.01 + sqrt(dx * dx + dy * dy + dz * dz));
+  }
+  rmax = wave_max(rmax);
+  if (lane == 0 && a.radius_out) a.radius_out[pose] = rmax;
+  double M[9];
+  rot_xyz(d[3], d[4], d[5], M, nullptr, nullptr, nullptr);
+  const int base = a.atom_off[pose];
+  for (int j = lane; j < L; j += kWave) {
+    double* o = a.xyz_out + 3 * (size_t)(base + j);
+    o[0] = (M[0] * sx[j] + M[1] * sy[j] + M[2] * sz[j]) + d[0];
+    o[1] = (M[3] * sx[j] + M[4] * sy[j] + M[5] * sz[j]) + d[1];
+    o[2] = (M[6] * sx[j] + M[7] * sy[j] + M[8] * sz[j]) + d[2];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const int pose = blockIdx.x;
+  if (pose >= a.npose) return;
+  const LigInfo li = a.ligs[a.pose_lig[pose]];
+  const int L = li.n_atoms, lane = threadIdx.x;
+  double* sx = reinterpret_cast<double*>(smem);
+  double* sy = sx + L;
+  double* sz = sy + L;
+  double* hx = sz + L;
+  double* hy = hx + L;
+  double* hz = hy + L;
+  const double* d = a.dofs + a.dof_off[pose];
+  double* gd = a.gdof + a.dof_off[pose];
+  load_conformer(a, li, sx, sy, sz);
+  forward_torsions(a, li, d, sx, sy, sz);
+  double M[9], dA[9], dB[9], dG[9];
+  rot_xyz(d[3], d[4], d[5], M, dA, dB, dG);
+  const int base = a.atom_off[pose];
+  double tx = 0.0, ty = 0.0, tz = 0.0, S[9];
+  for (int q = 0; q < 9; ++q) S[q] = 0.0;
+  for (int j = lane; j < L; j += kWave) {
+    const double* g = a.grad_xyz + 3 * (size_t)(base + j);
+    const double g0 = g[0], g1 = g[1], g2 = g[2];
+    tx += g0; ty += g1; tz += g2;
+    const double c[3] = {sx[j], sy[j], sz[j]};
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc) S[3 * r + cc] += (r == 0 ? g0 : r == 1 ? g1 : g2) * c[cc];
+    // h = M^T g
+    hx[j] = M[0] * g0 + M[3] * g1 + M[6] * g2;
+    hy[j] = M[1] * g0 + M[4] * g1 + M[7] * g2;
+    hz[j] = M[2] * g0 + M[5] * g1 + M[8] * g2;
+  }
+  tx = wave_sum(tx); ty = wave_sum(ty); tz = wave_sum(tz);
+  double ga = 0.0, gb = 0.0, gg = 0.0;
+  for (int q = 0; q < 9; ++q) {
+    const double sq = wave_sum(S[q]);
+    ga += dA[q] * sq; gb += dB[q] * sq; gg += dG[q] * sq;
+  }
+  if (lane == 0) { gd[0] = tx; gd[1] = ty; gd[2] = tz; gd[3] = ga; gd[4] = gb; gd[5] = gg; }
+  __syncthreads();
+  for (int k = li.nrb - 1; k >= 0; --k) {
+    const int left = a.rb_left[li.rb_base + k], right = a.rb_right[li.rb_base + k];
+    const double px = sx[right], py = sy[right], pz = sz[right];
+    const double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
+    const double mag = sqrt(vx * vx + vy * vy + vz * vz);
+    const double ax = vx / mag, ay = vy / mag, az = vz / mag;
+    const double tau = d[6 + k];
+    const Rod R = rodrigues(ax, ay, az, tau);
+    double sn, cs;
+    sincos(tau, &sn, &cs);
+    const double omc = 1.0 - cs;
+    const int g0 = a.rg_off[li.rg_base + k], g1 = a.rg_off[li.rg_base + k + 1];
+    double tb = 0.0, abx = 0.0, aby = 0.0, abz = 0.0, pbx = 0.0, pby = 0.0, pbz = 0.0;
+    for (int m = g0 + lane; m < g1; m += kWave) {
+      const int i = a.rg_atoms[m];
+      const double yx = sx[i] - px, yy = sy[i] - py, yz = sz[i] - pz;   // rotated offset
+      const double h0 = hx[i], h1 = hy[i], h2 = hz[i];
+      // dE/dtau: h . (a x y)
+      tb += h0 * (ay * yz - az * yy) + h1 * (az * yx - ax * yz) + h2 * (ax * yy - ay * yx);
+      // pre-rotation offset z = R^T y, and R^T h
+      const double zx = R.m[0] * yx + R.m[3] * yy + R.m[6] * yz;
+      const double zy = R.m[1] * yx + R.m[4] * yy + R.m[7] * yz;
+      const double zz = R.m[2] * yx + R.m[5] * yy + R.m[8] * yz;
+      const double rx = R.m[0] * h0 + R.m[3] * h1 + R.m[6] * h2;
+      const double ry = R.m[1] * h0 + R.m[4] * h1 + R.m[7] * h2;
+      const double rz = R.m[2] * h0 + R.m[5] * h1 + R.m[8] * h2;
+      const double az_ = ax * zx + ay * zy + az * zz, ah = ax * h0 + ay * h1 + az * h2;
+      abx += (zy * h2 - zz * h1) * sn + h0 * az_ * omc + zx * ah * omc;
+      aby += (zz * h0 - zx * h2) * sn + h1 * az_ * omc + zy * ah * omc;
+      abz += (zx * h1 - zy * h0) * sn + h2 * az_ * omc + zz * ah * omc;
+      pbx += h0 - rx; pby += h1 - ry; pbz += h2 - rz;
+      hx[i] = rx; hy[i] = ry; hz[i] = rz;
+      sx[i] = zx + px; sy[i] = zy + py; sz[i] = zz + pz;   // back to c^k
+    }
+    tb = wave_sum(tb);
+    abx = wave_sum(abx); aby = wave_sum(aby); abz = wave_sum(abz);
+    pbx = wave_sum(pbx); pby = wave_sum(pby); pbz = wave_sum(pbz);
+    __syncthreads();
+    if (lane == 0) {
+      gd[6 + k] = tb;
+      const double adot = ax * abx + ay * aby + az * abz;
+      const double vbx = (abx - ax * adot) / mag, vby = (aby - ay * adot) / mag,
+                   vbz = (abz - az * adot) / mag;
+      hx[right] += vbx + pbx; hy[right] += vby + pby; hz[right] += vbz + pbz;
+      hx[left] -= vbx; hy[left] -= vby; hz[left] -= vbz;
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_apply_config(const DofArgs& a, hipStream_t s) {
+  if (a.npose == 0) return hipSuccess;
+  const size_t lds = (size_t)a.max_atoms * 3 * sizeof(double);
+  hipLaunchKernelGGL(k_apply_config, dim3(a.npose), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s) {
+  if (a.npose == 0) return hipSuccess;
+  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
+  hipLaunchKernelGGL(k_dof_grad, dim3(a.npose), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mmo

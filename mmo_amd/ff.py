@@ -1,0 +1,269 @@
+"""Python handle on libmmo_ff.so: contexts, ligand registry, batches, rigid scans.
+
+``Context`` owns one GPU context (``mmo_ctx``): the receptor, the UFF table, grids and
+registered ligands.  Everything computes on the GPU through the C ABI; there is no
+CPU path.  Molecules are passed as ``mmo_amd.mol.Mol`` objects or plain dicts with the
+``Mol.t`` fields (``xs, ys, zs, q, anum``; ``dists`` for E_intra).
+"""
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, MMOError
+
+GLOBAL, SHIFTED, INTERP = _lib.GLOBAL, _lib.SHIFTED, _lib.INTERP
+F_INTER_SHIFTED, F_INTER_GLOBAL, F_INTER_INTERP = (_lib.F_INTER_SHIFTED, _lib.F_INTER_GLOBAL,
+                                                   _lib.F_INTER_INTERP)
+F_INTRA, F_GRAD, F_COMPONENTS = _lib.F_INTRA, _lib.F_GRAD, _lib.F_COMPONENTS
+_MODE_FLAG = {SHIFTED: F_INTER_SHIFTED, GLOBAL: F_INTER_GLOBAL, INTERP: F_INTER_INTERP}
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a):
+    return ct.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _field(m, *names):
+    for n in names:
+        if isinstance(m, dict):
+            if n in m:
+                return m[n]
+        elif hasattr(m, n):
+            return getattr(m, n)
+    raise KeyError("molecule has none of %s" % (names,))
+
+
+def mol_arrays(m):
+    """(xs, ys, zs, q, anum) of a Mol or dict as contiguous arrays."""
+    return (_f64(_field(m, "xs")), _f64(_field(m, "ys")), _f64(_field(m, "zs")),
+            _f64(_field(m, "q", "q_a")), _i32(_field(m, "anum", "elt_a")))
+
+
+def dists_colmajor(m):
+    """Topological distances as stored in Mol.t.dists_a: flat[i + j*n] = row i, column j
+    of the .pqrs matrix (src/pqrs.ml:225-240)."""
+    d = _field(m, "dists", "dists_a")
+    n = len(_field(m, "xs"))
+    d = np.asarray(d, dtype=np.int32)
+    if d.ndim == 2:
+        return _i32(d.T.reshape(-1))
+    return _i32(d.reshape(-1))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ct.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class Context:
+    """One GPU scoring context (receptor, UFF table, grids, ligand registry)."""
+
+    def __init__(self, device=0):
+        h = ct.c_void_p()
+        self._check(lib.mmo_ctx_create(int(device), ct.byref(h)), None)
+        self.h = h
+        self.device = device
+        self.nrec = 0
+        self.grid_shape = None
+        self._ligs = []
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.mmo_ctx_destroy(h)
+            self.h = None
+
+    def _check(self, rc, h=-1):
+        if rc != 0:
+            msg = lib.mmo_last_error(self.h if h == -1 else h)
+            raise MMOError(rc, msg.decode() if msg else "")
+        return rc
+
+    # ------------------------------------------------------------------ setup
+    def set_uff_table(self, table):
+        """table: {anum: (x_i, D_i)}; replaces UFF.anum_xi_Di (src/UFF.ml:30-42)."""
+        an = _i32(list(table))
+        x = _f64([table[a][0] for a in table])
+        d = _f64([table[a][1] for a in table])
+        self._check(lib.mmo_set_uff_table(self.h, _ptr(an), _ptr(x), _ptr(d), len(an)))
+
+    def set_receptor(self, prot, origin=None):
+        xs, ys, zs, q, an = mol_arrays(prot)
+        o = _f64(origin) if origin is not None else None
+        self._check(lib.mmo_set_receptor(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), _ptr(an),
+                                         len(xs), _ptr(o)))
+        self.nrec = len(xs)
+
+    def set_grids(self, grids, step, origin):
+        """grids: (ntypes, nz, ny, nx) FP32 (x fastest, src/grid.ml:98-99)."""
+        g = np.ascontiguousarray(grids, dtype=np.float32)
+        nt, nz, ny, nx = g.shape
+        ptrs = (ct.c_void_p * nt)(*[g[t].ctypes.data for t in range(nt)])
+        o = _f64(origin)
+        self._check(lib.mmo_set_grids(self.h, ptrs, nt, float(step), nx, ny, nz, _ptr(o)))
+        self.grid_shape = (nt, nz, ny, nx)
+        self._grids_keep = g
+
+    @property
+    def fixed_point_bits(self):
+        return lib.mmo_fixed_point_bits(self.h)
+
+    # ------------------------------------------------------------ plain forms
+    def ene_inter(self, lig, mode=SHIFTED, components=False):
+        xs, ys, zs, q, an = mol_arrays(lig)
+        e, el, vd = ct.c_double(), ct.c_double(), ct.c_double()
+        self._check(lib.mmo_ene_inter(self.h, int(mode), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q),
+                                      _ptr(an), len(xs), ct.byref(e), ct.byref(el), ct.byref(vd)))
+        return (e.value, el.value, vd.value) if components else e.value
+
+    def ene_inter_grad(self, lig, mode=SHIFTED):
+        xs, ys, zs, q, an = mol_arrays(lig)
+        n = len(xs)
+        g = np.zeros((3, n))
+        e = ct.c_double()
+        self._check(lib.mmo_ene_inter_grad(self.h, int(mode), _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q),
+                                           _ptr(an), n, ct.byref(e), _ptr(g[0]), _ptr(g[1]),
+                                           _ptr(g[2])))
+        return e.value, g.T.copy()
+
+    def ene_intra(self, lig):
+        xs, ys, zs, q, an = mol_arrays(lig)
+        d = dists_colmajor(lig)
+        e = ct.c_double()
+        self._check(lib.mmo_ene_intra(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), _ptr(an),
+                                      _ptr(d), len(xs), ct.byref(e)))
+        return e.value
+
+    def ene_intra_grad(self, lig):
+        xs, ys, zs, q, an = mol_arrays(lig)
+        d = dists_colmajor(lig)
+        n = len(xs)
+        g = np.zeros((3, n))
+        e = ct.c_double()
+        self._check(lib.mmo_ene_intra_grad(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), _ptr(an),
+                                           _ptr(d), n, ct.byref(e), _ptr(g[0]), _ptr(g[1]),
+                                           _ptr(g[2])))
+        return e.value, g.T.copy()
+
+    def ene_inter_interp(self, lig, types=None):
+        xs, ys, zs = (_f64(_field(lig, k)) for k in ("xs", "ys", "zs"))
+        t = _i32(types if types is not None else _field(lig, "types_id", "t_a"))
+        e = ct.c_double()
+        self._check(lib.mmo_ene_inter_interp(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(t), len(xs),
+                                             ct.byref(e)))
+        return e.value
+
+    def ene_inter_interp_grad(self, lig, types=None):
+        xs, ys, zs = (_f64(_field(lig, k)) for k in ("xs", "ys", "zs"))
+        t = _i32(types if types is not None else _field(lig, "types_id", "t_a"))
+        n = len(xs)
+        g = np.zeros((3, n))
+        e = ct.c_double()
+        self._check(lib.mmo_ene_inter_interp_grad(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(t), n,
+                                                  ct.byref(e), _ptr(g[0]), _ptr(g[1]), _ptr(g[2])))
+        return e.value, g.T.copy()
+
+    # --------------------------------------------------------- ligand registry
+    def add_ligand(self, lig, types=None):
+        """Registers a centred ligand (topology, rotatable bonds, grid types); returns its id."""
+        xs, ys, zs, q, an = mol_arrays(lig)
+        d = dists_colmajor(lig)
+        n = len(xs)
+        rbl = _i32(_field(lig, "rb_left")) if _has(lig, "rb_left") else _i32([])
+        rbr = _i32(_field(lig, "rb_right")) if _has(lig, "rb_right") else _i32([])
+        rgo = _i32(_field(lig, "rg_off")) if _has(lig, "rg_off") else _i32([0])
+        rga = _i32(_field(lig, "rg_atoms")) if _has(lig, "rg_atoms") else _i32([])
+        if types is None and _has(lig, "types_id"):
+            types = _field(lig, "types_id")
+        t = _i32(types) if types is not None else None
+        out = ct.c_int()
+        self._check(lib.mmo_ligand_add(self.h, _ptr(xs), _ptr(ys), _ptr(zs), _ptr(q), _ptr(an),
+                                       _ptr(d), n, _ptr(rbl), _ptr(rbr), _ptr(rgo),
+                                       _ptr(rga if len(rga) else None), len(rbl), _ptr(t),
+                                       ct.byref(out)))
+        self._ligs.append(n)
+        return out.value
+
+    def ligand_natoms(self, lig_id):
+        return self._ligs[lig_id]
+
+    def batch(self, pose_lig):
+        return Batch(self, pose_lig)
+
+    # --------------------------------------------------------------- scans
+    def scan(self, lig_id, mode, rots, trans, out=None, stream=None):
+        """Rigid scan on the GPU; returns a torch FP64 tensor (nrot * ntrans) on the device."""
+        import torch
+        r = _f64(np.asarray(rots).reshape(-1, 9))
+        t = _f64(np.asarray(trans).reshape(-1, 3))
+        n = len(r) * len(t)
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device="cuda:%d" % self.device)
+        assert out.is_cuda and out.dtype == torch.float64 and out.numel() >= n and out.is_contiguous()
+        self._check(lib.mmo_scan_dev(self.h, int(lig_id), int(mode), _ptr(r), len(r), _ptr(t), len(t),
+                                     ct.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+        return out
+
+    def last_kernel_ms(self, name):
+        return lib.mmo_last_kernel_ms(self.h, name.encode())
+
+
+def _has(m, name):
+    return (name in m) if isinstance(m, dict) else hasattr(m, name)
+
+
+class Batch:
+    """A fixed list of poses (pose p is a conformation of ligand pose_lig[p])."""
+
+    def __init__(self, ctx, pose_lig):
+        self.ctx = ctx
+        pl = _i32(pose_lig)
+        h = ct.c_void_p()
+        ctx._check(lib.mmo_batch_create(ctx.h, _ptr(pl), len(pl), ct.byref(h)))
+        self.h = h
+        self.npose = len(pl)
+        self.natoms = lib.mmo_batch_natoms(h)
+        self.atom_off = np.zeros(self.npose + 1, dtype=np.int32)
+        lib.mmo_batch_atom_offsets(h, _ptr(self.atom_off))
+        self.dof_off = np.zeros(self.npose + 1, dtype=np.int32)
+        lib.mmo_batch_dof_offsets(h, _ptr(self.dof_off))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.mmo_batch_destroy(h)
+            self.h = None
+
+    def score(self, xyz, flags):
+        """Host form: xyz (natoms, 3) FP64 -> (e (npose, 4), grad (natoms, 3) or None)."""
+        x = _f64(xyz).reshape(-1)
+        assert x.size == 3 * self.natoms
+        e = np.zeros((self.npose, 4))
+        g = np.zeros((self.natoms, 3)) if flags & F_GRAD else None
+        self.ctx._check(lib.mmo_batch_score(self.h, int(flags), _ptr(x), _ptr(e), _ptr(g)))
+        return e, g
+
+    def score_dev(self, xyz, e_out, grad=None, flags=F_INTER_SHIFTED, stream=None):
+        """Device form on torch CUDA FP64 tensors (enqueued on `stream`)."""
+        self.ctx._check(lib.mmo_batch_score_dev(
+            self.h, int(flags), ct.c_void_p(xyz.data_ptr()), ct.c_void_p(e_out.data_ptr()),
+            ct.c_void_p(grad.data_ptr()) if grad is not None else None, _stream_ptr(stream)))
+
+    def score_dofs_dev(self, dofs, e_out, xyz_out=None, gdof=None, radius=None,
+                       flags=F_INTER_SHIFTED | F_INTRA, stream=None):
+        """Optim.apply_config form on torch CUDA FP64 tensors."""
+        def p(t):
+            return ct.c_void_p(t.data_ptr()) if t is not None else None
+        self.ctx._check(lib.mmo_batch_score_dofs_dev(self.h, int(flags), p(dofs), p(xyz_out),
+                                                     p(e_out), p(gdof), p(radius),
+                                                     _stream_ptr(stream)))

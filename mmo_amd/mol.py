@@ -1,0 +1,293 @@
+"""Mol.t and the reference's scoring entry points, backed by the GPU library.
+
+Mirrors the OCaml interface of MMO's scoring path so that a caller of
+``Mol.ene_*`` / ``UFF`` / ``FF`` finds the same names and argument meaning:
+
+    ene_inter_UFF_global_brute prot lig            src/mol.ml:796
+    ene_inter_UFF_shifted_brute prot lig           src/mol.ml:822
+    ene_inter_UFF_shifted_bst prot_bst prot lig    src/mol.ml:958
+    ene_inter_UFF_shifted_bst_components ...       src/mol.ml:928
+    ene_inter_UFF_interp grid ff_comps lig         src/mol.ml:1012
+    ene_intra_UFFNB_brute lig                      src/mol.ml:881
+    index_protein_atoms prot                       src/lds.ml:376-381 (-> receptor handle)
+
+The receptor is uploaded once per receptor object (the reference also builds its BST
+once, src/lds.ml:1919) and scoring runs on the GPU; nothing here computes energies
+on the CPU.
+"""
+import weakref
+
+import numpy as np
+
+from . import ff
+
+# UFF nonbond parameters (x_i [A], D_i [kcal/mol]) by atomic number, src/UFF.ml:30-42
+ANUM_XI_DI = {0: (0.0, 0.0), 1: (2.886, 0.044), 6: (3.851, 0.105), 7: (3.660, 0.069),
+              8: (3.500, 0.060), 9: (3.364, 0.050), 12: (3.021, 0.111), 15: (4.147, 0.305),
+              16: (4.035, 0.274), 17: (3.947, 0.227), 35: (4.189, 0.251), 53: (4.500, 0.339)}
+ELEC_WEIGHT_PROTEIN = 332.0637 / 4.0      # src/UFF.ml:45
+SYM2ANUM = {"C": 6, "H": 1, "N": 7, "O": 8, "P": 15, "S": 16, "F": 9, "Cl": 17, "Br": 35,
+            "I": 53, "Mg": 12}            # src/ptable.ml:88-101
+ANUM2SYM = {v: k for k, v in SYM2ANUM.items()}
+
+
+class Mol:
+    """SoA molecule, the fields of Mol.t (src/mol.ml:17-35).
+
+    dists_a is the flat topological distance matrix, dists_a[i + j*n] (src/mol.ml:151);
+    rbonds_a is a list of (left, right) with right on the moving side, rgroups_a the
+    moving atoms of each rbond without `right` (src/pqrs.ml:204-250); t_a are grid types.
+    """
+
+    def __init__(self, name, xs, ys, zs, q_a, elt_a, r_a=None, dists_a=None, rbonds_a=(),
+                 rgroups_a=(), t_a=None):
+        self.name = name
+        self.xs = np.asarray(xs, dtype=np.float64)
+        self.ys = np.asarray(ys, dtype=np.float64)
+        self.zs = np.asarray(zs, dtype=np.float64)
+        self.q_a = np.asarray(q_a, dtype=np.float64)
+        self.elt_a = np.asarray(elt_a, dtype=np.int32)
+        n = len(self.xs)
+        self.r_a = np.zeros(n) if r_a is None else np.asarray(r_a, dtype=np.float64)
+        self.dists_a = None if dists_a is None else np.asarray(dists_a, dtype=np.int32).reshape(-1)
+        self.rbonds_a = [tuple(b) for b in rbonds_a]
+        self.rgroups_a = [np.asarray(g, dtype=np.int32) for g in rgroups_a]
+        self.t_a = None if t_a is None else np.asarray(t_a, dtype=np.int32)
+        self.update_center()
+
+    # accessors used by ff.mol_arrays / dists_colmajor / add_ligand
+    @property
+    def q(self):
+        return self.q_a
+
+    @property
+    def anum(self):
+        return self.elt_a
+
+    @property
+    def dists(self):
+        return self.dists_a
+
+    @property
+    def rb_left(self):
+        return np.array([b[0] for b in self.rbonds_a], dtype=np.int32)
+
+    @property
+    def rb_right(self):
+        return np.array([b[1] for b in self.rbonds_a], dtype=np.int32)
+
+    @property
+    def rg_off(self):
+        return np.concatenate([[0], np.cumsum([len(g) for g in self.rgroups_a])]).astype(np.int32)
+
+    @property
+    def rg_atoms(self):
+        if not self.rgroups_a:
+            return np.zeros(0, dtype=np.int32)
+        return np.concatenate(self.rgroups_a).astype(np.int32)
+
+    @property
+    def types_id(self):
+        if self.t_a is None:
+            raise KeyError("t_a")
+        return self.t_a
+
+    def num_atoms(self):
+        return len(self.xs)
+
+    def num_rbonds(self):
+        return len(self.rbonds_a)
+
+    def update_center(self):
+        self.center = np.array([self.xs.mean(), self.ys.mean(), self.zs.mean()]) \
+            if len(self.xs) else np.zeros(3)
+
+    def coords(self):
+        return np.stack([self.xs, self.ys, self.zs], axis=1)
+
+    def with_coords(self, xyz):
+        m = Mol(self.name, xyz[:, 0], xyz[:, 1], xyz[:, 2], self.q_a, self.elt_a, self.r_a,
+                self.dists_a, self.rbonds_a, self.rgroups_a, self.t_a)
+        return m
+
+    @classmethod
+    def from_dict(cls, d):
+        """From the records of mmo_amd.synth (dists as an n x n matrix)."""
+        n = len(d["xs"])
+        dists = None
+        if "dists" in d:
+            dm = np.asarray(d["dists"], dtype=np.int32).reshape(n, n)
+            dists = dm.T.reshape(-1)   # flat[i + j*n] = dm[i, j]
+        rb = list(zip(d.get("rb_left", []), d.get("rb_right", [])))
+        off = d.get("rg_off", [0])
+        ga = d.get("rg_atoms", [])
+        groups = [np.asarray(ga[off[k]:off[k + 1]]) for k in range(len(rb))]
+        return cls(d.get("name", "mol"), d["xs"], d["ys"], d["zs"], d["q"], d["anum"],
+                   dists_a=dists, rbonds_a=rb, rgroups_a=groups, t_a=d.get("t_a"))
+
+
+# ------------------------------------------------------------------------- .pqrs I/O
+def _parse_header(line):
+    """src/pqrs.ml:180-192: 'N:R:name' (ligand) or 'N:name' (receptor)."""
+    parts = line.strip().split(":")
+    if len(parts) >= 3:
+        try:
+            return int(parts[0]), int(parts[1]), ":".join(parts[2:])
+        except ValueError:
+            pass
+    return int(parts[0]), -1, ":".join(parts[1:])
+
+
+def _read_one(lines, pos, ligand):
+    n, nrb, name = _parse_header(lines[pos])
+    pos += 1
+    xs, ys, zs, q, r, an = [], [], [], [], [], []
+    for _ in range(n):
+        f = lines[pos].split()
+        pos += 1
+        xs.append(float(f[0])); ys.append(float(f[1])); zs.append(float(f[2]))
+        q.append(float(f[3])); r.append(float(f[4]))
+        if f[5] not in SYM2ANUM:
+            raise ValueError("unsupported element: %s" % f[5])   # src/ptable.ml:103-107
+        an.append(SYM2ANUM[f[5]])
+    if not ligand:
+        return Mol(name, xs, ys, zs, q, an, r_a=r), pos
+    rbonds, groups = [], []
+    for _ in range(max(nrb, 0)):
+        head, bits = lines[pos].split("=", 1)
+        pos += 1
+        beg, end = (int(v) for v in head.split("-"))
+        b = [int(v) for v in bits.split()]
+        if len(b) != n or b[beg] == b[end]:
+            raise ValueError("bad rot-bond line: %s" % lines[pos - 1])
+        # axis from the fixed atom to the moving one (src/pqrs.ml:214-218)
+        left, right = (beg, end) if b[end] else (end, beg)
+        rbonds.append((left, right))
+        groups.append([i for i in range(n) if b[i] and i != right])   # src/pqrs.ml:243-250
+    dm = np.zeros((n, n), dtype=np.int32)
+    for i in range(n):
+        row = [int(v) for v in lines[pos].split()]
+        pos += 1
+        if len(row) != n or row[i] != 0:
+            raise ValueError("bad distance matrix row %d of %s" % (i, name))
+        dm[i] = row
+    return Mol(name, xs, ys, zs, q, an, r_a=r, dists_a=dm.T.reshape(-1), rbonds_a=rbonds,
+               rgroups_a=groups), pos
+
+
+def ligands_of_pqrs_file(fn):
+    """Mol.ligands_of_pqrs_file, src/mol.ml:420-440 (duplicate names are an error)."""
+    lines = [l for l in open(fn).read().splitlines() if l.strip()]
+    out, pos, seen = [], 0, set()
+    while pos < len(lines):
+        m, pos = _read_one(lines, pos, True)
+        if m.name in seen:
+            raise ValueError("duplicate molecule name %s in %s" % (m.name, fn))
+        seen.add(m.name)
+        out.append(m)
+    return out
+
+
+def receptor_of_pqrs_file(fn):
+    """Mol.receptor_of_pqrs_file, src/mol.ml:413-417."""
+    lines = [l for l in open(fn).read().splitlines() if l.strip()]
+    return _read_one(lines, 0, False)[0]
+
+
+def write_pqrs(fn, mols, ligand=True):
+    with open(fn, "w") as out:
+        for m in mols:
+            n = m.num_atoms()
+            if ligand:
+                out.write("%d:%d:%s\n" % (n, m.num_rbonds(), m.name))
+            else:
+                out.write("%d:%s\n" % (n, m.name))
+            for i in range(n):
+                out.write("%.17g %.17g %.17g %.17g %g %s\n" % (m.xs[i], m.ys[i], m.zs[i], m.q_a[i],
+                                                               m.r_a[i], ANUM2SYM[int(m.elt_a[i])]))
+            if ligand:
+                for (left, right), g in zip(m.rbonds_a, m.rgroups_a):
+                    bits = np.zeros(n, dtype=int)
+                    bits[list(g)] = 1
+                    bits[right] = 1
+                    out.write("%d-%d= %s\n" % (left, right, " ".join(str(b) for b in bits)))
+                dm = m.dists_a.reshape(n, n).T
+                for i in range(n):
+                    out.write(" ".join(str(int(v)) for v in dm[i]) + "\n")
+
+
+# ------------------------------------------------------------------- scoring (GPU)
+_CTX = weakref.WeakKeyDictionary()
+
+
+class ReceptorIndex:
+    """Receptor uploaded to the GPU; plays the role of the BST index (src/lds.ml:376-381)."""
+
+    def __init__(self, prot, origin=None, device=0):
+        self.ctx = ff.Context(device)
+        self.ctx.set_receptor(prot, origin)
+        self.prot = prot
+
+
+def index_protein_atoms(prot, origin=None):
+    idx = ReceptorIndex(prot, origin)
+    try:
+        _CTX[prot] = idx
+    except TypeError:
+        pass
+    return idx
+
+
+def _index(prot):
+    try:
+        idx = _CTX.get(prot)
+    except TypeError:
+        idx = None
+    if idx is None:
+        idx = index_protein_atoms(prot)
+    return idx
+
+
+def ene_inter_UFF_global_brute(prot, lig):
+    return _index(prot).ctx.ene_inter(lig, ff.GLOBAL)
+
+
+def ene_inter_UFF_shifted_brute(prot, lig):
+    return _index(prot).ctx.ene_inter(lig, ff.SHIFTED)
+
+
+def ene_inter_UFF_shifted_bst_components(prot_bst, prot, lig):
+    """Returns (sum_elec, sum_vdW) with sum_elec already weighted (src/mol.ml:955)."""
+    _, el, vd = prot_bst.ctx.ene_inter(lig, ff.SHIFTED, components=True)
+    return el, vd
+
+
+def ene_inter_UFF_shifted_bst(prot_bst, prot, lig):
+    return prot_bst.ctx.ene_inter(lig, ff.SHIFTED)
+
+
+def ene_intra_UFFNB_brute(lig, ctx=None):
+    if ctx is None:
+        ctx = _default_ctx()
+    return ctx.ene_intra(lig)
+
+
+class GridFF:
+    """Grid.t + ff_comps uploaded to the GPU (src/grid.ml:11-19, src/G3D.ml:11)."""
+
+    def __init__(self, grids, step, origin, ctx=None):
+        self.ctx = ctx if ctx is not None else ff.Context()
+        self.ctx.set_grids(grids, step, origin)
+
+
+def ene_inter_UFF_interp(grid_ff, lig):
+    return grid_ff.ctx.ene_inter_interp(lig)
+
+
+_DEFAULT = []
+
+
+def _default_ctx():
+    if not _DEFAULT:
+        _DEFAULT.append(ff.Context())
+    return _DEFAULT[0]

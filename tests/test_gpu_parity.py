@@ -1,0 +1,307 @@
+"""GPU parity of the gfx950 kernels against the FP64 oracle, through the C ABI.
+
+Tolerances (tests/parity.py): energies 1e-5 (rel on well-conditioned poses and
+conditioned on all), gradients 1e-4 of the largest component; the interpolated form
+uses identical FP32 grids on both sides and is held to 1e-12."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from mmo_amd import ff, synth
+from mmo_amd._lib import MMOError
+from tests.parity import assert_energy, assert_grad, inter_abs_sum, intra_abs_sum
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sys0():
+    lig, rec = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    return ctx, lig, rec
+
+
+def _place(lig, R, t):
+    xyz = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1) @ np.asarray(R).T + np.asarray(t)
+    return dict(lig, xs=xyz[:, 0].copy(), ys=xyz[:, 1].copy(), zs=xyz[:, 2].copy()), xyz
+
+
+def _poses(n, seed=0, spread=4.75):
+    rng = np.random.default_rng(seed)
+    return synth.random_rotations(rng, n), rng.uniform(-spread, spread, (n, 3))
+
+
+def test_plain_inter_shifted_and_global(sys0):
+    ctx, lig, rec = sys0
+    rots, trans = _poses(12)
+    for mode, shifted in ((ff.SHIFTED, True), (ff.GLOBAL, False)):
+        es, refs, sums = [], [], []
+        for R, t in zip(rots, trans):
+            l, xyz = _place(lig, R, t)
+            print("plain inter mode", mode, flush=True)
+            e, el, vd = ctx.ene_inter(l, mode, components=True)
+            r, rel_, rvd = oracle.ene_inter(rec, l, shifted=shifted, components=True)
+            es.append(e)
+            refs.append(r)
+            sums.append(inter_abs_sum(rec, xyz, l, shifted))
+            assert el + vd == pytest.approx(e, rel=1e-12, abs=1e-9)
+        assert_energy(es, refs, sums, what="inter mode %d" % mode)
+
+
+def test_plain_inter_components(sys0):
+    ctx, lig, rec = sys0
+    l, xyz = _place(lig, np.eye(3), [0.5, -1.0, 2.0])
+    e, el, vd = ctx.ene_inter(l, ff.SHIFTED, components=True)
+    r, rel_, rvd = oracle.ene_inter(rec, l, shifted=True, components=True)
+    s = inter_abs_sum(rec, xyz, l, True)
+    assert abs(el - rel_) / s < 1e-5 and abs(vd - rvd) / s < 1e-5
+
+
+def test_plain_inter_far_away_is_exactly_zero(sys0):
+    ctx, lig, rec = sys0
+    l, _ = _place(lig, np.eye(3), [200.0, 0.0, 0.0])   # beyond the fixed-point range too
+    assert ctx.ene_inter(l, ff.SHIFTED) == 0.0
+    with pytest.raises(MMOError) as e:
+        ctx.ene_inter(l, ff.GLOBAL)
+    assert e.value.code == -8
+
+
+def test_plain_intra(sys0):
+    ctx, lig, rec = sys0
+    e = ctx.ene_intra(lig)
+    xyz = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1)
+    assert_energy(e, oracle.ene_intra(lig), intra_abs_sum(lig, xyz), tol=1e-10, what="intra")
+
+
+def test_unsupported_element_rejected(sys0):
+    ctx, lig, rec = sys0
+    bad = dict(lig, anum=np.where(np.arange(len(lig["xs"])) == 3, 5, lig["anum"]))
+    with pytest.raises(MMOError) as e:
+        ctx.ene_inter(bad, ff.SHIFTED)
+    assert e.value.code == -1
+
+
+def test_translated_frame_gives_same_energies(sys0):
+    """The reference works in a box frame at +36..+80 A (src/lds.ml:20-41): moving
+    receptor and ligand together leaves the GPU energies unchanged to 1e-5."""
+    ctx, lig, rec = sys0
+    off = np.array([55.0, 61.0, 48.0])
+    rec2 = dict(rec, xs=rec["xs"] + off[0], ys=rec["ys"] + off[1], zs=rec["zs"] + off[2])
+    c2 = ff.Context(0)
+    c2.set_receptor(rec2)
+    rots, trans = _poses(4, seed=3)
+    for R, t in zip(rots, trans):
+        l, xyz = _place(lig, R, t)
+        l2, xyz2 = _place(lig, R, t + off)
+        s = inter_abs_sum(rec, xyz, l, True)
+        assert abs(ctx.ene_inter(l) - c2.ene_inter(l2)) / s < 1e-5
+
+
+def _grids(rng, nt=3, n=24, step=0.5, origin=(-6.0, -6.0, -6.0)):
+    return rng.normal(0, 5, (nt, n, n, n)).astype(np.float32), step, np.array(origin)
+
+
+def test_plain_interp_matches_oracle_exactly():
+    rng = np.random.default_rng(7)
+    grids, step, origin = _grids(rng)
+    ctx = ff.Context(0)
+    ctx.set_grids(grids, step, origin)
+    g = oracle.Grid.make(step, (24, 24, 24))
+    for _ in range(5):
+        xyz = rng.uniform(-5.5, 5.0, (30, 3))
+        types = rng.integers(0, 3, 30)
+        l = {"xs": xyz[:, 0], "ys": xyz[:, 1], "zs": xyz[:, 2]}
+        e = ctx.ene_inter_interp(l, types)
+        r = oracle.ene_inter_interp(g, list(grids), xyz - origin, types)
+        assert e == pytest.approx(r, rel=1e-12, abs=1e-12)
+        e2, gr = ctx.ene_inter_interp_grad(l, types)
+        r2, grr = oracle.grad_interp(g, list(grids), xyz - origin, types)
+        assert e2 == pytest.approx(r2, rel=1e-12, abs=1e-12)
+        assert np.allclose(gr, grr, rtol=1e-10, atol=1e-10)
+    out = {"xs": np.array([7.0]), "ys": np.array([0.0]), "zs": np.array([0.0])}
+    with pytest.raises(MMOError) as e:
+        ctx.ene_inter_interp(out, [0])
+    assert e.value.code == -4
+
+
+def test_plain_gradients(sys0):
+    ctx, lig, rec = sys0
+    rots, trans = _poses(3, seed=11)
+    for R, t in zip(rots, trans):
+        l, xyz = _place(lig, R, t)
+        for mode, shifted in ((ff.SHIFTED, True), (ff.GLOBAL, False)):
+            e, g = ctx.ene_inter_grad(l, mode)
+            r, gr = oracle.grad_inter(rec, l, shifted=shifted)
+            assert_energy(e, r, inter_abs_sum(rec, xyz, l, shifted), what="grad-E")
+            assert_grad(g, gr, what="inter grad mode %d" % mode)
+    e, g = ctx.ene_intra_grad(lig)
+    r, gr = oracle.grad_intra(lig)
+    assert e == pytest.approx(r, rel=1e-10)
+    assert_grad(g, gr, tol=1e-10, what="intra grad")
+
+
+# ----------------------------------------------------------------- batches
+@pytest.fixture(scope="module")
+def lib_sys():
+    ligs = synth.library(12, seed=99)
+    _, rec = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    rng = np.random.default_rng(0)
+    grids, step, origin = _grids(rng, nt=4, n=40, step=0.5, origin=(-10.0, -10.0, -10.0))
+    ctx.set_grids(grids, step, origin)
+    ids = []
+    for k, l in enumerate(ligs):
+        l["types_id"] = (np.arange(len(l["xs"])) + k) % 4
+        ids.append(ctx.add_ligand(l))
+    return ctx, ligs, ids, rec, (grids, step, origin)
+
+
+def test_batch_ragged_poses(lib_sys):
+    ctx, ligs, ids, rec, (grids, step, origin) = lib_sys
+    rng = np.random.default_rng(1)
+    # uneven pose counts, interleaved ligands: 1, 3, 70, 5, ... poses
+    counts = [1, 3, 70, 5, 2, 9, 1, 17, 4, 1, 65, 6]
+    pose_lig = np.concatenate([[ids[k]] * c for k, c in enumerate(counts)])
+    rng.shuffle(pose_lig)
+    b = ctx.batch(pose_lig)
+    xyz_all = np.zeros((b.natoms, 3))
+    poses = []
+    for p, l in enumerate(pose_lig):
+        R = synth.random_rotations(rng, 1)[0]
+        t = rng.uniform(-3, 3, 3)
+        lg, xyz = _place(ligs[l], R, t)
+        xyz_all[b.atom_off[p]:b.atom_off[p + 1]] = xyz
+        poses.append((lg, xyz))
+    print("batch shifted+intra npose", len(pose_lig), flush=True)
+    e, _ = b.score(xyz_all, ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_COMPONENTS)
+    sample = rng.choice(len(pose_lig), 40, replace=False)
+    ref = [oracle.ene_inter(rec, poses[p][0]) for p in sample]
+    sums = [inter_abs_sum(rec, poses[p][1], poses[p][0]) for p in sample]
+    assert_energy(e[sample, 0], ref, sums, what="batch inter")
+    assert np.allclose(e[sample, 2] + e[sample, 3], e[sample, 0], rtol=1e-12, atol=1e-9)
+    refi = [oracle.ene_intra(poses[p][0]) for p in sample]
+    sumi = [intra_abs_sum(poses[p][0], poses[p][1]) for p in sample]
+    assert_energy(e[sample, 1], refi, sumi, tol=1e-10, what="batch intra")
+    # interpolated form on the same poses (atoms inside the 19.5 A grid)
+    print("batch interp", flush=True)
+    e2, _ = b.score(xyz_all, ff.F_INTER_INTERP)
+    g = oracle.Grid.make(step, (40, 40, 40))
+    for p in sample[:15]:
+        lg, xyz = poses[p]
+        r = oracle.ene_inter_interp(g, list(grids), xyz - origin, lg["types_id"])
+        if math.isnan(r):
+            assert math.isnan(e2[p, 0])
+        else:
+            assert e2[p, 0] == pytest.approx(r, rel=1e-12, abs=1e-12)
+    # gradients through the batch (inter + intra)
+    print("batch grad", flush=True)
+    e3, gb = b.score(xyz_all, ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD)
+    for p in sample[:6]:
+        lg, xyz = poses[p]
+        _, g1 = oracle.grad_inter(rec, lg)
+        _, g2 = oracle.grad_intra(lg)
+        assert_grad(gb[b.atom_off[p]:b.atom_off[p + 1]], g1 + g2, what="batch grad")
+
+
+# ----------------------------------------------------------------- rigid scan
+def test_rigid_scan_subset_matches_oracle(sys0):
+    import torch
+    ctx, lig, rec = sys0
+    lid = ctx.add_ligand(lig)
+    rots = synth.so3_rotations(4608)[::577]           # 8 rotations
+    trans = synth.lattice((0, 0, 0), 20, 0.5)[::125]  # 64 translations
+    print("scan subset", len(rots) * len(trans), flush=True)
+    for mode, shifted in ((ff.SHIFTED, True), (ff.GLOBAL, False)):
+        e = ctx.scan(lid, mode, rots, trans).cpu().numpy()
+        pr = np.repeat(np.arange(len(rots)), len(trans))
+        pt = np.tile(np.arange(len(trans)), len(rots))
+        sample = np.arange(0, len(e), 7)
+        if shifted:
+            ref = oracle.scan_shifted(rec, lig, rots, pr[sample], trans, pt[sample], nthreads=8)
+        else:
+            ref = []
+            for k in sample:
+                l, _ = _place(lig, rots[pr[k]], trans[pt[k]])
+                ref.append(oracle.ene_inter(rec, l, shifted=False))
+        sums = [inter_abs_sum(rec, _place(lig, rots[pr[k]], trans[pt[k]])[1], lig, shifted)
+                for k in sample]
+        assert_energy(e[sample], ref, sums, what="scan mode %d" % mode)
+    torch.cuda.synchronize()
+
+
+def test_rigid_scan_interp_matches_oracle():
+    import torch
+    lig, rec = synth.config0()
+    rng = np.random.default_rng(4)
+    n = 64
+    grids = rng.normal(0, 3, (len(lig["xs"]), n, n, n)).astype(np.float32)
+    origin, _, step = synth.grid_spec(n=n, step=0.375)
+    ctx = ff.Context(0)
+    ctx.set_grids(grids, step, origin)
+    lid = ctx.add_ligand(dict(lig, types_id=np.arange(len(lig["xs"]))))
+    rots = synth.so3_rotations(64)
+    trans = synth.lattice((0, 0, 0), 4, 0.5)
+    e = ctx.scan(lid, ff.INTERP, rots, trans).cpu().numpy()
+    g = oracle.Grid.make(step, (n, n, n))
+    pr = np.repeat(np.arange(len(rots)), len(trans))
+    pt = np.tile(np.arange(len(trans)), len(rots))
+    ref = oracle.scan_interp(g, list(grids), np.stack([lig["xs"], lig["ys"], lig["zs"]], 1),
+                             np.arange(len(lig["xs"])), rots, pr, trans - origin, pt, nthreads=4)
+    ok = ~np.isnan(ref)
+    assert ok.sum() > 100
+    assert np.array_equal(np.isnan(e), ~ok)
+    assert np.allclose(e[ok], ref[ok], rtol=1e-12, atol=1e-12)
+    torch.cuda.synchronize()
+
+
+# ----------------------------------------------------------------- DOF form
+def test_dof_coordinates_and_gradient(lib_sys):
+    import torch
+    ctx, ligs, ids, rec, _ = lib_sys
+    pick = [k for k in range(len(ligs)) if len(ligs[k]["rb_left"]) >= 2][:4]
+    pose_lig = np.array([ids[k] for k in pick for _ in range(3)])
+    b = ctx.batch(pose_lig)
+    rng = np.random.default_rng(5)
+    dofs = []
+    for p, l in enumerate(pose_lig):
+        nrb = len(ligs[l]["rb_left"])
+        dofs.append(np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-math.pi, math.pi, 3),
+                                    rng.uniform(-math.pi, math.pi, nrb)]))
+    dflat = np.concatenate(dofs)
+    dev = torch.device("cuda:0")
+    d_t = torch.tensor(dflat, device=dev)
+    xyz_t = torch.zeros((b.natoms, 3), dtype=torch.float64, device=dev)
+    e_t = torch.zeros((b.npose, 4), dtype=torch.float64, device=dev)
+    g_t = torch.zeros_like(d_t)
+    rad_t = torch.zeros(b.npose, dtype=torch.float64, device=dev)
+    print("dof batch", b.npose, flush=True)
+    b.score_dofs_dev(d_t, e_t, xyz_out=xyz_t, gdof=g_t, radius=rad_t,
+                     flags=ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD)
+    torch.cuda.synchronize()
+    xyz, e, gd, rad = (t.cpu().numpy() for t in (xyz_t, e_t, g_t, rad_t))
+    for p, l in enumerate(pose_lig):
+        lg = ligs[l]
+        c = np.stack([lg["xs"], lg["ys"], lg["zs"]], 1)
+        args = (lg["rb_left"], lg["rb_right"], lg["rg_off"], lg["rg_atoms"])
+        x_ref, r_ref = oracle.apply_config(c, *args, dofs[p])
+        assert np.allclose(xyz[b.atom_off[p]:b.atom_off[p + 1]], x_ref, atol=1e-10)
+        assert rad[p] == pytest.approx(r_ref, rel=1e-12)
+
+        def E(d):
+            x, _ = oracle.apply_config(c, *args, d)
+            m = dict(lg, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+            return oracle.ene_inter(rec, m) + oracle.ene_intra(m)
+        assert e[p, 0] + e[p, 1] == pytest.approx(E(dofs[p]), rel=1e-5, abs=1e-6)
+        fd = np.zeros(len(dofs[p]))
+        for k in range(len(dofs[p])):
+            h = 1e-5
+            dp, dm = dofs[p].copy(), dofs[p].copy()
+            dp[k] += h
+            dm[k] -= h
+            fd[k] = (E(dp) - E(dm)) / (2 * h)
+        got = gd[b.dof_off[p]:b.dof_off[p + 1]]
+        assert_grad(got, fd, tol=1e-4, what="dof grad pose %d" % p)
