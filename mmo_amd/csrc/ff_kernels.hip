@@ -361,12 +361,15 @@ __device__ __forceinline__ double trilin(const GridDev& g, const float* arr, dou
                                          double pz, bool& ok, double* dg) {
   // src/G3D.ml:97-157: i0 = int_of_float (x * one_div_step) (truncation)
   const double fx = px * g.ods, fy = py * g.ods, fz = pz * g.ods;
-  const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-  if (!(px >= 0.0 && py >= 0.0 && pz >= 0.0 && i0 + 1 < g.nx && j0 + 1 < g.ny &&
-        k0 + 1 < g.nz)) {
+  // cell test in floating point before the conversion (also rejects NaN and values
+  // that would saturate the integer conversion); equals 0 <= i0 < nx - 1 with i0 the
+  // truncation of a non-negative fx
+  if (!(px >= 0.0 && py >= 0.0 && pz >= 0.0 && fx < (double)(g.nx - 1) &&
+        fy < (double)(g.ny - 1) && fz < (double)(g.nz - 1))) {
     ok = false;
     return 0.0;
   }
+  const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
   const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
   const size_t b = i0 + j0 * nx + k0 * nxy;
   const double A = arr[b], B = arr[b + 1], C = arr[b + 1 + nx], D = arr[b + nx];

@@ -16,14 +16,17 @@ Choices the configuration text leaves open, recorded here:
   * UFF types map to elements for LJ (C_3,C_R->C, N_3,N_R->N, O_2,O_3->O, S_3->S,
     H_->H): the reference looks parameters up by atomic number only
     (src/UFF.ml:59-60, src/mol.ml:812).
-  * Conformers are grown atom by atom and rejected with the reference's clash rule
-    d^2 < (0.8 (r_vdW,i + r_vdW,j))^2 (src/ptable.ml:62-80) on pairs four or more bonds
-    apart: with 1.0 A bonds every 1-4 pair would fail the rule's topo >= 3 form.
+  * Conformers are grown depth-first with backtracking and rejected with the form of
+    the reference's clash rule d^2 < (a (r_vdW,i + r_vdW,j))^2 (src/ptable.ml:62-80)
+    on pairs four or more bonds apart, with a = 0.65 (0.5 for pairs with a hydrogen)
+    instead of 0.8: with 1.0 A bonds every 1-4 pair would fail the topo >= 3 form, and
+    large random trees do not fit at 0.8.
   * A receptor of 3000 atoms with a 2.6 A minimum separation cannot be packed in a
     6-14 A shell (random sequential adsorption jams near 630 atoms there), so the
     shell's outer radius is grown until 3000 atoms fit (``receptor_outer_radius``).
 """
 import math
+import os
 
 import numpy as np
 
@@ -39,6 +42,11 @@ RECEPTOR_FRACTIONS = {"C_3": 0.25, "C_R": 0.15, "N_3": 0.08, "N_R": 0.05, "O_2":
 # van der Waals radii used by the reference's clash rule (src/ptable.ml:32-54, :62-80)
 VDW_RADII = {1: 1.2, 6: 1.7, 7: 1.6, 8: 1.55, 9: 1.5, 12: 2.2, 15: 1.95, 16: 1.8, 17: 1.8,
              35: 1.9, 53: 2.1}
+
+# conformer-growth clash threshold d < alpha (r_vdW,i + r_vdW,j); the reference's docking
+# filter uses 0.8 (src/ptable.ml:62-80), growth uses 0.65 so that large random trees fit
+CLASH_ALPHA = 0.65
+CLASH_ALPHA_H = 0.5   # pairs involving a hydrogen
 
 SEEDS = {"config0": 20240, "config2": 20242, "config3": 20243, "config4": 20244}
 
@@ -126,7 +134,8 @@ def _build_tree(rng, n_heavy, n_h):
         cand = [a for a in order if len(adj[a]) < min(3, UFF_TYPES[types[a]][1])]
         if not cand:
             raise RuntimeError("tree cannot grow")
-        wts = np.array([{0: 1.0, 1: 1.0, 2: 0.15}[len(adj[a])] for a in cand])
+        # branching grows with size so that large ligands stay within 12 A of their centre
+        wts = np.array([{0: 1.0, 1: 1.0, 2: 0.15 + n_heavy / 300.0}[len(adj[a])] for a in cand])
         a = cand[rng.choice(len(cand), p=wts / wts.sum())]
         adj[a].append(i)
         adj[i].append(a)
@@ -145,92 +154,106 @@ def _build_tree(rng, n_heavy, n_h):
     return types, adj
 
 
-def _place(rng, types, adj, max_tries=200):
+def _place(rng, types, adj, tries=8, budget=6000):
+    """Depth-first conformer growth with backtracking: each atom's children are placed
+    at tetrahedral/trigonal angles with a random dihedral (extended ones first), the
+    subtree below is grown before the next sibling, and a subtree that cannot be
+    grown clash-free re-draws the dihedrals one level up."""
     n = len(types)
     anum = [UFF_TYPES[t][0] for t in types]
+    vdw = np.array([VDW_RADII[a] for a in anum])
+    anum_a = np.array(anum)
+    topo = _bfs_dists(adj)
     pos = np.zeros((n, 3))
     placed = np.zeros(n, dtype=bool)
     parent = [-1] * n
-    pos[0] = 0.0
-    placed[0] = True
-    # virtual references for the root
-    vref_a = np.array([-1.0, -1.0, 0.0])
-    vref_b = np.array([-1.0, 0.0, 0.0])
-    queue = [0]
-    topo = _bfs_dists(adj)
-    while queue:
-        b = queue.pop(0)
+    log = []
+    work = [0]
+
+    def clash_free(c, p):
+        pidx = np.nonzero(placed)[0]
+        far = pidx[topo[c, pidx] >= 4]
+        if not len(far):
+            return True
+        alpha = np.where((anum_a[c] == 1) | (anum_a[far] == 1), CLASH_ALPHA_H, CLASH_ALPHA)
+        lim = alpha * (vdw[c] + vdw[far])
+        return not np.any(np.sum((pos[far] - p) ** 2, axis=1) < lim * lim)
+
+    def put(c, p, b):
+        pos[c] = p
+        placed[c] = True
+        parent[c] = b
+        log.append(c)
+
+    def rollback(mark):
+        while len(log) > mark:
+            placed[log.pop()] = False
+
+    def refs(b):
+        a = parent[b]
+        if a < 0:
+            return None
+        za = parent[a]
+        if za < 0:
+            sib = [s for s in adj[a] if placed[s] and s != b]
+            return (pos[sib[0]] if sib else pos[a] + np.array([0.0, 1.0, 0.3])), pos[a]
+        return pos[za], pos[a]
+
+    def grow(b):
         kids = [c for c in adj[b] if not placed[c]]
         if not kids:
-            continue
+            return True
         sp2 = UFF_TYPES[types[b]][2]
         theta = math.radians(120.0 if sp2 else 109.5)
-        if parent[b] >= 0:
-            a = parent[b]
-            za = parent[a] if parent[a] >= 0 else None
-            if za is None:
-                # the parent is the root: use one of the root's other bonds as reference
-                sib = [s for s in adj[a] if placed[s] and s != b]
-                ref_a = pos[sib[0]] if sib else pos[a] + np.array([0.0, 1.0, 0.3])
-            else:
-                ref_a = pos[za]
-            ref_b = pos[a]
-        else:
-            # root: first neighbour along +x, the others placed around that bond
-            c0 = kids[0]
-            pos[c0] = np.array([rng.uniform(1.0, 1.54), 0.0, 0.0])
-            placed[c0] = True
-            parent[c0] = b
-            queue.append(c0)
-            kids = kids[1:]
-            if not kids:
-                continue
-            ref_a, ref_b = vref_a + pos[c0], pos[c0]
         step = 2.0 * math.pi / (2 if sp2 else 3)
-        for _ in range(max_tries):
-            phi0 = rng.uniform(-math.pi, math.pi)
-            trial = {}
-            for m, c in enumerate(kids):
-                length = rng.uniform(1.0, 1.54)
-                trial[c] = _nerf(ref_a, ref_b, pos[b], length, theta, phi0 + m * step)
+        for attempt in range(tries):
+            work[0] += 1
+            if work[0] > budget:
+                return False
+            mark = len(log)
+            r = refs(b)
+            first = []
+            if r is None:
+                # root: first neighbour along a random direction, the others around it
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                first = [(kids[0], pos[b] + d * rng.uniform(1.0, 1.54))]
+                ref_a, ref_b = pos[b] + np.cross(d, [0.3, 0.7, 0.2]) - d, first[0][1]
+                rest = kids[1:]
+                # the angle is measured at the root between the first bond and the others
+                trial = first + [(c, _nerf(ref_a, ref_b, pos[b], rng.uniform(1.0, 1.54), theta,
+                                           rng.uniform(-math.pi, math.pi) + m * step))
+                                 for m, c in enumerate(rest)]
+            else:
+                ref_a, ref_b = r
+                phi0 = (math.pi + rng.uniform(-math.pi / 3, math.pi / 3)) if attempt < tries // 2 \
+                    else rng.uniform(-math.pi, math.pi)
+                trial = [(c, _nerf(ref_a, ref_b, pos[b], rng.uniform(1.0, 1.54), theta,
+                                   phi0 + m * step)) for m, c in enumerate(kids)]
             ok = True
-            for c, p in trial.items():
-                for o in np.nonzero(placed)[0]:
-                    if topo[c, o] >= 4:
-                        lim = 0.8 * (VDW_RADII[anum[c]] + VDW_RADII[anum[o]])
-                        if np.sum((p - pos[o]) ** 2) < lim * lim:
-                            ok = False
-                            break
-                if not ok:
+            for c, p in trial:
+                if not clash_free(c, p):
+                    ok = False
                     break
-            if ok:
-                cs = list(trial)
-                for i1 in range(len(cs)):
-                    for i2 in range(i1 + 1, len(cs)):
-                        c1, c2 = cs[i1], cs[i2]
-                        if topo[c1, c2] >= 4:
-                            lim = 0.8 * (VDW_RADII[anum[c1]] + VDW_RADII[anum[c2]])
-                            if np.sum((trial[c1] - trial[c2]) ** 2) < lim * lim:
-                                ok = False
-            if ok:
-                break
-        else:
-            return None
-        for c, p in trial.items():
-            pos[c] = p
-            placed[c] = True
-            parent[c] = b
-            queue.append(c)
-    return pos
+                put(c, p, b)
+            if ok and all(grow(c) for c, _ in trial):
+                return True
+            rollback(mark)
+        return False
+
+    put(0, np.zeros(3), -1)
+    return pos if grow(0) else None
 
 
 def make_ligand(rng, n_heavy, n_h, n_rot, name="lig"):
     """A seeded random-tree ligand (see module docstring); charges ~N(0, 0.25) with
     net charge 0; n_rot rotatable bonds among internal heavy-heavy bonds."""
-    for _attempt in range(100):
+    for _attempt in range(400):
         types, adj = _build_tree(rng, n_heavy, n_h)
         pos = _place(rng, types, adj)
-        if pos is not None:
+        # the reference drops ligands whose enclosing radius exceeds 12 A
+        # (Mol.check_elongation_exn at load, src/lds.ml:1875-1885)
+        if pos is not None and 0.01 + np.max(np.linalg.norm(pos - pos.mean(0), axis=1)) <= 12.0:
             break
     else:
         raise RuntimeError("could not build a clash-free ligand")
@@ -411,15 +434,27 @@ def grid_spec(center=(0.0, 0.0, 0.0), n=64, step=0.375):
     return origin, (n, n, n), step
 
 
+def _lib_one(args):
+    seed, k, nh, n_h, nr = args
+    rng = np.random.Generator(np.random.PCG64(np.random.SeedSequence([seed, k])))
+    return make_ligand(rng, nh, n_h, nr, name="lib%04d" % k)
+
+
 def library(n_ligands=1024, seed=None, median_heavy=30, sigma=0.35, h_ratio=0.875,
-            max_rot=15):
+            max_rot=15, processes=1):
     """Ligand library: heavy ~ round(lognormal(ln median, sigma)) clipped to [12, 70],
-    H ~ h_ratio * heavy, rotatable bonds ~ U{0..max_rot}."""
-    rng = np.random.Generator(np.random.PCG64(SEEDS["config2"] if seed is None else seed))
-    ligs = []
+    H ~ h_ratio * heavy, rotatable bonds ~ U{0..max_rot}.  Ligand k is grown from its
+    own seed (seed, k), so the library is the same for any number of processes."""
+    seed = SEEDS["config2"] if seed is None else seed
+    rng = np.random.Generator(np.random.PCG64(seed))
+    jobs = []
     for k in range(n_ligands):
         nh = int(np.clip(round(rng.lognormal(math.log(median_heavy), sigma)), 12, 70))
-        n_h = int(round(h_ratio * nh))
         nr = int(rng.integers(0, max_rot + 1))
-        ligs.append(make_ligand(rng, nh, n_h, nr, name="lib%04d" % k))
-    return ligs
+        jobs.append((seed, k, nh, int(round(h_ratio * nh)), nr))
+    # worker processes are started fresh ("spawn"): nothing GPU-related is inherited
+    if processes and processes > 1 and n_ligands >= 32:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(min(processes, 8)) as pool:
+            return pool.map(_lib_one, jobs, chunksize=4)
+    return [_lib_one(j) for j in jobs]
