@@ -61,6 +61,15 @@ class MMOError(RuntimeError):
 def load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("libmmo_ff.so is not built (%s); run python -m mmo_amd.build" % LIB_PATH)
+    # PyTorch ships its own HIP runtime (libamdhip64.so.7).  Loading torch first makes
+    # the dynamic linker bind this library's libamdhip64.so.7 dependency to that same
+    # runtime: one HIP runtime per process, so device pointers and streams are shared.
+    # Loaded the other way round, torch would bring up a second runtime that finds no
+    # GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ct.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)   # AttributeError if the symbol is missing
