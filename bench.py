@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rotations", type=int, default=N_ROT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--tune", default="", help="kernel shape, e.g. cull_k=1,cull_waves=16")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the scoring path is built and measured on a single "
@@ -86,7 +87,11 @@ def main():
     trans = synth.lattice((0.0, 0.0, 0.0), N_TRANS_SIDE, TRANS_STEP)
     npose = len(rots) * len(trans)
     ctx = ff.Context(0)
+    tune = dict(kv.split("=") for kv in args.tune.split(",") if kv)
+    if "cell_size" in tune:
+        ctx.tune(cell_size=float(tune.pop("cell_size")))
     ctx.set_receptor(rec)
+    ctx.tune(**{k: float(v) for k, v in tune.items()})
     lid = ctx.add_ligand(lig)
     out = torch.empty(npose, dtype=torch.float64, device="cuda:0")
     print("bench: %d poses per step, %d warmup, %d timed" % (npose, args.warmup, args.steps),
@@ -119,7 +124,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair math, fp64 per-pose sums",
+        "dtype": "fp32", "accumulate": "fp32 chunks folded into fp64 per-pose sums",
         "data": "synthetic",
         "config": {"workload": "exhaustive rigid scan, direct shifted LJ+Coulomb",
                    "ligand_atoms": len(lig["xs"]), "receptor_atoms": len(rec["xs"]),
@@ -127,7 +132,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp32-valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_inter (scan)", "kernel_ms": avg_ms,
+                     "kernel": "k_inter_cull (scan)" if ctx.tuning("scan_cull") > 0 and
+                     ctx.tuning("cull_ok") > 0 else "k_inter (scan)", "kernel_ms": avg_ms,
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
     }
     if not args.no_cpu:

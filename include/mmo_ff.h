@@ -180,6 +180,11 @@ int mmo_scan_dev(mmo_ctx* ctx, int lig_id, int mode, const double* rots, int nro
 double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
 /* Fixed-point scale exponent k (coordinates are stored as round(x * 2^k)). */
 int mmo_fixed_point_bits(const mmo_ctx* ctx);
+/* Kernel shape parameters, set by the autotuner (mmo_amd/tune.py): "inter_k",
+ * "inter_waves", "inter_rtile", "cull_k", "cull_waves", "scan_cull", "cell_size"
+ * (the last applies to the next mmo_set_receptor).  No counterpart in the reference. */
+int mmo_tune_set(mmo_ctx* ctx, const char* key, double value);
+double mmo_tune_get(mmo_ctx* ctx, const char* key);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,8 @@ SIGNATURES = {
     "mmo_scan_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _p]),
     "mmo_last_kernel_ms": (_d, [_p, ct.c_char_p]),
     "mmo_fixed_point_bits": (_i, [_p]),
+    "mmo_tune_set": (_i, [_p, ct.c_char_p, _d]),
+    "mmo_tune_get": (_d, [_p, ct.c_char_p]),
 }
 
 

@@ -79,6 +79,14 @@ struct mmo_ctx {
   int fp_bits = 24;
   DevBuf<RecPos> rpos;
   DevBuf<RecAB> rab;
+  // receptor cell list (culled shifted kernel)
+  float cell_size = 5.0f;
+  bool cull_ok = false;
+  int scan_cull = 1;
+  int cull_k = 1;
+  int cull_waves = 16;
+  CellGrid cg{};
+  DevBuf<int> d_cell;
   // grids
   bool has_grid = false;
   int ntypes = 0;
@@ -103,6 +111,7 @@ struct mmo_ctx {
   DevBuf<int> s_int;
   DevBuf<Tile> s_tile;
   DevBuf<double> s_scan;   // rotations + translations of the last scan
+  DevBuf<int> s_tperm;
   // tuning
   int inter_k = 4, inter_waves = 4, inter_rtile = 2048;
   std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -256,6 +265,10 @@ InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAto
   L.waves = c->inter_waves;
   L.rtile = std::min(c->inter_rtile, std::max(c->nrec, 1));
   L.rslice = c->nrec;
+  L.cull = 0;
+  L.cull_waves = c->cull_waves;
+  L.cull_k = c->cull_k;
+  L.cg = c->cg;
   L.fr = c->fr;
   L.rpos = c->rpos.p;
   L.rab = c->rab.p;
@@ -387,7 +400,8 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
     c->s_xyz.release(); c->s_e.release(); c->s_g.release(); c->s_part.release();
     c->s_lig.release(); c->s_latom.release(); c->s_pij.release(); c->s_ppar.release();
-    c->s_int.release(); c->s_tile.release(); c->s_scan.release();
+    c->s_int.release(); c->s_tile.release(); c->s_scan.release(); c->s_tperm.release();
+    c->d_cell.release();
     for (auto& kv : c->ev) {
       (void)hipEventDestroy(kv.second.first);
       (void)hipEventDestroy(kv.second.second);
@@ -445,22 +459,53 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   fr.inv_s = (float)std::ldexp(1.0, -k);
   fr.inv_s2 = (float)std::ldexp(1.0, -2 * k);
   fr.lim = (1 << 30) - 1;
+  // cell list: atoms sorted by cell (x fastest), cell_start as exclusive prefix sums;
+  // coordinates in A relative to the frame origin
+  const double cs = c->cell_size;
+  const double gx0 = lo[0] - fr.ox - 1e-3, gy0 = lo[1] - fr.oy - 1e-3, gz0 = lo[2] - fr.oz - 1e-3;
+  const int gnx = (int)std::floor((hi[0] - fr.ox - gx0) / cs) + 1;
+  const int gny = (int)std::floor((hi[1] - fr.oy - gy0) / cs) + 1;
+  const int gnz = (int)std::floor((hi[2] - fr.oz - gz0) / cs) + 1;
+  const size_t ncell = (size_t)gnx * gny * gnz;
+  std::vector<int> cell(n), order(n);
+  for (int i = 0; i < n; ++i) {
+    const int cx = std::min(gnx - 1, std::max(0, (int)std::floor((xs[i] - fr.ox - gx0) / cs)));
+    const int cy = std::min(gny - 1, std::max(0, (int)std::floor((ys[i] - fr.oy - gy0) / cs)));
+    const int cz = std::min(gnz - 1, std::max(0, (int)std::floor((zs[i] - fr.oz - gz0) / cs)));
+    cell[i] = (cz * gny + cy) * gnx + cx;
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
+  std::vector<int> cstart(ncell + 1, 0);
+  for (int i = 0; i < n; ++i) cstart[cell[i] + 1]++;
+  for (size_t k = 0; k < ncell; ++k) cstart[k + 1] += cstart[k];
   std::vector<RecPos> pos(n);
   std::vector<RecAB> ab(n);
-  for (int i = 0; i < n; ++i) {
+  for (int k = 0; k < n; ++k) {
+    const int i = order[k];
     double A, B;
     r = atom_ab(c, anum[i], &A, &B);
     if (r) return r;
-    pos[i].x = (int)std::llrint((xs[i] - fr.ox) * fr.scale);
-    pos[i].y = (int)std::llrint((ys[i] - fr.oy) * fr.scale);
-    pos[i].z = (int)std::llrint((zs[i] - fr.oz) * fr.scale);
-    pos[i].q = (float)q[i];
-    ab[i].a = (float)A;
-    ab[i].b = (float)B;
+    pos[k].x = (int)std::llrint((xs[i] - fr.ox) * fr.scale);
+    pos[k].y = (int)std::llrint((ys[i] - fr.oy) * fr.scale);
+    pos[k].z = (int)std::llrint((zs[i] - fr.oz) * fr.scale);
+    pos[k].q = (float)q[i];
+    ab[k].a = (float)A;
+    ab[k].b = (float)B;
   }
   if ((r = upload(c, c->rpos, pos.data(), pos.size(), c->stream))) return r;
   if ((r = upload(c, c->rab, ab.data(), ab.size(), c->stream))) return r;
+  if ((r = upload(c, c->d_cell, cstart.data(), cstart.size(), c->stream))) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  CellGrid cg{};
+  cg.cell_start = c->d_cell.p;
+  cg.x0 = (float)gx0; cg.y0 = (float)gy0; cg.z0 = (float)gz0;
+  cg.c = (float)cs; cg.inv_c = (float)(1.0 / cs);
+  cg.nx = gnx; cg.ny = gny; cg.nz = gnz;
+  c->cg = cg;
+  // the culled kernel keeps the whole receptor and the cell table in LDS (160 KiB/CU)
+  const size_t lds = (size_t)n * (sizeof(RecPos) + sizeof(RecAB)) + (ncell + 1) * sizeof(int);
+  c->cull_ok = lds <= 150 * 1024;
   c->fr = fr;
   c->fp_bits = k;
   c->nrec = n;
@@ -820,10 +865,14 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
   if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
     const int mode = (inter & MMO_F_INTER_SHIFTED) ? MMO_SHIFTED : MMO_GLOBAL;
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    if (mode == MMO_SHIFTED && c->cull_ok && b->ntiles >= 1024) {
+      L.cull = 1;
+      L.cull_waves = std::max(1, std::min(c->cull_waves, b->ntiles / 512));
+    }
     // few tiles: split the receptor over block rows so that the chip fills up
     const int blocks = (b->ntiles + L.waves - 1) / L.waves;
     int nslice = 1;
-    while (blocks * nslice < 512 && c->nrec / (nslice * 2) >= 128) nslice *= 2;
+    while (!L.cull && blocks * nslice < 512 && c->nrec / (nslice * 2) >= 128) nslice *= 2;
     L.rslice = (c->nrec + nslice - 1) / nslice;
     L.rtile = std::min(L.rtile, L.rslice);
     nslice = (c->nrec + L.rslice - 1) / L.rslice;
@@ -956,6 +1005,35 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     for (int t : c->ligs[lig_id].type)
       if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
   hipStream_t s = (hipStream_t)stream;
+  // translations in Morton order of their per-axis ranks: 64 consecutive slots form a
+  // compact brick (a 4x4x4 block of a lattice)
+  std::vector<int> tperm(ntrans);
+  {
+    std::vector<uint32_t> rank[3];
+    for (int a = 0; a < 3; ++a) {
+      std::vector<double> v(ntrans);
+      for (int t = 0; t < ntrans; ++t) v[t] = trans[3 * t + a];
+      std::vector<double> u = v;
+      std::sort(u.begin(), u.end());
+      std::vector<double> uq;
+      for (double x : u)
+        if (uq.empty() || x - uq.back() > 1e-9) uq.push_back(x);
+      rank[a].resize(ntrans);
+      for (int t = 0; t < ntrans; ++t)
+        rank[a][t] = (uint32_t)(std::lower_bound(uq.begin(), uq.end(), v[t] - 1e-9) - uq.begin());
+    }
+    auto spread = [](uint32_t x) {
+      uint64_t r = 0;
+      for (int b = 0; b < 21; ++b) r |= (uint64_t)((x >> b) & 1u) << (3 * b);
+      return r;
+    };
+    std::vector<uint64_t> key(ntrans);
+    for (int t = 0; t < ntrans; ++t)
+      key[t] = spread(rank[0][t]) | (spread(rank[1][t]) << 1) | (spread(rank[2][t]) << 2);
+    for (int t = 0; t < ntrans; ++t) tperm[t] = t;
+    std::stable_sort(tperm.begin(), tperm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  if ((r = upload(c, c->s_tperm, tperm.data(), tperm.size(), s))) return r;
   HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
   HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double),
                            hipMemcpyHostToDevice, s));
@@ -966,12 +1044,13 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
   if (mode == MMO_INTERP) {
     InterpLaunch L{c->g, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p};
     timer_begin(c, "interp", s);
-    HIPCHK(c, launch_interp_scan(L, lig_id, d_rot, nrot, d_trans, ntrans, e_out, s));
+    HIPCHK(c, launch_interp_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "interp", s);
   } else {
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    L.cull = (mode == MMO_SHIFTED && c->cull_ok && c->scan_cull) ? 1 : 0;
     timer_begin(c, "inter", s);
-    HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, ntrans, e_out, s));
+    HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "inter", s);
   }
   // the host arrays were copied asynchronously from pageable memory; wait so that the
@@ -995,5 +1074,39 @@ double mmo_last_kernel_ms(mmo_ctx* c, const char* kernel) {
 }
 
 int mmo_fixed_point_bits(const mmo_ctx* c) { return c ? c->fp_bits : MMO_ERR_BAD_ARG; }
+
+// kernel shape parameters (the autotuner sets these); cell_size applies to the next
+// mmo_set_receptor call
+int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!key) return fail(c, MMO_ERR_BAD_ARG, "NULL key");
+  const std::string k = key;
+  const int iv = (int)v;
+  auto pow2_in = [](int x, int lo, int hi) { return x >= lo && x <= hi && (x & (x - 1)) == 0; };
+  if (k == "inter_k" && pow2_in(iv, 1, 8)) c->inter_k = iv;
+  else if (k == "inter_waves" && iv >= 1 && iv <= 4) c->inter_waves = iv;
+  else if (k == "inter_rtile" && iv >= 64 && iv <= 6144) c->inter_rtile = iv;
+  else if (k == "cull_k" && pow2_in(iv, 1, 4)) c->cull_k = iv;
+  else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
+  else if (k == "scan_cull" && (iv == 0 || iv == 1)) c->scan_cull = iv;
+  else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
+  else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
+  return MMO_OK;
+}
+
+double mmo_tune_get(mmo_ctx* c, const char* key) {
+  if (!c || !key) return -1.0;
+  const std::string k = key;
+  if (k == "inter_k") return c->inter_k;
+  if (k == "inter_waves") return c->inter_waves;
+  if (k == "inter_rtile") return c->inter_rtile;
+  if (k == "cull_k") return c->cull_k;
+  if (k == "cull_waves") return c->cull_waves;
+  if (k == "scan_cull") return c->scan_cull;
+  if (k == "cell_size") return c->cell_size;
+  if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;
+  return -1.0;
+}
 
 }  // extern "C"

@@ -50,6 +50,16 @@ struct Frame {
   int lim;           // 2^30 - 1
 };
 
+// Receptor cell list for culling: atoms sorted by cell (z, y, x order, x fastest);
+// cell_start[c]..cell_start[c+1] are the atoms of cell c.  Coordinates in A relative
+// to the fixed-point frame origin.
+struct CellGrid {
+  const int* cell_start;   // ncell + 1
+  float x0, y0, z0;        // corner of cell (0,0,0)
+  float c, inv_c;          // cell edge
+  int nx, ny, nz;
+};
+
 // A pose tile: npose poses of ligand `lig` starting at pose0 (poses of one ligand are
 // contiguous after planning); each pose is handled by `split` lanes.
 struct Tile { int lig, pose0, npose, split; };

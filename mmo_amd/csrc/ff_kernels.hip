@@ -35,10 +35,13 @@ struct CoordSrc {
   __device__ int row(int pose, int j) const { return atom_off[pose] + j; }
 };
 
-// rigid scan: pose = rot * ntrans + trans, x = R c + T (src/lds.ml:1081-1094)
+// rigid scan: pose = rot * ntrans + trans, x = R c + T (src/lds.ml:1081-1094).
+// Tiles walk the translations in the order tperm (Morton order of their positions),
+// so the 64 poses of a wavefront sit in a compact brick of space.
 struct ScanSrc {
   const double* rots;     // nrot x 9
   const double* trans;    // ntrans x 3
+  const int* tperm;       // tile slot -> translation index
   int ntrans, tiles_per_rot, lig;
   __device__ Tile tile(int t) const {
     const int r = t / tiles_per_rot, tb = t - r * tiles_per_rot;
@@ -49,7 +52,10 @@ struct ScanSrc {
     o.split = 1;
     return o;
   }
-  __device__ int pose_id(int slot) const { return slot; }
+  __device__ int pose_id(int slot) const {
+    const int r = slot / ntrans;
+    return r * ntrans + tperm[slot - r * ntrans];
+  }
   __device__ void pos(int pose, int, int j, const LigInfo& li, const LigAtom*,
                       const double* lig_xyz, double& x, double& y, double& z) const {
     const int r = pose / ntrans, t = pose - r * ntrans;
@@ -285,6 +291,144 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   }
 }
 
+// ------------------------------------------------------- inter kernel, culled
+// Shifted form only.  The whole receptor sits in LDS sorted by 4 A cells.  For each
+// block of K atoms per lane the wavefront takes the bounding box of its lanes' atom
+// positions and visits only the cell rows that intersect the sphere of radius
+// 12 A + half-diagonal around the box centre; every receptor atom within 12 A of any
+// of the wavefront's atoms is in a visited cell, and pairs beyond 12 A that are
+// visited contribute exactly 0 (the shift weight is clamped to 0).
+__device__ __forceinline__ float wave_minf(float v) {
+  for (int off = 1; off < kWave; off <<= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ __forceinline__ float wave_maxf(float v) {
+  for (int off = 1; off < kWave; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ __forceinline__ float uniformf(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <int K, class Src>
+__global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid cg) {
+  extern __shared__ __align__(16) char smem[];
+  const int nrec = a.nrec;
+  const int ncell = cg.nx * cg.ny * cg.nz;
+  RecPos* s_pos = reinterpret_cast<RecPos*>(smem);
+  RecAB* s_ab = reinterpret_cast<RecAB*>(smem + (size_t)nrec * sizeof(RecPos));
+  int* s_cell = reinterpret_cast<int*>(smem + (size_t)nrec * (sizeof(RecPos) + sizeof(RecAB)));
+  for (int i = threadIdx.x; i < nrec; i += blockDim.x) {
+    s_pos[i] = a.rpos[i];
+    s_ab[i] = a.rab[i];
+  }
+  for (int i = threadIdx.x; i <= ncell; i += blockDim.x) s_cell[i] = cg.cell_start[i];
+  __syncthreads();
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int tidx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  if (tidx >= a.ntiles) return;
+  const Tile t = a.src.tile(tidx);
+  const int split = t.split;
+  const int pslot = lane / split, aslot = lane - pslot * split;
+  const bool pose_ok = pslot < t.npose;
+  const int pose = pose_ok ? a.src.pose_id(t.pose0 + pslot) : 0;
+  const LigInfo li = a.ligs[t.lig];
+  const int per_lane = (li.n_atoms + split - 1) / split;
+  const int nblk = (per_lane + K - 1) / K;
+  const Frame fr = a.fr;
+  double e_el = 0.0, e_vd = 0.0;
+  int sat = 0;
+
+  for (int blk = 0; blk < nblk; ++blk) {
+    int lx[K], ly[K], lz[K];
+    double cq[K], ca[K], cb[K];
+    float mnx = 3.0e38f, mny = 3.0e38f, mnz = 3.0e38f;
+    float mxx = -3.0e38f, mxy = -3.0e38f, mxz = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = aslot + split * (blk * K + k);
+      if (pose_ok && j < li.n_atoms) {
+        double x, y, z;
+        a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
+        lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
+        ly[k] = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
+        lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+        const LigAtom la = a.latoms[li.atom_base + j];
+        cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
+        const float fx = (float)lx[k] * fr.inv_s, fy = (float)ly[k] * fr.inv_s,
+                    fz = (float)lz[k] * fr.inv_s;
+        mnx = fminf(mnx, fx); mny = fminf(mny, fy); mnz = fminf(mnz, fz);
+        mxx = fmaxf(mxx, fx); mxy = fmaxf(mxy, fy); mxz = fmaxf(mxz, fz);
+      } else {
+        // inert atom (zero parameters) parked at the first valid position's box
+        lx[k] = ly[k] = lz[k] = 0;
+        cq[k] = ca[k] = cb[k] = 0.0;
+      }
+    }
+    mnx = uniformf(wave_minf(mnx)); mny = uniformf(wave_minf(mny)); mnz = uniformf(wave_minf(mnz));
+    mxx = uniformf(wave_maxf(mxx)); mxy = uniformf(wave_maxf(mxy)); mxz = uniformf(wave_maxf(mxz));
+    if (!(mnx <= mxx)) continue;   // no atom in this block on any lane
+    const float qx = 0.5f * (mnx + mxx), qy = 0.5f * (mny + mxy), qz = 0.5f * (mnz + mxz);
+    const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+    const float R = uniformf(12.0f + sqrtf(hx * hx + hy * hy + hz * hz) + 0.05f);
+    const float R2 = R * R;
+    const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
+    const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
+    for (int cz = cz0; cz <= cz1; ++cz) {
+      const float zlo = cg.z0 + cz * cg.c;
+      const float dz = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
+      const float rz2 = R2 - dz * dz;
+      if (rz2 < 0.0f) continue;
+      const float ry = sqrtf(rz2);
+      const int cy0 = max(0, (int)floorf((qy - ry - cg.y0) * cg.inv_c));
+      const int cy1 = min(cg.ny - 1, (int)floorf((qy + ry - cg.y0) * cg.inv_c));
+      float sq[K], s12[K], s6[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
+      for (int cy = cy0; cy <= cy1; ++cy) {
+        const float ylo = cg.y0 + cy * cg.c;
+        const float dy = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
+        const float rx2 = rz2 - dy * dy;
+        if (rx2 < 0.0f) continue;
+        const float rx = sqrtf(rx2);
+        const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
+        const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+        if (cx0 > cx1) continue;
+        const int rowb = (cz * cg.ny + cy) * cg.nx;
+        const int a0 = uniformi(s_cell[rowb + cx0]);
+        const int a1 = uniformi(s_cell[rowb + cx1 + 1]);
+#pragma unroll 2
+        for (int i = a0; i < a1; ++i) {
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            pair_energy<MMO_SHIFTED>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, sq[k], s12[k], s6[k]);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        e_el = fma((double)sq[k], cq[k], e_el);
+        e_vd = fma((double)s12[k], ca[k], e_vd);
+        e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
+      }
+    }
+  }
+  e_el = group_sum(e_el, split);
+  e_vd = group_sum(e_vd, split);
+  (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
+  if (pose_ok && aslot == 0) {
+    const size_t o = (size_t)pose * a.e_stride;
+    a.e_out[o] = e_el + e_vd;
+    if (a.e_comp) {
+      a.e_comp[o + 2] = e_el;
+      a.e_comp[o + 3] = e_vd;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ intra kernel
 // one wavefront per pose; pairs of the ligand's list are spread over lanes
 template <bool GRAD>
@@ -472,6 +616,9 @@ static hipError_t launch_inter_any(int mode, int k, bool grad, const InterArgs<S
   return hipErrorInvalidValue;
 }
 
+template <class Src>
+static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hipStream_t s);
+
 hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const int* perm,
                                int ntiles, int npose, const double* xyz, const int* atom_off,
                                double* e_out, int e_stride, bool comp, double* grad, int write_e,
@@ -483,6 +630,11 @@ hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const in
   a.e_stride = e_stride; a.grad = grad; a.write_e = write_e;
   a.rslice = L.rslice > 0 ? L.rslice : L.nrec;
   a.npose_total = npose;
+  if (L.cull && !grad) {
+    a.e_out = e_out;
+    a.e_comp = comp ? e_out : nullptr;
+    return launch_cull(L, a, s);
+  }
   const int ny = (L.nrec + a.rslice - 1) / a.rslice;
   a.e_out = ny > 1 ? part : e_out;
   a.e_comp = comp ? a.e_out : nullptr;
@@ -494,15 +646,44 @@ hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const in
   return hipGetLastError();
 }
 
+template <class Src>
+static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hipStream_t s) {
+  const int waves = L.cull_waves;
+  const int blocks = (a.ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  const size_t lds = (size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+                     (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int);
+  // dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB per CU)
+#define MMO_CULL(KK)                                                                           \
+  case KK: {                                                                                   \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inter_cull<KK, Src>),  \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \
+    if (e != hipSuccess) return e;                                                             \
+    hipLaunchKernelGGL((k_inter_cull<KK, Src>), dim3(blocks), dim3(waves * kWave), lds, s, a,  \
+                       L.cg);                                                                  \
+    break;                                                                                     \
+  }
+  switch (L.cull_k) {
+    MMO_CULL(1)
+    MMO_CULL(2)
+    MMO_CULL(4)
+    default: return hipErrorInvalidValue;
+  }
+#undef MMO_CULL
+  return hipGetLastError();
+}
+
 hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, int nrot,
-                             const double* trans, int ntrans, double* e_out, hipStream_t s) {
+                             const double* trans, const int* tperm, int ntrans, double* e_out,
+                             hipStream_t s) {
   InterArgs<ScanSrc> a;
-  a.src.rots = rots; a.src.trans = trans; a.src.ntrans = ntrans;
+  a.src.rots = rots; a.src.trans = trans; a.src.tperm = tperm; a.src.ntrans = ntrans;
   a.src.tiles_per_rot = (ntrans + kWave - 1) / kWave; a.src.lig = lig;
   a.ntiles = nrot * a.src.tiles_per_rot; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab;
   a.nrec = L.nrec; a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms;
   a.lig_xyz = L.lig_xyz; a.e_out = e_out; a.e_stride = 1; a.e_comp = nullptr; a.grad = nullptr;
   a.write_e = 1; a.rslice = L.nrec; a.npose_total = nrot * ntrans;
+  if (L.cull) return launch_cull(L, a, s);
   return launch_inter_any(L.mode, L.k, false, a, L.waves, s);
 }
 
@@ -532,9 +713,10 @@ hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const 
 }
 
 hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots, int nrot,
-                              const double* trans, int ntrans, double* e_out, hipStream_t s) {
+                              const double* trans, const int* tperm, int ntrans, double* e_out,
+                              hipStream_t s) {
   InterpArgs<ScanSrc> a;
-  a.src.rots = rots; a.src.trans = trans; a.src.ntrans = ntrans;
+  a.src.rots = rots; a.src.trans = trans; a.src.tperm = tperm; a.src.ntrans = ntrans;
   a.src.tiles_per_rot = (ntrans + kWave - 1) / kWave; a.src.lig = lig;
   a.ntiles = nrot * a.src.tiles_per_rot; a.g = L.g; a.ligs = L.ligs; a.latoms = L.latoms;
   a.lig_xyz = L.lig_xyz; a.type_override = nullptr; a.e_out = e_out; a.e_stride = 1;

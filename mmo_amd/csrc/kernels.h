@@ -12,6 +12,10 @@ struct InterLaunch {
   int waves;           // wavefronts per block
   int rtile;           // receptor atoms per LDS tile
   int rslice;          // receptor atoms per block row (gridDim.y > 1 splits the receptor)
+  int cull;            // shifted form with the receptor cell list in LDS (k_inter_cull)
+  int cull_waves;      // wavefronts per block of the culled kernel
+  int cull_k;          // ligand atoms per lane in the culled kernel (1, 2, 4)
+  CellGrid cg;
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
@@ -75,14 +79,16 @@ hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const in
                                double* e_out, int e_stride, bool comp, double* grad, int write_e,
                                double* part, hipStream_t s);
 hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, int nrot,
-                             const double* trans, int ntrans, double* e_out, hipStream_t s);
+                             const double* trans, const int* tperm, int ntrans, double* e_out,
+                             hipStream_t s);
 hipError_t launch_intra(const IntraArgs& a, bool grad, hipStream_t s);
 hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const int* perm,
                                 int ntiles, const double* xyz, const int* atom_off,
                                 const int* type_override, double* e_out, int e_stride,
                                 double* grad, int grad_accumulate, hipStream_t s);
 hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots, int nrot,
-                              const double* trans, int ntrans, double* e_out, hipStream_t s);
+                              const double* trans, const int* tperm, int ntrans, double* e_out,
+                              hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);

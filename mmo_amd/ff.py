@@ -214,6 +214,13 @@ class Context:
                                      ct.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
 
+    def tune(self, **kv):
+        for k, v in kv.items():
+            self._check(lib.mmo_tune_set(self.h, k.encode(), float(v)))
+
+    def tuning(self, key):
+        return lib.mmo_tune_get(self.h, key.encode())
+
     def last_kernel_ms(self, name):
         return lib.mmo_last_kernel_ms(self.h, name.encode())
 

@@ -305,3 +305,62 @@ def test_dof_coordinates_and_gradient(lib_sys):
             fd[k] = (E(dp) - E(dm)) / (2 * h)
         got = gd[b.dof_off[p]:b.dof_off[p + 1]]
         assert_grad(got, fd, tol=1e-4, what="dof grad pose %d" % p)
+
+
+def test_rigid_scan_culled_equals_unculled_and_oracle(sys0):
+    """The culled kernel (cell list in LDS) drops only pairs beyond 12 A."""
+    import torch
+    ctx, lig, rec = sys0
+    lid = ctx.add_ligand(lig)
+    rots = synth.so3_rotations(4608)[::288]          # 16 rotations
+    trans = synth.lattice((0, 0, 0), 20, 0.5)        # 8000 translations (Morton-ordered bricks)
+    print("culled scan", len(rots) * len(trans), flush=True)
+    ctx.tune(scan_cull=1)
+    ec = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+    ctx.tune(scan_cull=0)
+    eu = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+    ctx.tune(scan_cull=1)
+    assert np.all(np.isfinite(ec))
+    rel = np.abs(ec - eu) / np.maximum(np.abs(eu), 1e-9)
+    assert np.percentile(rel, 99.9) < 2e-6, rel.max()
+    rng = np.random.default_rng(8)
+    sample = rng.choice(len(ec), 48, replace=False)
+    pr, pt = sample // len(trans), sample % len(trans)
+    ref = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=8)
+    sums = [inter_abs_sum(rec, _place(lig, rots[a], trans[b])[1], lig, True) for a, b in zip(pr, pt)]
+    assert_energy(ec[sample], ref, sums, what="culled scan")
+    for k in (2, 4):
+        ctx.tune(cull_k=k)
+        e2 = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+        assert np.allclose(e2, ec, rtol=2e-6, atol=1e-6)
+    ctx.tune(cull_k=1)
+    torch.cuda.synchronize()
+
+
+def test_batch_culled_path(sys0):
+    """Batches with >= 1024 wavefront tiles take the culled kernel."""
+    ctx, lig, rec = sys0
+    lib2 = synth.library(2, seed=5)
+    ids = [ctx.add_ligand(l) for l in lib2]
+    rng = np.random.default_rng(9)
+    npl = 33000
+    pose_lig = np.array([ids[0]] * npl + [ids[1]] * npl)
+    b = ctx.batch(pose_lig)
+    xyz = np.zeros((b.natoms, 3))
+    rots = synth.random_rotations(rng, len(pose_lig))
+    trans = rng.uniform(-4, 4, (len(pose_lig), 3))
+    for p, l in enumerate(pose_lig):
+        lg = lib2[ids.index(l)]
+        c = np.stack([lg["xs"], lg["ys"], lg["zs"]], 1)
+        xyz[b.atom_off[p]:b.atom_off[p + 1]] = c @ rots[p].T + trans[p]
+    print("culled batch", len(pose_lig), flush=True)
+    e, _ = b.score(xyz, ff.F_INTER_SHIFTED)
+    sample = rng.choice(len(pose_lig), 24, replace=False)
+    refs, sums = [], []
+    for p in sample:
+        lg = lib2[ids.index(pose_lig[p])]
+        x = xyz[b.atom_off[p]:b.atom_off[p + 1]]
+        m = dict(lg, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+        refs.append(oracle.ene_inter(rec, m))
+        sums.append(inter_abs_sum(rec, x, m))
+    assert_energy(e[sample, 0], refs, sums, what="culled batch")
