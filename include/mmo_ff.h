@@ -79,6 +79,18 @@ int mmo_set_receptor(mmo_ctx* ctx, const double* xs, const double* ys, const dou
 int mmo_set_grids(mmo_ctx* ctx, const float* const* grids, int ntypes, double step,
                   int nx, int ny, int nz, const double* origin);
 
+/* GPU grid precompute.  Per-type FP32 grids for single-atom probes (probe_anum[t],
+ * probe_q[t]) at the nodes origin + (i, j, k) * step: the shifted E_inter of the
+ * probe, clamped at max_e (Params.max_E = 1e5): Mol.ene_inter_UFF_shifted_grid
+ * (src/mol.ml:964-989) via pre_calculate_FF_components_grid (src/lds.ml:452-469).
+ * Nodes at distance >= roi_r from roi_center stay 0 (bitmask_ROI_only,
+ * src/lds.ml:269-305, uses roi_r = ROI radius + 24 A); roi_center NULL: every node.
+ * The result becomes the context's interpolation grids (like mmo_set_grids); out
+ * (host, nullable) receives ntypes * nx*ny*nz floats, grid t at out + t*nx*ny*nz. */
+int mmo_grid_compute(mmo_ctx* ctx, const int32_t* probe_anum, const double* probe_q, int ntypes,
+                     double step, int nx, int ny, int nz, const double* origin,
+                     const double* roi_center, double roi_r, double max_e, float* out);
+
 /* ------------------------------------------------- plain forms (one ligand) */
 /* Mol.ene_inter_UFF_global_brute / _shifted_brute / _shifted_bst_components
  * (src/mol.ml:796-960).  e_elec/e_vdw nullable; e_elec already includes the
@@ -174,15 +186,60 @@ int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double
 int mmo_scan_dev(mmo_ctx* ctx, int lig_id, int mode, const double* rots, int nrot,
                  const double* trans, int ntrans, double* e_out, void* stream);
 
+/* -------------------------------------------------------------- Monte Carlo */
+/* Batched MC chains (Lds.simulate_lig, src/lds.ml:741-1000) on the GPU: chain k
+ * docks ligand chain_lig[k] from the start pose (rot0[9k..], pos0[3k..]) with rigid
+ * moves on even frames and torsion moves on odd frames, E = E_inter (MMO_SHIFTED:
+ * direct sum; MMO_INTERP: the context's grids) + E_intra (UFF_NB when
+ * MMO_MC_INTRA_UFF), Metropolis at `temperature` K, adaptive step sizes, best pose.
+ * Random numbers are Philox4x32-10 (seed; counter = chain, frame, draw), so runs are
+ * reproducible and independent of launch shape.  Every chain runs total_frames
+ * frames in all.  The reference has one process per chain (src/lds.ml:2050); here all
+ * chains of all ligands run in one persistent launch. */
+typedef struct mmo_mc mmo_mc;
+enum {
+  MMO_MC_TWEAK_RBONDS = 1,       /* torsion moves (the reference's tweak_rbonds)       */
+  MMO_MC_HARD_ROI = 2,           /* --hard-ROI: reset outside the ROI (src/lds.ml:908) */
+  MMO_MC_INTRA_UFF = 4,          /* E_intra = Mol.ene_intra_UFFNB_brute                */
+  MMO_MC_METROPOLIS_ALWAYS = 8,  /* accept/reject also without hard ROI (see below)    */
+  MMO_MC_NO_FLIP = 16,           /* --no-flip-move (src/lds.ml:1828)                   */
+  MMO_MC_FP64 = 32               /* FP64 pair arithmetic for MMO_SHIFTED (parity mode)  */
+};
+/* Without MMO_MC_HARD_ROI the reference never reaches its acceptance test: OCaml binds
+ * the `else` of src/lds.ml:917 to the inner `if dist_roi > out_radius`.  The default
+ * reproduces that; MMO_MC_METROPOLIS_ALWAYS is the evident intent (not the reference). */
+int mmo_mc_create(mmo_ctx* ctx, const int32_t* chain_lig, int nchain, const double* rot0,
+                  const double* pos0, int inter_mode, int flags, const double* roi_center,
+                  double roi_r, double temperature, uint64_t seed, int total_frames,
+                  mmo_mc** out);
+/* Advances every unfinished chain by nsteps frames in one persistent launch. */
+int mmo_mc_run(mmo_mc* mc, int nsteps, void* stream);
+/* Same launch captured once in a hipGraph (queue reset + kernel) and replayed
+ * nreplay times: nreplay * nsteps frames per chain. */
+int mmo_mc_run_graph(mmo_mc* mc, int nsteps, int nreplay, void* stream);
+/* Plain form for comparison: one launch per ligand (its chains only). */
+int mmo_mc_run_per_ligand(mmo_mc* mc, int nsteps, void* stream);
+int mmo_mc_natoms(const mmo_mc* mc);
+/* Per chain (host arrays, each nullable): best_e[k]; cur_e[3k..] = E, E_inter,
+ * E_intra of the current state; best pose rotation and position; info[8k..] = frame,
+ * status (0 running, 1 done, 2 stopped by Mol.Too_long), accepted, resets, out-of-ROI
+ * resets, rigid steps, conformer steps, initialised; steps[2k..] = max_rot, max_trans. */
+int mmo_mc_get(mmo_mc* mc, double* best_e, double* cur_e, double* best_rot, double* best_pos,
+               int32_t* info, double* steps);
+/* Current conformers, best poses (rows of all chains, chain order) and torsion steps. */
+int mmo_mc_coords(mmo_mc* mc, double* conf, double* best_xyz, double* rb_dr);
+void mmo_mc_destroy(mmo_mc* mc);
+
 /* ------------------------------------------------------------------ tuning */
-/* Kernel time of the last launch of `kernel` ("inter", "interp", "intra") measured
+/* Kernel time of the last launch of `kernel` ("inter", "interp", "intra", "grid") measured
  * with HIP events on the launch stream, in milliseconds (-1 if none). */
 double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
 /* Fixed-point scale exponent k (coordinates are stored as round(x * 2^k)). */
 int mmo_fixed_point_bits(const mmo_ctx* ctx);
 /* Kernel shape parameters, set by the autotuner (mmo_amd/tune.py): "inter_k",
  * "inter_waves", "inter_rtile", "cull_k", "cull_waves", "scan_cull", "cell_size"
- * (the last applies to the next mmo_set_receptor).  No counterpart in the reference. */
+ * (applies to the next mmo_set_receptor), "mc_waves_per_cu".  No counterpart in the
+ * reference. */
 int mmo_tune_set(mmo_ctx* ctx, const char* key, double value);
 double mmo_tune_get(mmo_ctx* ctx, const char* key);
 

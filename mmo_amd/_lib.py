@@ -16,6 +16,7 @@ ERR_NO_RECEPTOR, ERR_NO_GRID, ERR_FORK, ERR_RANGE = -5, -6, -7, -8
 GLOBAL, SHIFTED, INTERP = 0, 1, 2
 F_INTER_SHIFTED, F_INTER_GLOBAL, F_INTER_INTERP = 1, 2, 4
 F_INTRA, F_GRAD, F_COMPONENTS = 8, 16, 32
+MC_TWEAK_RBONDS, MC_HARD_ROI, MC_INTRA_UFF, MC_METROPOLIS_ALWAYS, MC_NO_FLIP, MC_FP64 = 1, 2, 4, 8, 16, 32
 
 _d, _i, _p = ct.c_double, ct.c_int, ct.c_void_p
 _pp = ct.POINTER(ct.c_void_p)
@@ -29,6 +30,7 @@ SIGNATURES = {
     "mmo_set_uff_table": (_i, [_p, _p, _p, _p, _i]),
     "mmo_set_receptor": (_i, [_p, _p, _p, _p, _p, _p, _i, _p]),
     "mmo_set_grids": (_i, [_p, _p, _i, _d, _i, _i, _i, _p]),
+    "mmo_grid_compute": (_i, [_p, _p, _p, _i, _d, _i, _i, _i, _p, _p, _d, _d, _p]),
     "mmo_ene_inter": (_i, [_p, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p]),
     "mmo_ene_inter_interp": (_i, [_p, _p, _p, _p, _p, _i, _p]),
     "mmo_ene_intra": (_i, [_p, _p, _p, _p, _p, _p, _p, _i, _p]),
@@ -47,6 +49,14 @@ SIGNATURES = {
     "mmo_batch_score": (_i, [_p, _i, _p, _p, _p]),
     "mmo_batch_score_dofs_dev": (_i, [_p, _i, _p, _p, _p, _p, _p, _p]),
     "mmo_scan_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _p]),
+    "mmo_mc_create": (_i, [_p, _p, _i, _p, _p, _i, _i, _p, _d, _d, ct.c_uint64, _i, _pp]),
+    "mmo_mc_run": (_i, [_p, _i, _p]),
+    "mmo_mc_run_graph": (_i, [_p, _i, _i, _p]),
+    "mmo_mc_run_per_ligand": (_i, [_p, _i, _p]),
+    "mmo_mc_natoms": (_i, [_p]),
+    "mmo_mc_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
+    "mmo_mc_coords": (_i, [_p, _p, _p, _p]),
+    "mmo_mc_destroy": (None, [_p]),
     "mmo_last_kernel_ms": (_d, [_p, ct.c_char_p]),
     "mmo_fixed_point_bits": (_i, [_p]),
     "mmo_tune_set": (_i, [_p, ct.c_char_p, _d]),

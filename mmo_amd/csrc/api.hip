@@ -18,6 +18,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "mc.h"
 #include "mmo_ff.h"
 
 using namespace mmo;
@@ -84,6 +85,7 @@ struct mmo_ctx {
   bool cull_ok = false;
   int scan_cull = 1;
   int cull_k = 1;
+  int mc_waves_per_cu = 8;
   int cull_waves = 16;
   CellGrid cg{};
   DevBuf<int> d_cell;
@@ -116,6 +118,23 @@ struct mmo_ctx {
   int inter_k = 4, inter_waves = 4, inter_rtile = 2048;
   std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> ev;
   std::map<std::string, float> last_ms;   // -2: pending
+};
+
+struct mmo_mc {
+  mmo_ctx* ctx = nullptr;
+  int nchain = 0, natoms = 0, nrb = 0, max_atoms = 0, inter_mode = MMO_SHIFTED, flags = 0;
+  int total_frames = 0;
+  uint64_t seed = 0;
+  double beta = 0.0, roi[3] = {0.0, 0.0, 0.0}, roi_r = 0.0;
+  std::vector<int> chain_lig, chain_atom, chain_rb, work, work_by_lig;
+  std::vector<std::pair<int, int>> groups;   // (first, count) in work_by_lig per ligand
+  DevBuf<int> d_chain_lig, d_chain_atom, d_chain_rb, d_work, d_work_by_lig, d_queue;
+  DevBuf<McState> d_st;
+  DevBuf<double> d_conf, d_best, d_dr;
+  DevBuf<SwState> d_sw;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
 };
 
 struct mmo_batch {
@@ -377,7 +396,7 @@ int mmo_ctx_create(int device, mmo_ctx** out) {
     delete c;
     return fail(nullptr, MMO_ERR_DEVICE, "stream creation failed");
   }
-  for (const char* k : {"inter", "interp", "intra"}) {
+  for (const char* k : {"inter", "interp", "intra", "grid", "mc"}) {
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
       delete c;
@@ -865,8 +884,8 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
   if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
     const int mode = (inter & MMO_F_INTER_SHIFTED) ? MMO_SHIFTED : MMO_GLOBAL;
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
-    if (mode == MMO_SHIFTED && c->cull_ok && b->ntiles >= 1024) {
-      L.cull = 1;
+    if (mode == MMO_SHIFTED && b->ntiles >= 1024) {
+      L.cull = c->cull_ok ? 1 : 2;
       L.cull_waves = std::max(1, std::min(c->cull_waves, b->ntiles / 512));
     }
     // few tiles: split the receptor over block rows so that the chip fills up
@@ -1048,7 +1067,7 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     timer_end(c, "interp", s);
   } else {
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
-    L.cull = (mode == MMO_SHIFTED && c->cull_ok && c->scan_cull) ? 1 : 0;
+    L.cull = (mode == MMO_SHIFTED && c->scan_cull) ? (c->cull_ok ? 1 : 2) : 0;
     timer_begin(c, "inter", s);
     HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "inter", s);
@@ -1057,6 +1076,328 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
   // caller may reuse them
   HIPCHK(c, hipStreamSynchronize(s));
   return MMO_OK;
+}
+
+// ------------------------------------------------------------------ grid precompute
+int mmo_grid_compute(mmo_ctx* c, const int32_t* probe_anum, const double* probe_q, int ntypes,
+                     double step, int nx, int ny, int nz, const double* origin,
+                     const double* roi_center, double roi_r, double max_e, float* out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (!probe_anum || !probe_q || ntypes <= 0 || !(step > 0.0) || nx < 2 || ny < 2 || nz < 2 ||
+      !origin || !(max_e > 0.0))
+    return fail(c, MMO_ERR_BAD_ARG, "bad grid arguments");
+  const size_t nvox = (size_t)nx * ny * nz;
+  std::vector<double4> probe(ntypes);
+  for (int t = 0; t < ntypes; ++t) {
+    double A, B;
+    if ((r = atom_ab(c, probe_anum[t], &A, &B))) return r;
+    if (!std::isfinite(probe_q[t])) return fail(c, MMO_ERR_BAD_ARG, "probe %d: charge not finite", t);
+    probe[t] = make_double4(kElecWeight * probe_q[t], A, B, 0.0);
+  }
+  hipStream_t s = c->stream;
+  if ((r = upload(c, c->s_ppar, probe.data(), probe.size(), s))) return r;
+  HIPCHK(c, c->grid.ensure(nvox * ntypes));
+  GridArgs a{};
+  a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p; a.nrec = c->nrec;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.nvox = nvox;
+  a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2]; a.step = step;
+  if (roi_center) {
+    a.roi_x = roi_center[0]; a.roi_y = roi_center[1]; a.roi_z = roi_center[2];
+    a.roi_r2 = roi_r * roi_r;
+  } else {
+    a.roi_r2 = -1.0;
+  }
+  a.probe = c->s_ppar.p; a.ntypes = ntypes; a.max_e = max_e; a.out = c->grid.p;
+  timer_begin(c, "grid", s);
+  HIPCHK(c, launch_grid(a, c->cg, c->cull_ok, s));
+  timer_end(c, "grid", s);
+  if (out) HIPCHK(c, hipMemcpyAsync(out, c->grid.p, nvox * ntypes * sizeof(float),
+                                    hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  GridDev g{};
+  g.data = c->grid.p;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nvox = nvox;
+  g.step = step;
+  g.ods = 1.0 / step;
+  g.ox = origin[0]; g.oy = origin[1]; g.oz = origin[2];
+  c->g = g;
+  c->ntypes = ntypes;
+  c->has_grid = true;
+  return MMO_OK;
+}
+
+// ------------------------------------------------------------------ Monte Carlo
+int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double* rot0,
+                  const double* pos0, int inter_mode, int flags, const double* roi_center,
+                  double roi_r, double temperature, uint64_t seed, int total_frames,
+                  mmo_mc** out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!chain_lig || nchain <= 0 || !rot0 || !pos0 || !out || total_frames < 0 ||
+      !(temperature > 0.0))
+    return fail(c, MMO_ERR_BAD_ARG, "bad Monte Carlo arguments");
+  if (inter_mode == MMO_SHIFTED) {
+    if (!c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  } else if (inter_mode == MMO_INTERP) {
+    if (!c->has_grid) return fail(c, MMO_ERR_NO_GRID, "no grids set");
+  } else {
+    return fail(c, MMO_ERR_BAD_ARG, "Monte Carlo E_inter form must be SHIFTED or INTERP");
+  }
+  if ((flags & kMcHardRoi) && (!roi_center || !(roi_r > 0.0)))
+    return fail(c, MMO_ERR_BAD_ARG, "hard ROI needs a centre and a radius");
+  if ((r = sync_ligands(c))) return r;
+  *out = nullptr;
+  mmo_mc* m = new mmo_mc();
+  m->ctx = c;
+  m->nchain = nchain;
+  m->inter_mode = inter_mode;
+  m->flags = flags;
+  m->total_frames = total_frames;
+  m->seed = seed;
+  m->beta = 1.0 / (0.0019872041 * temperature);   // src/lds.ml:67-68, src/const.ml:97
+  if (roi_center) for (int k = 0; k < 3; ++k) m->roi[k] = roi_center[k];
+  m->roi_r = roi_r;
+  m->chain_lig.assign(chain_lig, chain_lig + nchain);
+  m->chain_atom.resize(nchain);
+  m->chain_rb.resize(nchain);
+  std::vector<double> conf;
+  std::vector<McState> st(nchain);
+  const double deg = (4.0 * std::atan(1.0)) / 180.0;   // Math.to_radian
+  for (int k = 0; k < nchain; ++k) {
+    const int l = chain_lig[k];
+    if (l < 0 || l >= (int)c->ligs.size()) {
+      delete m;
+      return fail(c, MMO_ERR_BAD_ARG, "chain %d: unknown ligand %d", k, l);
+    }
+    const LigHost& h = c->ligs[l];
+    if ((int)h.rb_left.size() > kMcMaxRb) {
+      delete m;
+      return fail(c, MMO_ERR_BAD_ARG, "ligand %d has more than %d rotatable bonds", l, kMcMaxRb);
+    }
+    if (inter_mode == MMO_INTERP)
+      for (int t : h.type)
+        if (t >= c->ntypes) {
+          delete m;
+          return fail(c, MMO_ERR_BAD_ARG, "ligand %d: grid type %d >= %d", l, t, c->ntypes);
+        }
+    m->chain_atom[k] = m->natoms;
+    m->chain_rb[k] = m->nrb;
+    m->natoms += h.n;
+    m->nrb += (int)h.rb_left.size();
+    m->max_atoms = std::max(m->max_atoms, h.n);
+    conf.insert(conf.end(), h.xyz.begin(), h.xyz.end());
+    McState& s0 = st[k];
+    std::memset(&s0, 0, sizeof(McState));
+    for (int q = 0; q < 9; ++q) s0.rot0[q] = s0.rot[q] = rot0[9 * k + q];
+    for (int q = 0; q < 3; ++q) s0.pos0[q] = s0.pos[q] = pos0[3 * k + q];
+    s0.max_rot = 15.0 * deg;     // Params.max_rot, src/params.ml:10-11
+    s0.max_trans = 0.15;         // Params.max_trans, src/params.ml:13-14
+    s0.status = kMcRunning;
+  }
+  std::vector<double> dr(std::max(m->nrb, 1), 5.0 * deg);   // Params.max_rbond_rot
+  std::vector<SwState> sw(std::max(m->nrb, 1));
+  std::memset(sw.data(), 0, sw.size() * sizeof(SwState));
+  // processing order: most expensive chains first (atoms and intra pairs)
+  m->work.resize(nchain);
+  for (int k = 0; k < nchain; ++k) m->work[k] = k;
+  auto cost = [&](int k) {
+    const LigHost& h = c->ligs[chain_lig[k]];
+    return (double)h.n * 64.0 + (double)h.pair_ij.size();
+  };
+  std::stable_sort(m->work.begin(), m->work.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  // per-ligand groups (the one-launch-per-ligand form)
+  m->work_by_lig.resize(nchain);
+  for (int k = 0; k < nchain; ++k) m->work_by_lig[k] = k;
+  std::stable_sort(m->work_by_lig.begin(), m->work_by_lig.end(),
+                   [&](int a, int b) { return chain_lig[a] < chain_lig[b]; });
+  for (int k = 0; k < nchain;) {
+    int e = k;
+    while (e < nchain && chain_lig[m->work_by_lig[e]] == chain_lig[m->work_by_lig[k]]) ++e;
+    m->groups.push_back({k, e - k});
+    k = e;
+  }
+  hipStream_t s = c->stream;
+  if ((r = upload(c, m->d_chain_lig, m->chain_lig.data(), nchain, s)) ||
+      (r = upload(c, m->d_chain_atom, m->chain_atom.data(), nchain, s)) ||
+      (r = upload(c, m->d_chain_rb, m->chain_rb.data(), nchain, s)) ||
+      (r = upload(c, m->d_work, m->work.data(), nchain, s)) ||
+      (r = upload(c, m->d_work_by_lig, m->work_by_lig.data(), nchain, s)) ||
+      (r = upload(c, m->d_st, st.data(), st.size(), s)) ||
+      (r = upload(c, m->d_conf, conf.data(), conf.size(), s)) ||
+      (r = upload(c, m->d_best, conf.data(), conf.size(), s)) ||
+      (r = upload(c, m->d_dr, dr.data(), dr.size(), s)) ||
+      (r = upload(c, m->d_sw, sw.data(), sw.size(), s))) {
+    delete m;
+    return r;
+  }
+  hipError_t e = m->d_queue.ensure(1);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    delete m;
+    return fail(c, MMO_ERR_DEVICE, "Monte Carlo setup: %s", hipGetErrorString(e));
+  }
+  *out = m;
+  return MMO_OK;
+}
+
+static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
+  mmo_ctx* c = m->ctx;
+  McArgs a{};
+  a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.rb_left = c->d_rb_left.p;
+  a.rb_right = c->d_rb_right.p; a.rg_off = c->d_rg_off.p; a.rg_atoms = c->d_rg_atoms.p;
+  a.pair_ij = c->d_pair_ij.p; a.pair_par = c->d_pair_par.p; a.max_atoms = m->max_atoms;
+  a.nwork = nwork; a.work = work; a.chain_lig = m->d_chain_lig.p; a.chain_atom = m->d_chain_atom.p;
+  a.chain_rb = m->d_chain_rb.p; a.st = m->d_st.p; a.conf = m->d_conf.p; a.best_xyz = m->d_best.p;
+  a.rb_dr = m->d_dr.p; a.rb_sw = m->d_sw.p; a.queue = m->d_queue.p;
+  a.nsteps = nsteps; a.total_frames = m->total_frames; a.flags = m->flags;
+  a.seed_lo = (uint32_t)m->seed; a.seed_hi = (uint32_t)(m->seed >> 32);
+  a.beta = m->beta;
+  const double deg = (4.0 * std::atan(1.0)) / 180.0;
+  a.max_rot0 = 15.0 * deg; a.max_trans0 = 0.15;
+  a.roi_x = m->roi[0]; a.roi_y = m->roi[1]; a.roi_z = m->roi[2]; a.roi_r = m->roi_r;
+  a.inter_mode = m->inter_mode; a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p;
+  a.cg = c->cg; a.g = c->g;
+  return a;
+}
+
+static int mc_blocks(mmo_mc* m, int nwork) {
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  return std::max(1, std::min(nwork, ncu * m->ctx->mc_waves_per_cu));
+}
+
+int mmo_mc_run(mmo_mc* m, int nsteps, void* stream) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (nsteps <= 0) return fail(c, MMO_ERR_BAD_ARG, "nsteps must be positive");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  McArgs a = mc_args(m, nsteps, m->nchain, m->d_work.p);
+  timer_begin(c, "mc", s);
+  HIPCHK(c, launch_mc(a, mc_blocks(m, m->nchain), s));
+  timer_end(c, "mc", s);
+  return MMO_OK;
+}
+
+int mmo_mc_run_graph(mmo_mc* m, int nsteps, int nreplay, void* stream) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (nsteps <= 0 || nreplay <= 0) return fail(c, MMO_ERR_BAD_ARG, "bad graph arguments");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  if (m->gexec && m->graph_steps != nsteps) {
+    (void)hipGraphExecDestroy(m->gexec);
+    (void)hipGraphDestroy(m->graph);
+    m->gexec = nullptr;
+    m->graph = nullptr;
+  }
+  if (!m->gexec) {
+    // capture queue reset + persistent launch on the context's own stream
+    McArgs a = mc_args(m, nsteps, m->nchain, m->d_work.p);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_mc(a, mc_blocks(m, m->nchain), c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess)
+      return fail(c, MMO_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    m->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
+    m->graph_steps = nsteps;
+  }
+  if (s != c->stream) {
+    // order the replays after the caller's prior work
+    hipEvent_t ev;
+    HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(ev, s));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, ev, 0));
+    (void)hipEventDestroy(ev);
+  }
+  timer_begin(c, "mc", c->stream);
+  for (int k = 0; k < nreplay; ++k) HIPCHK(c, hipGraphLaunch(m->gexec, c->stream));
+  timer_end(c, "mc", c->stream);
+  if (s != c->stream) {
+    hipEvent_t ev;
+    HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(ev, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(s, ev, 0));
+    (void)hipEventDestroy(ev);
+  }
+  return MMO_OK;
+}
+
+int mmo_mc_run_per_ligand(mmo_mc* m, int nsteps, void* stream) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (nsteps <= 0) return fail(c, MMO_ERR_BAD_ARG, "nsteps must be positive");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  timer_begin(c, "mc", s);
+  for (const auto& gr : m->groups) {
+    McArgs a = mc_args(m, nsteps, gr.second, m->d_work_by_lig.p + gr.first);
+    HIPCHK(c, launch_mc(a, mc_blocks(m, gr.second), s));
+  }
+  timer_end(c, "mc", s);
+  return MMO_OK;
+}
+
+int mmo_mc_natoms(const mmo_mc* m) { return m ? m->natoms : MMO_ERR_BAD_ARG; }
+
+int mmo_mc_get(mmo_mc* m, double* best_e, double* cur_e, double* best_rot, double* best_pos,
+               int32_t* info, double* steps) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  std::vector<McState> st(m->nchain);
+  HIPCHK(c, hipDeviceSynchronize());
+  HIPCHK(c, hipMemcpy(st.data(), m->d_st.p, st.size() * sizeof(McState), hipMemcpyDeviceToHost));
+  for (int k = 0; k < m->nchain; ++k) {
+    const McState& x = st[k];
+    if (best_e) best_e[k] = x.best_e;
+    if (cur_e) { cur_e[3 * k] = x.prev_e; cur_e[3 * k + 1] = x.prev_e_inter; cur_e[3 * k + 2] = x.prev_e_intra; }
+    if (best_rot) for (int q = 0; q < 9; ++q) best_rot[9 * k + q] = x.best_rot[q];
+    if (best_pos) for (int q = 0; q < 3; ++q) best_pos[3 * k + q] = x.best_pos[q];
+    if (info) {
+      int32_t* o = info + 8 * k;
+      o[0] = x.frame; o[1] = x.status; o[2] = x.accepted; o[3] = x.resets; o[4] = x.ooroi;
+      o[5] = x.rigid_step; o[6] = x.conf_step; o[7] = x.initialized;
+    }
+    if (steps) { steps[2 * k] = x.max_rot; steps[2 * k + 1] = x.max_trans; }
+  }
+  return MMO_OK;
+}
+
+int mmo_mc_coords(mmo_mc* m, double* conf, double* best_xyz, double* rb_dr) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  HIPCHK(c, hipDeviceSynchronize());
+  if (conf) HIPCHK(c, hipMemcpy(conf, m->d_conf.p, 3 * (size_t)m->natoms * sizeof(double), hipMemcpyDeviceToHost));
+  if (best_xyz) HIPCHK(c, hipMemcpy(best_xyz, m->d_best.p, 3 * (size_t)m->natoms * sizeof(double), hipMemcpyDeviceToHost));
+  if (rb_dr && m->nrb) HIPCHK(c, hipMemcpy(rb_dr, m->d_dr.p, (size_t)m->nrb * sizeof(double), hipMemcpyDeviceToHost));
+  return MMO_OK;
+}
+
+void mmo_mc_destroy(mmo_mc* m) {
+  if (!m) return;
+  if (getpid() == m->ctx->pid) {
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->d_chain_lig.release(); m->d_chain_atom.release(); m->d_chain_rb.release();
+    m->d_work.release(); m->d_work_by_lig.release(); m->d_queue.release(); m->d_st.release();
+    m->d_conf.release(); m->d_best.release(); m->d_dr.release(); m->d_sw.release();
+  }
+  delete m;
 }
 
 double mmo_last_kernel_ms(mmo_ctx* c, const char* kernel) {
@@ -1091,6 +1432,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
   else if (k == "scan_cull" && (iv == 0 || iv == 1)) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
+  else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
   return MMO_OK;
 }
@@ -1106,6 +1448,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "scan_cull") return c->scan_cull;
   if (k == "cell_size") return c->cell_size;
   if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;
+  if (k == "mc_waves_per_cu") return c->mc_waves_per_cu;
   return -1.0;
 }
 

@@ -13,6 +13,7 @@
 //             src/mol.ml:1012-1020, FP64 weights.
 // Every kernel has an adjoint (GRAD) variant giving dE/dxyz of the ligand atoms.
 #include "common.h"
+#include "cull.h"
 #include "kernels.h"
 
 namespace mmo {
@@ -91,30 +92,6 @@ struct InterArgs {
 };
 
 // --------------------------------------------------------------- pair arithmetic
-template <int MODE>
-__device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                            int lz, float inv_s2, float& sq, float& s12,
-                                            float& s6) {
-  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
-  const float r2c = fmaxf(r2, kMinR2);
-  const float ir = __builtin_amdgcn_rsqf(r2c);
-  const float u = ir * ir;
-  const float u3 = u * u * u;
-  const float u6 = u3 * u3;
-  if (MODE == MMO_SHIFTED) {
-    const float t = shift_t(r2c);
-    const float w = t * t;
-    sq = fmaf(w * p.q, ir, sq);
-    s12 = fmaf(w * ab.a, u6, s12);
-    s6 = fmaf(w * ab.b, u3, s6);
-  } else {
-    sq = fmaf(p.q, ir, sq);
-    s12 = fmaf(ab.a, u6, s12);
-    s6 = fmaf(ab.b, u3, s6);
-  }
-}
-
 // energy + dE/dr2 contribution; gradient accumulated as coef * (x_rec - x_lig) in
 // fixed-point units (caller scales by -2 * 2^-k).
 template <int MODE>
@@ -291,40 +268,13 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   }
 }
 
-// ------------------------------------------------------- inter kernel, culled
-// Shifted form only.  The whole receptor sits in LDS sorted by 4 A cells.  For each
-// block of K atoms per lane the wavefront takes the bounding box of its lanes' atom
-// positions and visits only the cell rows that intersect the sphere of radius
-// 12 A + half-diagonal around the box centre; every receptor atom within 12 A of any
-// of the wavefront's atoms is in a visited cell, and pairs beyond 12 A that are
-// visited contribute exactly 0 (the shift weight is clamped to 0).
-__device__ __forceinline__ float wave_minf(float v) {
-  for (int off = 1; off < kWave; off <<= 1) v = fminf(v, __shfl_xor(v, off, kWave));
-  return v;
-}
-__device__ __forceinline__ float wave_maxf(float v) {
-  for (int off = 1; off < kWave; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-  return v;
-}
-__device__ __forceinline__ float uniformf(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-__device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-template <int K, class Src>
+template <int K, bool LDS, class Src>
 __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid cg) {
   extern __shared__ __align__(16) char smem[];
-  const int nrec = a.nrec;
-  const int ncell = cg.nx * cg.ny * cg.nz;
-  RecPos* s_pos = reinterpret_cast<RecPos*>(smem);
-  RecAB* s_ab = reinterpret_cast<RecAB*>(smem + (size_t)nrec * sizeof(RecPos));
-  int* s_cell = reinterpret_cast<int*>(smem + (size_t)nrec * (sizeof(RecPos) + sizeof(RecAB)));
-  for (int i = threadIdx.x; i < nrec; i += blockDim.x) {
-    s_pos[i] = a.rpos[i];
-    s_ab[i] = a.rab[i];
-  }
-  for (int i = threadIdx.x; i <= ncell; i += blockDim.x) s_cell[i] = cg.cell_start[i];
-  __syncthreads();
+  const RecPos* s_pos;
+  const RecAB* s_ab;
+  const int* s_cell;
+  load_receptor<LDS>(a.rpos, a.rab, a.nrec, cg, s_pos, s_ab, s_cell, smem);
 
   const int lane = threadIdx.x & (kWave - 1);
   const int tidx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
@@ -343,13 +293,13 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
 
   for (int blk = 0; blk < nblk; ++blk) {
     int lx[K], ly[K], lz[K];
+    bool valid[K];
     double cq[K], ca[K], cb[K];
-    float mnx = 3.0e38f, mny = 3.0e38f, mnz = 3.0e38f;
-    float mxx = -3.0e38f, mxy = -3.0e38f, mxz = -3.0e38f;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = aslot + split * (blk * K + k);
-      if (pose_ok && j < li.n_atoms) {
+      valid[k] = pose_ok && j < li.n_atoms;
+      if (valid[k]) {
         double x, y, z;
         a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
         lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
@@ -357,64 +307,22 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
         lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
         const LigAtom la = a.latoms[li.atom_base + j];
         cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
-        const float fx = (float)lx[k] * fr.inv_s, fy = (float)ly[k] * fr.inv_s,
-                    fz = (float)lz[k] * fr.inv_s;
-        mnx = fminf(mnx, fx); mny = fminf(mny, fy); mnz = fminf(mnz, fz);
-        mxx = fmaxf(mxx, fx); mxy = fmaxf(mxy, fy); mxz = fmaxf(mxz, fz);
-      } else {
-        // inert atom (zero parameters) parked at the first valid position's box
+      } else {   // inert: zero parameters
         lx[k] = ly[k] = lz[k] = 0;
         cq[k] = ca[k] = cb[k] = 0.0;
       }
     }
-    mnx = uniformf(wave_minf(mnx)); mny = uniformf(wave_minf(mny)); mnz = uniformf(wave_minf(mnz));
-    mxx = uniformf(wave_maxf(mxx)); mxy = uniformf(wave_maxf(mxy)); mxz = uniformf(wave_maxf(mxz));
-    if (!(mnx <= mxx)) continue;   // no atom in this block on any lane
-    const float qx = 0.5f * (mnx + mxx), qy = 0.5f * (mny + mxy), qz = 0.5f * (mnz + mxz);
-    const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
-    const float R = uniformf(12.0f + sqrtf(hx * hx + hy * hy + hz * hz) + 0.05f);
-    const float R2 = R * R;
-    const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
-    const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
-    for (int cz = cz0; cz <= cz1; ++cz) {
-      const float zlo = cg.z0 + cz * cg.c;
-      const float dz = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
-      const float rz2 = R2 - dz * dz;
-      if (rz2 < 0.0f) continue;
-      const float ry = sqrtf(rz2);
-      const int cy0 = max(0, (int)floorf((qy - ry - cg.y0) * cg.inv_c));
-      const int cy1 = min(cg.ny - 1, (int)floorf((qy + ry - cg.y0) * cg.inv_c));
-      float sq[K], s12[K], s6[K];
+    float qx, qy, qz, R;
+    if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
+    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+                  [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
-      for (int cy = cy0; cy <= cy1; ++cy) {
-        const float ylo = cg.y0 + cy * cg.c;
-        const float dy = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
-        const float rx2 = rz2 - dy * dy;
-        if (rx2 < 0.0f) continue;
-        const float rx = sqrtf(rx2);
-        const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
-        const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
-        if (cx0 > cx1) continue;
-        const int rowb = (cz * cg.ny + cy) * cg.nx;
-        const int a0 = uniformi(s_cell[rowb + cx0]);
-        const int a1 = uniformi(s_cell[rowb + cx1 + 1]);
-#pragma unroll 2
-        for (int i = a0; i < a1; ++i) {
-          const RecPos p = s_pos[i];
-          const RecAB ab = s_ab[i];
-#pragma unroll
-          for (int k = 0; k < K; ++k)
-            pair_energy<MMO_SHIFTED>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, sq[k], s12[k], s6[k]);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        e_el = fma((double)sq[k], cq[k], e_el);
-        e_vd = fma((double)s12[k], ca[k], e_vd);
-        e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
-      }
-    }
+                    for (int k = 0; k < K; ++k) {
+                      e_el = fma((double)sq[k], cq[k], e_el);
+                      e_vd = fma((double)s12[k], ca[k], e_vd);
+                      e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
+                    }
+                  });
   }
   e_el = group_sum(e_el, split);
   e_vd = group_sum(e_vd, split);
@@ -426,6 +334,63 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
       a.e_comp[o + 2] = e_el;
       a.e_comp[o + 3] = e_vd;
     }
+  }
+}
+
+// ------------------------------------------------------------- grid precompute
+// Per-type FP32 grids for single-atom probes (anum_t, q_t) at every grid point:
+// E_t = min(max_E, ke q_t Sq + A_t S12 - 2 B_t S6) with the receptor sums of the
+// shifted form (Mol.ene_inter_UFF_shifted_grid src/mol.ml:964-989, clamp
+// src/lds.ml:466).  Points outside the ROI mask stay 0 (src/G3D.ml:47-51,
+// src/lds.ml:269-305).  A wavefront handles a 4x4x4 brick of points.
+template <bool LDS>
+__global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
+  extern __shared__ __align__(16) char smem[];
+  const RecPos* s_pos;
+  const RecAB* s_ab;
+  const int* s_cell;
+  load_receptor<LDS>(a.rpos, a.rab, a.nrec, cg, s_pos, s_ab, s_cell, smem);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int brick = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  const int bx = (a.nx + 3) / 4, by = (a.ny + 3) / 4, bz = (a.nz + 3) / 4;
+  if (brick >= bx * by * bz) return;
+  const int ib = brick % bx, jb = (brick / bx) % by, kb = brick / (bx * by);
+  const int i = ib * 4 + (lane & 3), j = jb * 4 + ((lane >> 2) & 3), k = kb * 4 + (lane >> 4);
+  const bool inside = i < a.nx && j < a.ny && k < a.nz;
+  // node position (Grid.t: node (i,j,k) at origin + (i,j,k) step, src/grid.ml:40-52)
+  const double x = a.ox + i * a.step, y = a.oy + j * a.step, z = a.oz + k * a.step;
+  bool valid[1];
+  valid[0] = inside;
+  if (inside && a.roi_r2 > 0.0) {
+    const double dx = a.roi_x - x, dy = a.roi_y - y, dz = a.roi_z - z;
+    valid[0] = (dx * dx + dy * dy + dz * dz) < a.roi_r2;
+  }
+  const Frame fr = a.fr;
+  int sat = 0;
+  int lx[1], ly[1], lz[1];
+  lx[0] = valid[0] ? to_fixed(x, fr.ox, fr.scale, fr.lim, &sat) : 0;
+  ly[0] = valid[0] ? to_fixed(y, fr.oy, fr.scale, fr.lim, &sat) : 0;
+  lz[0] = valid[0] ? to_fixed(z, fr.oz, fr.scale, fr.lim, &sat) : 0;
+  double Sq = 0.0, S12 = 0.0, S6 = 0.0;
+  float qx, qy, qz, R;
+  if (wave_query<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) {
+    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+                  [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
+                    Sq += (double)sq[0];
+                    S12 += (double)s12[0];
+                    S6 += (double)s6[0];
+                  });
+  }
+  if (!inside) return;
+  const size_t idx = (size_t)i + (size_t)j * a.nx + (size_t)k * a.nx * a.ny;
+  for (int t = 0; t < a.ntypes; ++t) {
+    float v = 0.0f;
+    if (valid[0]) {
+      const double4 p = a.probe[t];   // ke q_t, A_t, B_t
+      const double e = p.x * Sq + p.y * S12 - 2.0 * p.z * S6;
+      v = (float)(e < a.max_e ? e : a.max_e);
+    }
+    a.out[(size_t)t * a.nvox + idx] = v;
   }
 }
 
@@ -651,16 +616,24 @@ static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hip
   const int waves = L.cull_waves;
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
-  const size_t lds = (size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
-                     (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int);
+  const bool in_lds = L.cull == 1;   // 2: receptor read from global memory
+  const size_t lds = in_lds ? (size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+                                  (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int)
+                            : 0;
   // dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB per CU)
 #define MMO_CULL(KK)                                                                           \
   case KK: {                                                                                   \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inter_cull<KK, Src>),  \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \
-    if (e != hipSuccess) return e;                                                             \
-    hipLaunchKernelGGL((k_inter_cull<KK, Src>), dim3(blocks), dim3(waves * kWave), lds, s, a,  \
-                       L.cg);                                                                  \
+    if (in_lds) {                                                                              \
+      hipError_t e = hipFuncSetAttribute(                                                      \
+          reinterpret_cast<const void*>(&k_inter_cull<KK, true, Src>),                         \
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                               \
+      if (e != hipSuccess) return e;                                                           \
+      hipLaunchKernelGGL((k_inter_cull<KK, true, Src>), dim3(blocks), dim3(waves * kWave),     \
+                         lds, s, a, L.cg);                                                     \
+    } else {                                                                                   \
+      hipLaunchKernelGGL((k_inter_cull<KK, false, Src>), dim3(blocks), dim3(waves * kWave),    \
+                         0, s, a, L.cg);                                                       \
+    }                                                                                          \
     break;                                                                                     \
   }
   switch (L.cull_k) {
@@ -725,6 +698,23 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((k_interp<false, ScanSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s) {
+  const int bricks = ((a.nx + 3) / 4) * ((a.ny + 3) / 4) * ((a.nz + 3) / 4);
+  const int waves = 16;
+  const int blocks = (bricks + waves - 1) / waves;
+  if (in_lds) {
+    const size_t lds = (size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+                       (size_t)(cg.nx * cg.ny * cg.nz + 1) * sizeof(int);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grid<true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_grid<true>, dim3(blocks), dim3(waves * kWave), lds, s, a, cg);
+  } else {
+    hipLaunchKernelGGL(k_grid<false>, dim3(blocks), dim3(waves * kWave), 0, s, a, cg);
+  }
   return hipGetLastError();
 }
 

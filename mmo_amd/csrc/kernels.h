@@ -12,7 +12,7 @@ struct InterLaunch {
   int waves;           // wavefronts per block
   int rtile;           // receptor atoms per LDS tile
   int rslice;          // receptor atoms per block row (gridDim.y > 1 splits the receptor)
-  int cull;            // shifted form with the receptor cell list in LDS (k_inter_cull)
+  int cull;            // culled shifted form: 1 receptor cell list in LDS, 2 in global memory
   int cull_waves;      // wavefronts per block of the culled kernel
   int cull_k;          // ligand atoms per lane in the culled kernel (1, 2, 4)
   CellGrid cg;
@@ -55,6 +55,22 @@ struct IntraArgs {
   int grad_accumulate;
 };
 
+struct GridArgs {
+  Frame fr;
+  const RecPos* rpos;
+  const RecAB* rab;
+  int nrec;
+  int nx, ny, nz;
+  size_t nvox;
+  double ox, oy, oz, step;       // node (0,0,0) and spacing, caller frame
+  double roi_x, roi_y, roi_z;    // mask: points with |p - roi| < sqrt(roi_r2); roi_r2 <= 0: all
+  double roi_r2;
+  const double4* probe;          // per type: ke q, A, B
+  int ntypes;
+  double max_e;                  // clamp (Params.max_E = 1e5, src/params.ml:26)
+  float* out;                    // ntypes * nvox, index i + j nx + k nx ny
+};
+
 struct DofArgs {
   const LigInfo* ligs;
   const double* lig_xyz;        // centred conformers
@@ -90,6 +106,7 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
                               const double* trans, const int* tperm, int ntrans, double* e_out,
                               hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s);
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
 

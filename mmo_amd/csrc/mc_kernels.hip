@@ -1,0 +1,449 @@
+// mc_kernels.hip -- batched Monte Carlo chains on gfx950, one wavefront per chain.
+//
+// Restates Lds.simulate_lig (src/lds.ml:741-1000) for many ligands and chains in one
+// persistent launch: wavefronts take chains from a work queue (most expensive first)
+// and advance each by `nsteps` frames; the chain state lives in HBM between launches,
+// so a launch (or a hipGraph replay of it) continues where the last one stopped.
+//
+// Per frame (src/lds.ml:882-995):
+//   even frames: rigid move, rot' = R_axis(theta) rot (Move.rand_rot src/move.ml:20-31,
+//     passive axis matrices src/rot.ml:22-46), pos' = pos + dt U(-1,1)^3 (src/move.ml:46-54);
+//   odd frames: torsion move on a copy of the conformer, tweak U(-dr_i, dr_i) or, every
+//     100th conformer step, flip U(-pi, pi) (src/lds.ml:781-798, src/mol.ml:635-647);
+//     enclosing radius > 12 A (Mol.check_elongation_exn) ends the chain (the exception
+//     leaves the frame loop, src/lds.ml:996-998);
+//   pose x = rot' (conf' - centre(conf')) + pos' (src/mol.ml:705-710);
+//   E_intra (odd frames) and E_inter are stored before the acceptance test
+//   (src/lds.ml:900-904);
+//   hard ROI: a pose centre beyond the ROI radius resets the chain; otherwise
+//   E_inter == 0 resets it and Metropolis at beta = 1/(kB T) decides (src/lds.ml:906-954).
+//   OCaml attaches the `else` of src/lds.ml:917 to the inner `if dist_roi > out_radius`,
+//   so without hard ROI no move is ever accepted; MMO_MC_METROPOLIS_ALWAYS runs the
+//   E == 0 / Metropolis branch in that case as well (not the reference's behaviour).
+//   Best pose, adaptive steps (x0.95 / x1.05 outside the [0.45, 0.55] acceptance band
+//   over 100-event windows, src/lds.ml:586-621, :971-988).
+// Random numbers: Philox4x32-10 keyed by the seed, counter (chain, frame, draw).
+#include "common.h"
+#include "cull.h"
+#include "kernels.h"
+#include "mc.h"
+
+namespace mmo {
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC 2011).
+__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+// two uniform doubles in [0, 1) with 53 random bits each
+__device__ __forceinline__ void uniform2(uint4 r, double& a, double& b) {
+  a = ((double)(r.x >> 5) * 67108864.0 + (double)(r.y >> 6)) * (1.0 / 9007199254740992.0);
+  b = ((double)(r.z >> 5) * 67108864.0 + (double)(r.w >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// ------------------------------------------------------------------ sliding window
+// SW.t (src/SW.ml:7-42): last 100 accept/reject events; ratio = accepts / events.
+__device__ __forceinline__ void sw_process(SwState& w, bool ev) {
+  const int word = w.head >> 5, bit = w.head & 31;
+  if (w.count == kMcBlock) {
+    if ((w.bits[word] >> bit) & 1u) --w.acc;
+  } else {
+    ++w.count;
+  }
+  if (ev) { w.bits[word] |= 1u << bit; ++w.acc; }
+  else w.bits[word] &= ~(1u << bit);
+  w.head = (w.head + 1 == kMcBlock) ? 0 : w.head + 1;
+}
+__device__ __forceinline__ double sw_ratio(const SwState& w) {
+  return (double)w.acc / (double)w.count;   // 0/0 = NaN: no adaptation (comparisons false)
+}
+__device__ __forceinline__ void sw_reset(SwState& w) {
+  w.bits[0] = w.bits[1] = w.bits[2] = w.bits[3] = 0u;
+  w.head = w.count = w.acc = 0;
+}
+
+// Rot.of_axis_angle (src/rot.ml:121-146)
+__device__ __forceinline__ void axis_angle(double x, double y, double z, double th, double* m) {
+  double s, c;
+  sincos(th, &s, &c);
+  const double o = 1.0 - c;
+  m[0] = c + x * x * o;     m[1] = x * y * o - z * s; m[2] = x * z * o + y * s;
+  m[3] = x * y * o + z * s; m[4] = c + y * y * o;     m[5] = y * z * o - x * s;
+  m[6] = x * z * o - y * s; m[7] = y * z * o + x * s; m[8] = c + z * z * o;
+}
+
+// --------------------------------------------------------------- energies of a pose
+// pose atom j: rot (conf_j - centre) + pos
+struct PoseView {
+  const double* sx;
+  const double* sy;
+  const double* sz;
+  double m[9], cx, cy, cz, px, py, pz;
+  __device__ void at(int j, double& x, double& y, double& z) const {
+    const double a = sx[j] - cx, b = sy[j] - cy, c = sz[j] - cz;
+    x = (m[0] * a + m[1] * b + m[2] * c) + px;
+    y = (m[3] * a + m[4] * b + m[5] * c) + py;
+    z = (m[6] * a + m[7] * b + m[8] * c) + pz;
+  }
+};
+
+__device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const PoseView& v) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double e = 0.0;
+  int bad = 0;
+  const GridDev& g = a.g;
+  for (int j = lane; j < li.n_atoms; j += kWave) {
+    double x, y, z;
+    v.at(j, x, y, z);
+    const double px = x - g.ox, py = y - g.oy, pz = z - g.oz;
+    const double fx = px * g.ods, fy = py * g.ods, fz = pz * g.ods;
+    if (!(px >= 0.0 && py >= 0.0 && pz >= 0.0 && fx < (double)(g.nx - 1) &&
+          fy < (double)(g.ny - 1) && fz < (double)(g.nz - 1))) {
+      bad = 1;
+      continue;
+    }
+    const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
+    const float* arr = g.data + (size_t)a.latoms[li.atom_base + j].type * g.nvox;
+    const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
+    const size_t b = i0 + j0 * nx + k0 * nxy;
+    const double wlx = (px - i0 * g.step) * g.ods, wly = (py - j0 * g.step) * g.ods,
+                 wlz = (pz - k0 * g.step) * g.ods;
+    const double whx = 1.0 - wlx, why = 1.0 - wly, whz = 1.0 - wlz;
+    // corner order of G3D.trilin, src/G3D.ml:144-151
+    e += (double)arr[b] * (whx * why * whz) + (double)arr[b + 1] * (wlx * why * whz) +
+         (double)arr[b + 1 + nx] * (wlx * wly * whz) + (double)arr[b + nx] * (whx * wly * whz) +
+         (double)arr[b + nxy] * (whx * why * wlz) + (double)arr[b + 1 + nxy] * (wlx * why * wlz) +
+         (double)arr[b + 1 + nx + nxy] * (wlx * wly * wlz) +
+         (double)arr[b + nx + nxy] * (whx * wly * wlz);
+  }
+  e = wave_sum(e);
+  for (int off = 1; off < kWave; off <<= 1) bad |= __shfl_xor(bad, off, kWave);
+  return bad ? __builtin_nan("") : e;
+}
+
+// shifted direct sum, lanes over atoms, receptor cell rows culled by the pose's box
+__device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v) {
+  constexpr int K = 2;
+  const int lane = threadIdx.x & (kWave - 1);
+  const Frame fr = a.fr;
+  double e_el = 0.0, e_vd = 0.0;
+  int sat = 0;
+  const int nblk = (li.n_atoms + kWave * K - 1) / (kWave * K);
+  for (int blk = 0; blk < nblk; ++blk) {
+    int lx[K], ly[K], lz[K];
+    bool valid[K];
+    double cq[K], ca[K], cb[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = lane + kWave * (blk * K + k);
+      valid[k] = j < li.n_atoms;
+      if (valid[k]) {
+        double x, y, z;
+        v.at(j, x, y, z);
+        lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
+        ly[k] = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
+        lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+        const LigAtom la = a.latoms[li.atom_base + j];
+        cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
+      } else {
+        lx[k] = ly[k] = lz[k] = 0;
+        cq[k] = ca[k] = cb[k] = 0.0;
+      }
+    }
+    float qx, qy, qz, R;
+    if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
+    if (a.flags & kMcFp64) {
+      e_vd += cull_sweep_f64<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, 1.0 / fr.scale, qx, qy, qz,
+                                R, lx, ly, lz, cq, ca, cb);
+      continue;
+    }
+    cull_sweep<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+                      [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                          e_el = fma((double)sq[k], cq[k], e_el);
+                          e_vd = fma((double)s12[k], ca[k], e_vd);
+                          e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
+                        }
+                      });
+  }
+  return wave_sum(e_el + e_vd);
+}
+
+// E_intra of the conformer in LDS (rigid-motion invariant, so the pose is not needed);
+// src/mol.ml:881-903 over the precomputed pair list, FP64
+__device__ double mc_intra(const McArgs& a, const LigInfo& li, const double* sx, const double* sy,
+                           const double* sz) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double e = 0.0;
+  for (int p = lane; p < li.n_pairs; p += kWave) {
+    const uint32_t ij = a.pair_ij[li.pair_base + p];
+    const int i = ij & 0xffff, j = ij >> 16;
+    const double4 pp = a.pair_par[li.pair_base + p];
+    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = rr < 0.01 ? 0.01 : rr;
+    const double ir = 1.0 / r, u = ir * ir, u3 = u * u * u, u6 = u3 * u3;
+    e += pp.x * ir + pp.y * u6 - pp.z * u3;
+  }
+  return wave_sum(e);
+}
+
+__device__ double mc_inter(const McArgs& a, const LigInfo& li, const PoseView& v) {
+  return a.inter_mode == MMO_INTERP ? mc_inter_interp(a, li, v) : mc_inter_direct(a, li, v);
+}
+
+// --------------------------------------------------------------------- the kernel
+__global__ void __launch_bounds__(64) k_mc(McArgs a) {
+  extern __shared__ __align__(16) double mc_smem[];
+  const int lane = threadIdx.x;
+  double* cx_ = mc_smem;                       // current conformer
+  double* cy_ = cx_ + a.max_atoms;
+  double* cz_ = cy_ + a.max_atoms;
+  double* px_ = cz_ + a.max_atoms;             // proposed conformer
+  double* py_ = px_ + a.max_atoms;
+  double* pz_ = py_ + a.max_atoms;
+  __shared__ int s_chain;
+  const double pi = 3.14159265358979311600;    // 4 atan 1 (src/math.ml:13)
+
+  for (;;) {
+    if (lane == 0) s_chain = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int q = s_chain;
+    __syncthreads();
+    if (q >= a.nwork) break;
+    const int chain = a.work[q];
+    McState st = a.st[chain];
+    if (st.status != kMcRunning) continue;
+    const int lig = a.chain_lig[chain];
+    const LigInfo li = a.ligs[lig];
+    const int L = li.n_atoms, nrb = li.nrb;
+    double* gconf = a.conf + 3 * (size_t)a.chain_atom[chain];
+    double* gbest = a.best_xyz + 3 * (size_t)a.chain_atom[chain];
+    double* dr = a.rb_dr + a.chain_rb[chain];
+    SwState* rsw = a.rb_sw + a.chain_rb[chain];
+    for (int j = lane; j < L; j += kWave) {
+      cx_[j] = gconf[3 * j]; cy_[j] = gconf[3 * j + 1]; cz_[j] = gconf[3 * j + 2];
+    }
+    __syncthreads();
+    const bool flex = (a.flags & kMcTweak) && nrb > 0;
+    const bool use_intra = (a.flags & kMcIntraUff) != 0;
+
+    if (!st.initialized) {
+      // start pose: Mol.rotate_then_translate_copy centered_lig rot0 pos0, no
+      // re-centring (src/lds.ml:759, src/mol.ml:669-672); E of the start pose
+      PoseView v{cx_, cy_, cz_, {}, 0.0, 0.0, 0.0, st.pos0[0], st.pos0[1], st.pos0[2]};
+      for (int k = 0; k < 9; ++k) v.m[k] = st.rot0[k];
+      st.prev_e_intra = use_intra ? mc_intra(a, li, cx_, cy_, cz_) : 0.0;
+      st.prev_e_inter = mc_inter(a, li, v);
+      st.prev_e = st.prev_e_inter + st.prev_e_intra;
+      st.best_e = st.prev_e;
+      for (int k = 0; k < 9; ++k) { st.rot[k] = st.rot0[k]; st.best_rot[k] = st.rot0[k]; }
+      for (int k = 0; k < 3; ++k) { st.pos[k] = st.pos0[k]; st.best_pos[k] = st.pos0[k]; }
+      for (int j = lane; j < L; j += kWave) {
+        double x, y, z;
+        v.at(j, x, y, z);
+        gbest[3 * j] = x; gbest[3 * j + 1] = y; gbest[3 * j + 2] = z;
+      }
+      st.initialized = 1;
+    }
+
+    const int frame_end = min(st.frame + a.nsteps, a.total_frames);
+    for (; st.frame < frame_end; ++st.frame) {
+      const bool rigid = (st.frame & 1) == 0;
+      const uint4 r0 = philox(make_uint4((uint32_t)chain, (uint32_t)st.frame, 0u, 0x4D4D4F31u), a.seed_lo, a.seed_hi);
+      const uint4 r1 = philox(make_uint4((uint32_t)chain, (uint32_t)st.frame, 1u, 0x4D4D4F31u), a.seed_lo, a.seed_hi);
+      const uint4 r2 = philox(make_uint4((uint32_t)chain, (uint32_t)st.frame, 2u, 0x4D4D4F31u), a.seed_lo, a.seed_hi);
+      double u0, u1, u2, u3, u4, u5;
+      uniform2(r0, u0, u1);
+      uniform2(r1, u2, u3);
+      uniform2(r2, u4, u5);
+
+      // ---- proposal
+      double nrot[9], npos[3], ccx, ccy, ccz;
+      const double* sx = cx_;
+      const double* sy = cy_;
+      const double* sz = cz_;
+      int just_rotated = -1;
+      double ndr[kMcMaxRb];
+      for (int k = 0; k < nrb; ++k) ndr[k] = dr[k];
+      ccx = st.center[0]; ccy = st.center[1]; ccz = st.center[2];
+      if (!rigid) {
+        for (int k = 0; k < 9; ++k) nrot[k] = st.rot[k];
+        for (int k = 0; k < 3; ++k) npos[k] = st.pos[k];
+        if (flex) {
+          // Mol.flip_rbond / tweak_rbond on a copy (src/lds.ml:784-795, src/mol.ml:635-647)
+          int i = (int)(u0 * nrb);
+          i = i < nrb ? i : nrb - 1;
+          const bool flip = st.conf_step > 0 && st.conf_step % kMcBlock == 0 && !(a.flags & kMcNoFlip);
+          const double amp = flip ? pi : ndr[i];
+          const double ang = u1 * (2.0 * amp) - amp;   // Move.rand_angle, src/move.ml:14-16
+          for (int j = lane; j < L; j += kWave) { px_[j] = cx_[j]; py_[j] = cy_[j]; pz_[j] = cz_[j]; }
+          __syncthreads();
+          const int left = a.rb_left[li.rb_base + i], right = a.rb_right[li.rb_base + i];
+          const double ox = px_[right], oy = py_[right], oz = pz_[right];
+          const double vx = ox - px_[left], vy = oy - py_[left], vz = oz - pz_[left];
+          const double mag = sqrt(vx * vx + vy * vy + vz * vz);
+          double Rm[9];
+          axis_angle(vx / mag, vy / mag, vz / mag, ang, Rm);
+          __syncthreads();
+          const int g0 = a.rg_off[li.rg_base + i], g1 = a.rg_off[li.rg_base + i + 1];
+          for (int m = g0 + lane; m < g1; m += kWave) {
+            const int at = a.rg_atoms[m];
+            const double zx = px_[at] - ox, zy = py_[at] - oy, zz = pz_[at] - oz;
+            px_[at] = (Rm[0] * zx + Rm[1] * zy + Rm[2] * zz) + ox;
+            py_[at] = (Rm[3] * zx + Rm[4] * zy + Rm[5] * zz) + oy;
+            pz_[at] = (Rm[6] * zx + Rm[7] * zy + Rm[8] * zz) + oz;
+          }
+          __syncthreads();
+          // update_center (A.favg) and enclosing radius (src/mol.ml:353-356, :576-591)
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+          for (int j = lane; j < L; j += kWave) { s0 += px_[j]; s1 += py_[j]; s2 += pz_[j]; }
+          ccx = wave_sum(s0) / L; ccy = wave_sum(s1) / L; ccz = wave_sum(s2) / L;
+          double rmax = 0.0;
+          for (int j = lane; j < L; j += kWave) {
+            const double dx = ccx - px_[j], dy = ccy - py_[j], dz = ccz - pz_[j];
+            rmax = fmax(rmax, 0.01 + sqrt(dx * dx + dy * dy + dz * dz));
+          }
+          rmax = wave_max(rmax);
+          if (rmax > 12.0) {   // Mol.Too_long: the chain ends (src/lds.ml:794, :997)
+            st.status = kMcTooLong;
+            break;
+          }
+          sx = px_; sy = py_; sz = pz_;
+          just_rotated = i;
+        }
+      } else {
+        // Move.rand_rot: left-multiply the passive axis rotation Rot.rx / ry / rz
+        // (src/move.ml:20-31, src/rot.ml:22-46).  Two rows of rot mix,
+        // p' = c p + s q and q' = c q - s p, with (p, q) = rows (1, 2) for x, (2, 0) for y,
+        // (0, 1) for z.  Written with selects: a switch that built the 3x3 matrix per axis
+        // was mis-lowered by hipcc (the -s entry of the z matrix came out 0).
+        const double th = u0 * (2.0 * st.max_rot) - st.max_rot;
+        int ax = (int)(u1 * 3.0);
+        ax = ax < 3 ? ax : 2;
+        double s, c;
+        sincos(th, &s, &c);
+        const bool ax0 = ax == 0, ax1 = ax == 1;
+        for (int cc = 0; cc < 3; ++cc) {
+          const double r0 = st.rot[cc], r1 = st.rot[3 + cc], r2 = st.rot[6 + cc];
+          const double p = ax0 ? r1 : (ax1 ? r2 : r0);
+          const double q = ax0 ? r2 : (ax1 ? r0 : r1);
+          const double pn = c * p + s * q, qn = c * q - s * p;
+          nrot[cc] = ax0 ? r0 : (ax1 ? qn : pn);
+          nrot[3 + cc] = ax0 ? pn : (ax1 ? r1 : qn);
+          nrot[6 + cc] = ax0 ? qn : (ax1 ? pn : r2);
+        }
+        // Move.rand_trans (src/move.ml:46-54)
+        npos[0] = st.pos[0] + (u2 * 2.0 - 1.0) * st.max_trans;
+        npos[1] = st.pos[1] + (u3 * 2.0 - 1.0) * st.max_trans;
+        npos[2] = st.pos[2] + (u4 * 2.0 - 1.0) * st.max_trans;
+      }
+
+      // ---- energies (stored before the acceptance test, src/lds.ml:900-904)
+      PoseView v{sx, sy, sz, {}, ccx, ccy, ccz, npos[0], npos[1], npos[2]};
+      for (int k = 0; k < 9; ++k) v.m[k] = nrot[k];
+      if (!rigid && use_intra && flex) st.prev_e_intra = mc_intra(a, li, sx, sy, sz);
+      st.prev_e_inter = mc_inter(a, li, v);
+      const double curr_e = st.prev_e_inter + st.prev_e_intra;
+      const double dxr = a.roi_x - npos[0], dyr = a.roi_y - npos[1], dzr = a.roi_z - npos[2];
+      const double dist_roi = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
+      const bool hard = (a.flags & kMcHardRoi) != 0;
+      bool reset = false, metropolis = false;
+      if (hard) {
+        if (dist_roi > a.roi_r) { reset = true; ++st.ooroi; }
+        else metropolis = true;
+      } else {
+        metropolis = (a.flags & kMcMetropolisAlways) != 0;
+      }
+      if (metropolis && st.prev_e_inter == 0.0) { reset = true; metropolis = false; }
+      if (reset) {
+        // reset_run_params (src/lds.ml:632-648): the conformer is kept
+        st.max_rot = a.max_rot0;
+        st.max_trans = a.max_trans0;
+        for (int k = 0; k < 9; ++k) { st.rot[k] = st.rot0[k]; st.best_rot[k] = (k % 4 == 0) ? 1.0 : 0.0; }
+        for (int k = 0; k < 3; ++k) { st.pos[k] = st.pos0[k]; st.best_pos[k] = 0.0; }
+        st.prev_e = __builtin_inf();
+        st.best_e = __builtin_inf();
+        sw_reset(st.sw);
+        ++st.resets;
+      } else if (metropolis) {
+        const bool accept = (curr_e <= st.prev_e) || (u5 < exp(-(curr_e - st.prev_e) * a.beta));
+        if (rigid) sw_process(st.sw, accept);
+        else if (just_rotated > -1) sw_process(rsw[just_rotated], accept);
+        if (accept) {
+          ++st.accepted;
+          for (int k = 0; k < 9; ++k) st.rot[k] = nrot[k];
+          for (int k = 0; k < 3; ++k) st.pos[k] = npos[k];
+          st.prev_e = curr_e;
+          if (sx != cx_) {   // conf := conf'
+            for (int j = lane; j < L; j += kWave) { cx_[j] = px_[j]; cy_[j] = py_[j]; cz_[j] = pz_[j]; }
+            st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
+            __syncthreads();
+            sx = cx_; sy = cy_; sz = cz_;
+            v.sx = cx_; v.sy = cy_; v.sz = cz_;
+          }
+        }
+        if (curr_e < st.best_e) {   // best pose, accepted or not (src/lds.ml:959-969)
+          st.best_e = curr_e;
+          for (int k = 0; k < 9; ++k) st.best_rot[k] = nrot[k];
+          for (int k = 0; k < 3; ++k) st.best_pos[k] = npos[k];
+          for (int j = lane; j < L; j += kWave) {
+            double x, y, z;
+            v.at(j, x, y, z);
+            gbest[3 * j] = x; gbest[3 * j + 1] = y; gbest[3 * j + 2] = z;
+          }
+        }
+        // adaptive rigid steps every 100 rigid frames (src/lds.ml:971-974, :586-600)
+        if (rigid && st.rigid_step > 0 && st.rigid_step % kMcBlock == 0) {
+          const double ar = sw_ratio(st.sw);
+          if (ar <= 0.45) { st.max_trans *= 0.95; st.max_rot *= 0.95; }
+          else if (ar >= 0.55) { st.max_trans *= 1.05; st.max_rot = fmin(pi, 1.05 * st.max_rot); }
+        }
+        // adaptive torsion steps on the proposal's copy (src/lds.ml:976-979, :603-621):
+        // kept only when the proposal was accepted (conf := conf')
+        if (flex && !rigid && st.conf_step > 0 && st.conf_step % (kMcBlock * nrb) == 0) {
+          for (int k = 0; k < nrb; ++k) {
+            const double ar = sw_ratio(rsw[k]);
+            if (ar <= 0.45) ndr[k] = 0.95 * ndr[k];
+            else if (ar >= 0.55) ndr[k] = fmin(pi, 1.05 * ndr[k]);
+          }
+          if (accept)
+            for (int k = 0; k < nrb; ++k) dr[k] = ndr[k];
+        }
+      }
+      if (rigid) ++st.rigid_step;
+      else ++st.conf_step;
+    }
+    if (st.status == kMcRunning && st.frame >= a.total_frames) st.status = kMcDone;
+    // write the chain back (lane 0 for the scalars, all lanes for the conformer)
+    for (int j = lane; j < L; j += kWave) {
+      gconf[3 * j] = cx_[j]; gconf[3 * j + 1] = cy_[j]; gconf[3 * j + 2] = cz_[j];
+    }
+    if (lane == 0) a.st[chain] = st;
+    __syncthreads();
+  }
+}
+
+__global__ void k_mc_reset_queue(int* q) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
+}
+
+hipError_t launch_mc(const McArgs& a, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_mc_reset_queue, dim3(1), dim3(1), 0, s, a.queue);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
+  hipLaunchKernelGGL(k_mc, dim3(nblocks), dim3(kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace mmo

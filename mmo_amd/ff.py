@@ -114,6 +114,21 @@ class Context:
         self.grid_shape = (nt, nz, ny, nx)
         self._grids_keep = g
 
+    def compute_grids(self, probe_anum, probe_q, step, dims, origin, roi_center=None,
+                      roi_r=0.0, max_e=1e5, copy_out=True):
+        """GPU grid precompute for probe types (anum, q); installs the grids for the
+        interpolated form and returns them as (ntypes, nz, ny, nx) FP32 if copy_out."""
+        an = _i32(probe_anum)
+        q = _f64(probe_q)
+        nx, ny, nz = (int(v) for v in dims)
+        o = _f64(origin)
+        rc = _f64(roi_center) if roi_center is not None else None
+        out = np.zeros((len(an), nz, ny, nx), dtype=np.float32) if copy_out else None
+        self._check(lib.mmo_grid_compute(self.h, _ptr(an), _ptr(q), len(an), float(step), nx, ny, nz,
+                                         _ptr(o), _ptr(rc), float(roi_r), float(max_e), _ptr(out)))
+        self.grid_shape = (len(an), nz, ny, nx)
+        return out
+
     @property
     def fixed_point_bits(self):
         return lib.mmo_fixed_point_bits(self.h)

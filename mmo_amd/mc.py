@@ -1,0 +1,114 @@
+"""Batched Monte Carlo docking chains on the GPU (Lds.simulate_lig, src/lds.ml:741-1000).
+
+    mc = MonteCarlo(ctx, chain_lig, rot0, pos0, mode=ff.SHIFTED, hard_roi=(center, r))
+    mc.run(1000)                 # one persistent launch, all chains advance 1000 frames
+    best_e, info = mc.best_energies(), mc.info()
+
+All chains of all ligands run in one launch; ``run_graph`` replays that launch from a
+captured hipGraph and ``run_per_ligand`` launches once per ligand (the plain form).
+"""
+import ctypes as ct
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+from .ff import _f64, _i32, _ptr, _stream_ptr
+
+TWEAK_RBONDS = _lib.MC_TWEAK_RBONDS
+HARD_ROI = _lib.MC_HARD_ROI
+INTRA_UFF = _lib.MC_INTRA_UFF
+METROPOLIS_ALWAYS = _lib.MC_METROPOLIS_ALWAYS
+NO_FLIP = _lib.MC_NO_FLIP
+FP64 = _lib.MC_FP64
+ROOM_TEMP_K = 293.15            # Const.room_temp_K, src/const.ml:94
+
+
+class MonteCarlo:
+    def __init__(self, ctx, chain_lig, rot0, pos0, mode, total_frames, flags=TWEAK_RBONDS | INTRA_UFF,
+                 roi_center=None, roi_r=0.0, temperature=ROOM_TEMP_K, seed=12345):
+        self.ctx = ctx
+        cl = _i32(chain_lig)
+        r0 = _f64(np.asarray(rot0).reshape(len(cl), 9))
+        p0 = _f64(np.asarray(pos0).reshape(len(cl), 3))
+        rc = _f64(roi_center) if roi_center is not None else None
+        h = ct.c_void_p()
+        ctx._check(lib.mmo_mc_create(ctx.h, _ptr(cl), len(cl), _ptr(r0), _ptr(p0), int(mode), int(flags),
+                                     _ptr(rc), float(roi_r), float(temperature),
+                                     ct.c_uint64(int(seed)), int(total_frames), ct.byref(h)))
+        self.h = h
+        self.nchain = len(cl)
+        self.chain_lig = cl
+        self.natoms = lib.mmo_mc_natoms(h)
+        sizes = np.array([ctx.ligand_natoms(int(l)) for l in cl], dtype=np.int64)
+        self.chain_atom = np.concatenate([[0], np.cumsum(sizes)])
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib.mmo_mc_destroy(h)
+            self.h = None
+
+    def run(self, nsteps, stream=None):
+        self.ctx._check(lib.mmo_mc_run(self.h, int(nsteps), _stream_ptr(stream) if stream else None))
+
+    def run_graph(self, nsteps, nreplay=1, stream=None):
+        self.ctx._check(lib.mmo_mc_run_graph(self.h, int(nsteps), int(nreplay),
+                                             _stream_ptr(stream) if stream else None))
+
+    def run_per_ligand(self, nsteps, stream=None):
+        self.ctx._check(lib.mmo_mc_run_per_ligand(self.h, int(nsteps),
+                                                  _stream_ptr(stream) if stream else None))
+
+    def state(self):
+        n = self.nchain
+        best_e = np.zeros(n)
+        cur = np.zeros((n, 3))
+        brot = np.zeros((n, 9))
+        bpos = np.zeros((n, 3))
+        info = np.zeros((n, 8), dtype=np.int32)
+        steps = np.zeros((n, 2))
+        self.ctx._check(lib.mmo_mc_get(self.h, _ptr(best_e), _ptr(cur), _ptr(brot), _ptr(bpos),
+                                       _ptr(info), _ptr(steps)))
+        return {"best_e": best_e, "e": cur[:, 0], "e_inter": cur[:, 1], "e_intra": cur[:, 2],
+                "best_rot": brot.reshape(n, 3, 3), "best_pos": bpos, "frame": info[:, 0],
+                "status": info[:, 1], "accepted": info[:, 2], "resets": info[:, 3],
+                "ooroi": info[:, 4], "rigid_step": info[:, 5], "conf_step": info[:, 6],
+                "max_rot": steps[:, 0], "max_trans": steps[:, 1]}
+
+    def coords(self):
+        conf = np.zeros((self.natoms, 3))
+        best = np.zeros((self.natoms, 3))
+        self.ctx._check(lib.mmo_mc_coords(self.h, _ptr(conf), _ptr(best), None))
+        split = lambda a: [a[self.chain_atom[k]:self.chain_atom[k + 1]] for k in range(self.nchain)]
+        return split(conf), split(best)
+
+
+def philox4x32(counter, key):
+    """Philox4x32-10 (Salmon et al., SC 2011) on the host, the same stream as the kernel:
+    counter and key are uint32 sequences of length 4 and 2."""
+    c = [int(v) & 0xFFFFFFFF for v in counter]
+    k0, k1 = int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF
+    for _ in range(10):
+        p0 = 0xD2511F53 * c[0]
+        p1 = 0xCD9E8D57 * c[2]
+        hi0, lo0 = (p0 >> 32) & 0xFFFFFFFF, p0 & 0xFFFFFFFF
+        hi1, lo1 = (p1 >> 32) & 0xFFFFFFFF, p1 & 0xFFFFFFFF
+        c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+        k0 = (k0 + 0x9E3779B9) & 0xFFFFFFFF
+        k1 = (k1 + 0xBB67AE85) & 0xFFFFFFFF
+    return c
+
+
+def frame_uniforms(seed, chain, frame):
+    """The six uniforms [0, 1) the kernel draws for (chain, frame)."""
+    out = []
+    for draw in range(3):
+        r = philox([chain, frame, draw, 0x4D4D4F31], [seed & 0xFFFFFFFF, seed >> 32])
+        out.append(((r[0] >> 5) * 67108864.0 + (r[1] >> 6)) / 9007199254740992.0)
+        out.append(((r[2] >> 5) * 67108864.0 + (r[3] >> 6)) / 9007199254740992.0)
+    return out
+
+
+philox = philox4x32

@@ -1,0 +1,160 @@
+"""CPU restatement of Lds.simulate_lig (src/lds.ml:741-1000) for checking the GPU chains:
+FP64 energies from the oracle, the same Philox uniforms as the kernel (one stream per
+chain and frame), the reference's control flow including the dangling `else` of
+src/lds.ml:917 (no acceptance test without hard ROI) and its adaptive step rules."""
+import math
+
+import numpy as np
+
+import oracle
+from mmo_amd.mc import frame_uniforms
+
+PI = 4.0 * math.atan(1.0)
+DEG = PI / 180.0
+
+
+class SW:
+    """SW.t, src/SW.ml:7-42."""
+
+    def __init__(self, size=100):
+        self.size, self.events = size, []
+
+    def process(self, ev):
+        self.events.append(bool(ev))
+        if len(self.events) > self.size:
+            self.events.pop(0)
+
+    def ratio(self):
+        n = len(self.events)
+        return sum(self.events) / n if n else float("nan")
+
+    def reset(self):
+        self.events = []
+
+
+def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True, hard_roi=None,
+             metropolis_always=False, beta=1.0 / (0.0019872041 * 293.15), inter_fn=None,
+             trace=None):
+    """Runs one chain for `nframes`; `trace`, when a list, receives one dict per frame."""
+    n = len(lig["xs"])
+    conf = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1).astype(np.float64)
+    center = np.zeros(3)
+    nrb = len(lig["rb_left"])
+    flex = tweak and nrb > 0
+    groups = [np.asarray(lig["rg_atoms"][lig["rg_off"][k]:lig["rg_off"][k + 1]]) for k in range(nrb)]
+    rot, pos = np.array(rot0, dtype=float).reshape(3, 3), np.array(pos0, dtype=float)
+    max_rot0, max_trans0 = 15.0 * DEG, 0.15
+    max_rot, max_trans = max_rot0, max_trans0
+    dr = [5.0 * DEG] * nrb
+    sw = SW()
+    rsw = [SW() for _ in range(nrb)]
+
+    def e_intra(c):
+        return oracle.ene_intra(dict(lig, xs=c[:, 0], ys=c[:, 1], zs=c[:, 2])) if intra else 0.0
+
+    def e_inter(x):
+        if inter_fn is not None:
+            return inter_fn(x)
+        return oracle.ene_inter(rec, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]), shifted=True)
+
+    start = conf @ rot.T + pos
+    prev_intra = e_intra(conf)
+    prev_inter = e_inter(start)
+    prev_e = prev_inter + prev_intra
+    best_e = prev_e
+    best_rot, best_pos = rot.copy(), pos.copy()
+    rigid_step = conf_step = 0
+    accepted = resets = ooroi = 0
+    status = 0
+    for frame in range(nframes):
+        u = frame_uniforms(seed, chain, frame)
+        rigid = frame % 2 == 0
+        conf2, c2, jr, ndr = conf, center, -1, list(dr)
+        rot2, pos2 = rot, pos
+        if not rigid:
+            if flex:
+                i = min(int(u[0] * nrb), nrb - 1)
+                flip = conf_step > 0 and conf_step % 100 == 0
+                amp = PI if flip else ndr[i]
+                ang = u[1] * (2.0 * amp) - amp
+                conf2 = oracle.rotate_bond(conf, int(lig["rb_left"][i]), int(lig["rb_right"][i]),
+                                           groups[i], ang)
+                c2 = conf2.sum(axis=0) / n
+                rad = np.max(0.01 + np.linalg.norm(conf2 - c2, axis=1))
+                if rad > 12.0:
+                    status = 2
+                    break
+                jr = i
+        else:
+            th = u[0] * (2.0 * max_rot) - max_rot
+            ax = min(int(u[1] * 3.0), 2)
+            A = (oracle.rot_x, oracle.rot_y, oracle.rot_z)[ax](th)
+            rot2 = A @ rot
+            pos2 = pos + (np.array(u[2:5]) * 2.0 - 1.0) * max_trans
+        pose = (conf2 - c2) @ rot2.T + pos2
+        if not rigid and flex and intra:
+            prev_intra = e_intra(conf2)
+        prev_inter = e_inter(pose)
+        cur = prev_inter + prev_intra
+        metropolis = False
+        reset = False
+        if hard_roi is not None:
+            if np.linalg.norm(np.asarray(hard_roi[0]) - pos2) > hard_roi[1]:
+                reset = True
+                ooroi += 1
+            else:
+                metropolis = True
+        else:
+            metropolis = metropolis_always
+        if metropolis and prev_inter == 0.0:
+            reset, metropolis = True, False
+        if reset:
+            max_rot, max_trans = max_rot0, max_trans0
+            rot, pos = np.array(rot0, dtype=float).reshape(3, 3), np.array(pos0, dtype=float)
+            best_rot, best_pos = np.eye(3), np.zeros(3)
+            prev_e = math.inf
+            best_e = math.inf
+            sw.reset()
+            resets += 1
+        elif metropolis:
+            acc = (cur <= prev_e) or (u[5] < math.exp(-(cur - prev_e) * beta))
+            if rigid:
+                sw.process(acc)
+            elif jr > -1:
+                rsw[jr].process(acc)
+            if acc:
+                accepted += 1
+                rot, pos, prev_e = rot2, pos2, cur
+                conf, center = conf2, c2
+            if cur < best_e:
+                best_e, best_rot, best_pos = cur, rot2, pos2
+            if rigid and rigid_step > 0 and rigid_step % 100 == 0:
+                ar = sw.ratio()
+                if ar <= 0.45:
+                    max_trans *= 0.95
+                    max_rot *= 0.95
+                elif ar >= 0.55:
+                    max_trans *= 1.05
+                    max_rot = min(PI, 1.05 * max_rot)
+            if flex and not rigid and conf_step > 0 and conf_step % (100 * nrb) == 0:
+                for k in range(nrb):
+                    ar = rsw[k].ratio()
+                    if ar <= 0.45:
+                        ndr[k] = 0.95 * ndr[k]
+                    elif ar >= 0.55:
+                        ndr[k] = min(PI, 1.05 * ndr[k])
+                if acc:
+                    dr = ndr
+        if rigid:
+            rigid_step += 1
+        else:
+            conf_step += 1
+        if trace is not None:
+            trace.append({"frame": frame, "e_inter": prev_inter, "e_intra": prev_intra,
+                          "e": prev_e, "best_e": best_e, "accepted": accepted,
+                          "resets": resets, "pos": pos.copy(), "rot": rot.copy(),
+                          "conf": conf.copy(), "center": np.array(center), "pose": pose.copy()})
+    return {"best_e": best_e, "e": prev_e, "e_inter": prev_inter, "e_intra": prev_intra,
+            "rot": rot, "pos": pos, "conf": conf, "accepted": accepted, "resets": resets,
+            "ooroi": ooroi, "status": status, "max_rot": max_rot, "max_trans": max_trans,
+            "best_rot": best_rot, "best_pos": best_pos, "dr": dr}

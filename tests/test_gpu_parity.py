@@ -364,3 +364,222 @@ def test_batch_culled_path(sys0):
         refs.append(oracle.ene_inter(rec, m))
         sums.append(inter_abs_sum(rec, x, m))
     assert_energy(e[sample, 0], refs, sums, what="culled batch")
+
+
+# ----------------------------------------------------------------- grid precompute
+def _abs_sum_at(rec, pts, anum, q):
+    """sum_i |w e_pair| of a single-atom probe at each point (conditioning scale)."""
+    from oracle import twin
+    pxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    out = []
+    for p in pts:
+        r2, r, elec, vdw = twin._pair_terms(pxyz, np.asarray(rec["q"]), rec["anum"], p[None, :],
+                                            np.array([q]), [anum])
+        w = np.where(r2 < 144.0, (1.0 - (r / 12.0) ** 2) ** 2, 0.0)
+        out.append(float(np.sum(np.abs(w * (twin.KE * elec + vdw)))))
+    return np.array(out)
+
+
+def test_grid_precompute_matches_oracle(sys0):
+    ctx, lig, rec = sys0
+    dims = (20, 18, 16)
+    step = 0.75
+    origin = np.array([-7.0, -6.5, -6.0])
+    probe_a = np.array([6, 1, 8, 16])
+    probe_q = np.array([0.1, -0.3, 0.0, 0.25])
+    roi_c, roi_r = np.array([0.5, 0.0, -0.5]), 9.0
+    print("grid precompute", dims, flush=True)
+    g = ctx.compute_grids(probe_a, probe_q, step, dims, origin, roi_center=roi_c, roi_r=roi_r)
+    og = oracle.Grid.make(step, dims)
+    mask = oracle.bitmask_roi(og, roi_c - origin, roi_r - 24.0)
+    shifted = dict(rec, xs=rec["xs"] - origin[0], ys=rec["ys"] - origin[1], zs=rec["zs"] - origin[2])
+    ref = oracle.grid_energies(shifted, og, probe_a, probe_q, mask=mask)
+    m3 = mask.reshape(dims[2], dims[1], dims[0]).astype(bool)
+    assert np.all(g[:, ~m3] == 0.0)
+    assert m3.sum() > 1000 and (~m3).sum() > 100
+    rng = np.random.default_rng(3)
+    kk, jj, ii = np.nonzero(m3)
+    pick = rng.choice(len(ii), 200, replace=False)
+    pts = np.stack([ii[pick], jj[pick], kk[pick]], 1) * step + origin
+    for t in range(len(probe_a)):
+        a = g[t, kk[pick], jj[pick], ii[pick]].astype(np.float64)
+        b = ref[t, kk[pick], jj[pick], ii[pick]].astype(np.float64)
+        clamped = b >= 1e5
+        assert np.array_equal(a >= 1e5, clamped)
+        scale = _abs_sum_at(rec, pts, probe_a[t], probe_q[t])
+        err = np.abs(a - b)[~clamped] / np.maximum(scale[~clamped], 1e-30)
+        assert err.max() < 1e-5, (t, err.max())
+    # the computed grids are installed for the interpolated form
+    xyz = rng.uniform(-4.0, 4.0, (25, 3))
+    types = rng.integers(0, 4, 25)
+    e = ctx.ene_inter_interp({"xs": xyz[:, 0], "ys": xyz[:, 1], "zs": xyz[:, 2]}, types)
+    r = oracle.ene_inter_interp(og, list(g), xyz - origin, types)
+    assert e == pytest.approx(r, rel=1e-12, abs=1e-12)
+
+
+def test_large_receptor_global_memory_cull():
+    """A receptor too large for LDS (15,000 atoms in a 60 A box) takes the culled
+    kernels with the cell list read from global memory."""
+    import torch
+    rng = np.random.default_rng(21)
+    rec = synth.make_receptor(rng, 15000, min_sep=2.0, box=60.0)
+    lig, _ = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    assert ctx.tuning("cull_ok") == 0.0
+    lid = ctx.add_ligand(lig)
+    rots = synth.so3_rotations(4608)[::1152]
+    trans = synth.lattice((0, 0, 0), 8, 1.0)
+    print("global-memory cull scan", len(rots) * len(trans), flush=True)
+    ec = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+    sample = rng.choice(len(ec), 16, replace=False)
+    pr, pt = sample // len(trans), sample % len(trans)
+    ref = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=8)
+    sums = [inter_abs_sum(rec, _place(lig, rots[a], trans[b])[1], lig, True) for a, b in zip(pr, pt)]
+    assert_energy(ec[sample], ref, sums, what="global cull scan")
+    g = ctx.compute_grids([6, 8], [0.2, -0.4], 1.0, (12, 12, 12), (-6.0, -6.0, -6.0))
+    og = oracle.Grid.make(1.0, (12, 12, 12))
+    shifted = dict(rec, xs=rec["xs"] + 6.0, ys=rec["ys"] + 6.0, zs=rec["zs"] + 6.0)
+    ref = oracle.grid_energies(shifted, og, [6, 8], [0.2, -0.4])
+    ok = ref < 1e5
+    assert np.allclose(g[ok], ref[ok], rtol=1e-4, atol=1e-3)
+    torch.cuda.synchronize()
+
+
+# ----------------------------------------------------------------- Monte Carlo
+def _mc_setup(seed=17):
+    from mmo_amd import mc as mcmod
+    _, rec = synth.config0()
+    ligs = [l for l in synth.library(10, seed=seed) if 2 <= len(l["rb_left"]) <= 8][:2]
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    ids = [ctx.add_ligand(l) for l in ligs]
+    rng = np.random.default_rng(seed)
+    chain_lig = np.array([ids[0], ids[1], ids[0], ids[1], ids[0]])
+    rot0 = synth.random_rotations(rng, len(chain_lig))
+    # on the receptor's outer surface (r_out ~ 28 A): moderate energies, so that no
+    # accept/reject decision hinges on the last digits of E
+    pos0 = SURFACE + rng.uniform(-1.0, 1.0, (len(chain_lig), 3))
+    return mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0
+
+
+SURFACE = np.array([36.0, 0.0, 0.0])
+
+
+def test_mc_matches_cpu_restatement():
+    from tests.mc_ref import simulate
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup()
+    roi = (SURFACE, 4.0)
+    # FP64 pair arithmetic; receptor charges and LJ factors stay FP32 and coordinates
+    # 2^-24 A fixed point, so energies agree to ~1e-7 relative, not to the last bit
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI | mcmod.FP64
+    m = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=80, flags=flags,
+                         roi_center=roi[0], roi_r=roi[1], seed=99)
+    print("mc run 2 x 40 frames", flush=True)
+    m.run(40)
+    m.run(40)
+    st = m.state()
+    conf, best = m.coords()
+    assert (st["frame"] == 80).all() and (st["status"] != 0).all()
+    for k, l in enumerate(chain_lig):
+        ref = simulate(ligs[ids.index(l)], rec, rot0[k], pos0[k], 80, seed=99, chain=k,
+                       hard_roi=roi)
+        print("chain", k, "gpu", st["accepted"][k], st["best_e"][k], st["e"][k], "cpu",
+              ref["accepted"], ref["best_e"], ref["e"], flush=True)
+        assert st["status"][k] == (1 if ref["status"] == 0 else 2)
+        assert st["accepted"][k] == ref["accepted"], (k, st["accepted"][k], ref["accepted"])
+        assert st["resets"][k] == ref["resets"] and st["ooroi"][k] == ref["ooroi"]
+        for key in ("best_e", "e"):
+            if math.isinf(ref[key]):
+                assert math.isinf(st[key][k])
+            else:
+                assert st[key][k] == pytest.approx(ref[key], rel=1e-6, abs=1e-4), key
+        assert np.allclose(st["max_rot"][k], ref["max_rot"]) and np.allclose(st["max_trans"][k], ref["max_trans"])
+        assert np.allclose(conf[k], ref["conf"], atol=1e-9)
+
+
+def test_mc_fp32_matches_cpu_restatement():
+    """Default (FP32 pair arithmetic) chains follow the FP64 restatement: same
+    decisions, energies within the FP32 scoring tolerance."""
+    from tests.mc_ref import simulate
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=19)
+    roi = (SURFACE, 4.0)
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    m = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=60, flags=flags,
+                         roi_center=roi[0], roi_r=roi[1], seed=3)
+    m.run(60)
+    st = m.state()
+    for k, l in enumerate(chain_lig):
+        ref = simulate(ligs[ids.index(l)], rec, rot0[k], pos0[k], 60, seed=3, chain=k, hard_roi=roi)
+        assert st["accepted"][k] == ref["accepted"]
+        assert st["resets"][k] == ref["resets"]
+        assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-5, abs=1e-3)
+
+
+def test_mc_launch_forms_agree():
+    """One persistent launch, hipGraph replays and one launch per ligand give the same
+    chains bit for bit (each chain depends only on its own counter-based stream)."""
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=23)
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    kw = dict(total_frames=60, flags=flags, roi_center=SURFACE, roi_r=4.0, seed=5)
+    a = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, **kw)
+    b = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, **kw)
+    c = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, **kw)
+    print("mc launch forms", flush=True)
+    a.run(60)
+    b.run_graph(20, nreplay=3)
+    c.run_per_ligand(30)
+    c.run_per_ligand(30)
+    sa, sb, sc = a.state(), b.state(), c.state()
+    for key in ("best_e", "e", "accepted", "resets", "frame", "max_rot"):
+        assert np.array_equal(sa[key], sb[key]), key
+        assert np.array_equal(sa[key], sc[key]), key
+    assert np.array_equal(a.coords()[0][0], b.coords()[0][0])
+
+
+def test_mc_without_hard_roi_never_accepts():
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=29)
+    m = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=40,
+                         flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF, seed=1)
+    m.run(40)
+    st = m.state()
+    assert (st["accepted"] == 0).all()
+    m2 = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=40,
+                          flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.METROPOLIS_ALWAYS,
+                          seed=1)
+    m2.run(40)
+    assert (m2.state()["accepted"] > 0).any()
+
+
+def test_mc_interp_mode_matches_cpu_restatement():
+    from tests.mc_ref import simulate
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=31)
+    # per-type grids for each ligand atom's (anum, q), computed on the GPU
+    types, probe_a, probe_q = [], [], []
+    for l in ligs:
+        t = []
+        for a, q in zip(l["anum"], l["q"]):
+            probe_a.append(int(a))
+            probe_q.append(float(q))
+            t.append(len(probe_a) - 1)
+        types.append(np.array(t))
+    origin, dims, step = synth.grid_spec(center=SURFACE, n=64, step=0.5)
+    grids = ctx.compute_grids(probe_a, probe_q, step, dims, origin)
+    ids2 = [ctx.add_ligand(l, types=t) for l, t in zip(ligs, types)]
+    cl = np.array([ids2[ids.index(l)] for l in chain_lig])
+    roi = (SURFACE, 3.0)
+    m = mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.INTERP, total_frames=60,
+                         flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI,
+                         roi_center=roi[0], roi_r=roi[1], seed=8)
+    m.run(60)
+    st = m.state()
+    g = oracle.Grid.make(step, dims)
+    for k, l in enumerate(chain_lig):
+        li = ids.index(l)
+
+        def inter(x, li=li):
+            return oracle.ene_inter_interp(g, list(grids), x - origin, types[li])
+        ref = simulate(ligs[li], rec, rot0[k], pos0[k], 60, seed=8, chain=k, hard_roi=roi,
+                       inter_fn=inter)
+        assert st["accepted"][k] == ref["accepted"]
+        assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-9, abs=1e-9)
