@@ -1,13 +1,22 @@
-"""Benchmark: exhaustive rigid-pose scan with the plain (direct-sum) method on one GPU.
+"""Benchmark of the UFF scoring path on one GPU.
 
-Workload: the 75-atom synthetic ligand (40 heavy + 35 H, 6 rotatable bonds) and the
-3000-atom Poisson-disc receptor; 4,608 super-Fibonacci SO(3) rotations x 20^3 lattice
-translations at 0.5 A = 36,864,000 poses.  One step scores every pose with the shifted
+Headline (the JSON line's "value"): exhaustive rigid-pose scan with the plain
+direct-sum method.  The 75-atom synthetic ligand (40 heavy + 35 H, 6 rotatable bonds)
+and the 3000-atom Poisson-disc receptor; 4,608 super-Fibonacci SO(3) rotations x 20^3
+lattice translations at 0.5 A = 36,864,000 poses per step, each scored with the shifted
 LJ 12-6 + Coulomb direct sum (Mol.ene_inter_UFF_shifted_brute semantics).  Inputs are
 resident on the GPU before timing starts; outputs stay on the GPU.
 
-Prints one JSON line (see the "roofline" and "cpu_baseline" objects for the kernel's
-fraction of the FP32 VALU peak and the FP64 CPU restatement timed on this host).
+Secondary figures (the "extra" object), each on its own workload:
+  scan_interp  the same scan through trilinear interpolation in 10 per-type FP32 grids of
+               64^3 at 0.375 A, precomputed on the GPU (precompute time reported);
+  egrad        energy + adjoint gradient in DOF space for 4,096 poses of 1,024 library
+               ligands: one batched launch sequence vs one per ligand;
+  mc           Monte Carlo chains, 64 per ligand for 1,024 library ligands, 1,000 frames,
+               batched persistent launch replayed from a hipGraph vs one launch per ligand.
+
+Prints one JSON line with "roofline" (the scan kernel's fraction of the FP32 peak) and
+"cpu_baseline" (the FP64 C restatement timed on this host's cores).
 """
 import argparse
 import json
@@ -24,6 +33,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3        # MI355X FP32 vector peak (AMD spec)
 PEAK_HBM_GBS = 8000.0
 N_ROT, N_TRANS_SIDE, TRANS_STEP = 4608, 20, 0.5
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
 
 
 def algorithmic_flops_per_pose(lig, rec, rots, trans, sample=4096, seed=0):
@@ -42,12 +56,16 @@ def algorithmic_flops_per_pose(lig, rec, rots, trans, sample=4096, seed=0):
     return 8.0 * L * P + 19.0 * mean_in, mean_in
 
 
+def cpu_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(n, 16))
+
+
 def cpu_baseline(lig, rec, rots, trans, target_s=12.0):
     """FP64 C restatement of the reference loop (oracle, OpenMP over poses) on a random
     sample of the same scan's poses."""
     import oracle
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 16))
+    threads = cpu_threads()
     rng = np.random.default_rng(1)
     n = 64
     while True:
@@ -64,6 +82,166 @@ def cpu_baseline(lig, rec, rots, trans, target_s=12.0):
                       % (n, dt)}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = t.get(kernel)
+    return rec.get("bytes_per_launch") if rec else None
+
+
+def timed_scan(ctx, lid, mode, rots, trans, out, steps, warmup, kernel):
+    import torch
+    from mmo_amd import ff
+    for _ in range(warmup):
+        ctx.scan(lid, mode, rots, trans, out=out)
+    torch.cuda.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.scan(lid, mode, rots, trans, out=out)
+        kernel_ms.append(ctx.last_kernel_ms(kernel))
+        log("%s scan step %d kernel %.1f ms" % ("direct" if mode == ff.SHIFTED else "interp",
+                                                k, kernel_ms[-1]))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, kernel_ms
+
+
+def bench_interp(ctx, lig, rots, trans, steps, warmup):
+    """Scan through 10 per-type grids (64^3 at 0.375 A) computed on the GPU."""
+    import torch
+    from mmo_amd import ff, synth
+    probe_a, probe_q, types = synth.grid_types([lig], 10)
+    origin, dims, step = synth.grid_spec((0.0, 0.0, 0.0), 64, 0.375)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.compute_grids(probe_a, probe_q, step, dims, origin, copy_out=False)
+    torch.cuda.synchronize()
+    grid_s = time.perf_counter() - t0
+    grid_ms = ctx.last_kernel_ms("grid")
+    lid = ctx.add_ligand(lig, types=types[0])
+    npose = len(rots) * len(trans)
+    out = torch.empty(npose, dtype=torch.float64, device="cuda:0")
+    elapsed, kms = timed_scan(ctx, lid, ff.INTERP, rots, trans, out, steps, warmup, "interp")
+    e = out.cpu().numpy()
+    inside = int(np.isfinite(e).sum())
+    avg = float(np.mean(kms))
+    lookups = inside * len(lig["xs"])
+    return {"value": npose * steps / elapsed, "unit": "poses/s", "kernel_ms": avg,
+            "grids": "10 x 64^3 FP32 at 0.375 A (%.1f MB)" % (10 * 64 ** 3 * 4 / 1e6),
+            "grid_precompute_ms": grid_ms, "grid_precompute_wall_s": grid_s,
+            "poses_inside_grid": inside, "poses_out_of_grid": npose - inside,
+            "gather_GBs": 32.0 * lookups / (avg * 1e-3) / 1e9,
+            "note": "poses with an atom outside the grid return NaN and are counted"}
+
+
+def library_system(n_lig):
+    from mmo_amd import ff, synth
+    ligs = synth.library(n_lig, processes=min(16, cpu_threads()))
+    _, rec = synth.config0()
+    return ligs, rec
+
+
+def bench_egrad(ctx, ligs, ids, poses_per_lig=4, reps=20):
+    """Energy + gradient in DOF space: 4 poses of each library ligand, batched vs one
+    launch sequence per ligand."""
+    import torch
+    from mmo_amd import ff
+    rng = np.random.default_rng(3)
+    pose_lig = np.repeat(np.asarray(ids), poses_per_lig)
+    flags = ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD
+
+    def dofs_for(pl):
+        d = []
+        for l in pl:
+            nrb = len(ligs[ids.index(l)]["rb_left"])
+            d.append(np.concatenate([rng.uniform(-1.5, 1.5, 3), rng.uniform(-math.pi, math.pi, 3),
+                                     rng.uniform(-math.pi, math.pi, nrb)]))
+        return torch.tensor(np.concatenate(d), device="cuda:0")
+
+    b = ctx.batch(pose_lig)
+    d = dofs_for(pose_lig)
+    e = torch.empty(b.npose, 4, dtype=torch.float64, device="cuda:0")
+    g = torch.empty_like(d)
+    b.score_dofs_dev(d, e, gdof=g, flags=flags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.score_dofs_dev(d, e, gdof=g, flags=flags)
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / reps
+    # one launch sequence per ligand
+    per = []
+    for l in ids:
+        bl = ctx.batch(np.full(poses_per_lig, l))
+        dl = dofs_for([l] * poses_per_lig)
+        per.append((bl, dl, torch.empty(poses_per_lig, 4, dtype=torch.float64, device="cuda:0"),
+                    torch.empty_like(dl)))
+    for bl, dl, el, gl in per:
+        bl.score_dofs_dev(dl, el, gdof=gl, flags=flags)
+    torch.cuda.synchronize()
+    preps = max(1, reps // 5)
+    t0 = time.perf_counter()
+    for _ in range(preps):
+        for bl, dl, el, gl in per:
+            bl.score_dofs_dev(dl, el, gdof=gl, flags=flags)
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / preps
+    n = len(pose_lig)
+    return {"batched": n / tb, "per_ligand": n / tp, "speedup": tp / tb, "unit": "poses/s",
+            "poses": n, "ligands": len(ids), "ms_batched": tb * 1e3, "ms_per_ligand": tp * 1e3,
+            "dtype": "fp32 pairs, fp64 per-pose sums and DOF chain"}
+
+
+def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample):
+    """MC steps/s: all chains of all ligands in one launch (hipGraph replays) vs one
+    launch per ligand (the latter on the first `per_lig_sample` ligands)."""
+    import torch
+    from mmo_amd import ff, synth
+    from mmo_amd import mc as mcmod
+    rng = np.random.default_rng(4)
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    roi = (np.zeros(3), 6.0)
+
+    def make(lig_ids):
+        cl = np.repeat(np.asarray(lig_ids), chains_per_lig)
+        rot0 = synth.random_rotations(rng, len(cl))
+        pos0 = rng.uniform(-1.0, 1.0, (len(cl), 3))
+        return mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.SHIFTED, total_frames=frames, flags=flags,
+                                roi_center=roi[0], roi_r=roi[1], seed=2024)
+
+    m = make(ids)
+    nrep = 4 if frames % 4 == 0 else 1
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.run_graph(frames // nrep, nreplay=nrep)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    st = m.state()
+    done = int((st["frame"] == frames).sum())
+    steps_b = float(st["frame"].sum())
+    sub = ids[:per_lig_sample]
+    m2 = make(sub)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m2.run_per_ligand(frames)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    steps_p = float(m2.state()["frame"].sum())
+    return {"batched": steps_b / tb, "per_ligand": steps_p / tp,
+            "speedup": (steps_b / tb) / (steps_p / tp), "unit": "MC steps/s",
+            "chains": m.nchain, "frames": frames, "chains_finished": done,
+            "too_long_chains": int((st["status"] == 2).sum()),
+            "accept_rate": float(st["accepted"].sum() / max(steps_b, 1.0)),
+            "s_batched": tb, "s_per_ligand": tp,
+            "per_ligand_sample": "%d ligands x %d chains" % (len(sub), chains_per_lig),
+            "launch": "one persistent launch per hipGraph replay, %d replays of %d frames"
+                      % (nrep, frames // nrep)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +249,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rotations", type=int, default=N_ROT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only-scan", action="store_true", help="headline scan only (no extras)")
+    ap.add_argument("--ligands", type=int, default=1024)
+    ap.add_argument("--mc-chains", type=int, default=64)
+    ap.add_argument("--mc-frames", type=int, default=1000)
+    ap.add_argument("--mc-per-ligand-sample", type=int, default=32)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this size first (counter scale reference)")
     ap.add_argument("--tune", default="", help="kernel shape, e.g. cull_k=1,cull_waves=16")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -94,25 +279,25 @@ def main():
     ctx.tune(**{k: float(v) for k, v in tune.items()})
     lid = ctx.add_ligand(lig)
     out = torch.empty(npose, dtype=torch.float64, device="cuda:0")
-    print("bench: %d poses per step, %d warmup, %d timed" % (npose, args.warmup, args.steps),
-          file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        ctx.scan(lid, ff.SHIFTED, rots, trans, out=out)
-    torch.cuda.synchronize()
-    kernel_ms = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ctx.scan(lid, ff.SHIFTED, rots, trans, out=out)
-        kernel_ms.append(ctx.last_kernel_ms("inter"))
-        print("bench: step %d kernel %.1f ms" % (k, kernel_ms[-1]), file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d MiB read + %d MiB written" % (args.copy_probe_mib, args.copy_probe_mib))
+        del src, dst
+    log("%d poses per step, %d warmup, %d timed" % (npose, args.warmup, args.steps))
+    elapsed, kernel_ms = timed_scan(ctx, lid, ff.SHIFTED, rots, trans, out, args.steps,
+                                    args.warmup, "inter")
     e = out.cpu().numpy()
     assert np.isfinite(e).all()
 
     flops_pose, mean_in = algorithmic_flops_per_pose(lig, rec, rots, trans)
     avg_ms = float(np.mean(kernel_ms))
     achieved = flops_pose * npose / (avg_ms * 1e-3) / 1e12
+    kernel = ("k_inter_cull" if ctx.tuning("scan_cull") > 0 and ctx.tuning("cull_ok") > 0
+              else "k_inter")
     line = {
         "metric": "poses/sec (energy, energy+grad) on 1 MI355X at <=1e-5 rel err vs FP64; MC steps/s",
         "value": npose * args.steps / elapsed,
@@ -131,11 +316,24 @@ def main():
                    "rotations": len(rots), "translations": len(trans), "poses": npose,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "fp32-valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_inter_cull (scan)" if ctx.tuning("scan_cull") > 0 and
-                     ctx.tuning("cull_ok") > 0 else "k_inter (scan)", "kernel_ms": avg_ms,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": measured_traffic(kernel + " scan"),
+                     "kernel": kernel + " (scan)", "kernel_ms": avg_ms,
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
     }
+    if not args.only_scan:
+        extra = {}
+        log("interp scan")
+        extra["scan_interp"] = bench_interp(ctx, lig, rots, trans, args.steps, args.warmup)
+        log("library of %d ligands" % args.ligands)
+        ligs, _ = library_system(args.ligands)
+        ids = [ctx.add_ligand(l) for l in ligs]
+        log("energy+gradient, batched vs per ligand")
+        extra["egrad"] = bench_egrad(ctx, ligs, ids)
+        log("Monte Carlo, batched vs per ligand")
+        extra["mc"] = bench_mc(ctx, ligs, ids, args.mc_chains, args.mc_frames,
+                               args.mc_per_ligand_sample)
+        line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lig, rec, rots, trans)
     print(json.dumps(line), flush=True)

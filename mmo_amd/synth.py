@@ -427,6 +427,47 @@ def config1_scan(n_rot=4608, n_trans=20, trans_step=0.5):
     return so3_rotations(n_rot), lattice((0.0, 0.0, 0.0), n_trans, trans_step)
 
 
+def config4_receptor():
+    """15,000 atoms in a 60 A box.  Random sequential adsorption jams near 2.45 A at
+    this density, so the minimum separation is 2.0 A (recorded with the workload)."""
+    if "config4" not in _CACHE:
+        rng = np.random.Generator(np.random.PCG64(SEEDS["config4"]))
+        _CACHE["config4"] = make_receptor(rng, 15000, box=60.0, min_sep=2.0, name="config4_rec")
+    return _CACHE["config4"]
+
+
+def grid_types(ligs, ntypes=10):
+    """Probe types for the per-type grids (src/lds.ml:452-469 builds one grid per
+    distinct (anum, q)).  Atoms of `ligs` are grouped by element and charge sign, and
+    the group with the widest charge range is split at its median until there are
+    `ntypes` groups; each group's probe charge is its mean.  Returns
+    (probe_anum, probe_q, [type id per atom of each ligand])."""
+    an = np.concatenate([np.asarray(l["anum"]) for l in ligs])
+    q = np.concatenate([np.asarray(l["q"], dtype=np.float64) for l in ligs])
+    groups = {}
+    for i, (a, v) in enumerate(zip(an, q)):
+        groups.setdefault((int(a), v >= 0.0), []).append(i)
+    groups = [np.array(g) for _, g in sorted(groups.items())]
+    while len(groups) < ntypes:
+        k = max(range(len(groups)), key=lambda j: np.ptp(q[groups[j]]) if len(groups[j]) > 1 else -1)
+        g = groups[k]
+        if len(g) < 2:
+            break
+        order = g[np.argsort(q[g], kind="stable")]
+        groups[k:k + 1] = [order[:len(order) // 2], order[len(order) // 2:]]
+    tid = np.zeros(len(an), dtype=np.int32)
+    for t, g in enumerate(groups):
+        tid[g] = t
+    probe_a = [int(an[g[0]]) for g in groups]
+    probe_q = [float(q[g].mean()) for g in groups]
+    out, off = [], 0
+    for l in ligs:
+        n = len(l["anum"])
+        out.append(tid[off:off + n].copy())
+        off += n
+    return probe_a, probe_q, out
+
+
 def grid_spec(center=(0.0, 0.0, 0.0), n=64, step=0.375):
     """A cubic grid of n^3 points at `step` centred on `center` -> (origin, dims, step)."""
     half = step * (n - 1) / 2.0

@@ -307,6 +307,71 @@ def test_dof_coordinates_and_gradient(lib_sys):
         assert_grad(got, fd, tol=1e-4, what="dof grad pose %d" % p)
 
 
+def test_autograd_pose_and_dof_energy(lib_sys):
+    """torch.autograd wrappers: xyz.grad against the FP64 analytic gradient of the
+    oracle, dofs.grad against central differences of the oracle energy (h = 1e-5: the
+    random torsions clash, and at h = 1e-4 the FD truncation error alone reaches 3e-5
+    of the largest component), both scaled by the per-pose weights of the backward."""
+    import torch
+    from mmo_amd.autograd import dof_energy, pose_energy
+    ctx, ligs, ids, rec, _ = lib_sys
+    pick = [k for k in range(len(ligs)) if len(ligs[k]["rb_left"]) >= 1][:3]
+    pose_lig = np.array([ids[k] for k in pick for _ in range(2)])
+    b = ctx.batch(pose_lig)
+    rng = np.random.default_rng(12)
+    dev = torch.device("cuda:0")
+    # coordinates form
+    xyz = np.zeros((b.natoms, 3))
+    mols = []
+    for p, l in enumerate(pose_lig):
+        lg, x = _place(ligs[l], synth.random_rotations(rng, 1)[0], rng.uniform(-3, 3, 3))
+        xyz[b.atom_off[p]:b.atom_off[p + 1]] = x
+        mols.append(lg)
+    w = rng.uniform(0.5, 2.0, len(pose_lig))
+    x_t = torch.tensor(xyz, device=dev, requires_grad=True)
+    print("autograd pose energy", len(pose_lig), flush=True)
+    E = pose_energy(b, x_t)
+    (E * torch.tensor(w, device=dev)).sum().backward()
+    torch.cuda.synchronize()
+    e, gx = E.detach().cpu().numpy(), x_t.grad.cpu().numpy()
+    for p, lg in enumerate(mols):
+        ei, gi = oracle.grad_inter(rec, lg)
+        ea, ga = oracle.grad_intra(lg)
+        assert e[p] == pytest.approx(ei + ea, rel=1e-5, abs=1e-6)
+        assert_grad(gx[b.atom_off[p]:b.atom_off[p + 1]], w[p] * (gi + ga), tol=1e-4,
+                    what="autograd xyz pose %d" % p)
+    # DOF form
+    dofs = []
+    for l in pose_lig:
+        nrb = len(ligs[l]["rb_left"])
+        dofs.append(np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-math.pi, math.pi, 3 + nrb)]))
+    d_t = torch.tensor(np.concatenate(dofs), device=dev, requires_grad=True)
+    print("autograd dof energy", flush=True)
+    E = dof_energy(b, d_t)
+    (E * torch.tensor(w, device=dev)).sum().backward()
+    torch.cuda.synchronize()
+    e, gd = E.detach().cpu().numpy(), d_t.grad.cpu().numpy()
+    for p, l in enumerate(pose_lig):
+        lg = ligs[l]
+        c = np.stack([lg["xs"], lg["ys"], lg["zs"]], 1)
+        args = (lg["rb_left"], lg["rb_right"], lg["rg_off"], lg["rg_atoms"])
+
+        def Ef(d):
+            x, _ = oracle.apply_config(c, *args, d)
+            m = dict(lg, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+            return oracle.ene_inter(rec, m) + oracle.ene_intra(m)
+        assert e[p] == pytest.approx(Ef(dofs[p]), rel=1e-5, abs=1e-6)
+        fd = np.zeros(len(dofs[p]))
+        for k in range(len(dofs[p])):
+            h = 1e-5
+            dp, dm = dofs[p].copy(), dofs[p].copy()
+            dp[k] += h
+            dm[k] -= h
+            fd[k] = (Ef(dp) - Ef(dm)) / (2 * h)
+        assert_grad(gd[b.dof_off[p]:b.dof_off[p + 1]], w[p] * fd, tol=1e-4,
+                    what="autograd dof pose %d" % p)
+
+
 def test_rigid_scan_culled_equals_unculled_and_oracle(sys0):
     """The culled kernel (cell list in LDS) drops only pairs beyond 12 A."""
     import torch
