@@ -196,23 +196,30 @@ def bench_egrad(ctx, ligs, ids, poses_per_lig=4, reps=20):
             "dtype": "fp32 pairs, fp64 per-pose sums and DOF chain"}
 
 
-def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample):
+def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample, autotune):
     """MC steps/s: all chains of all ligands in one launch (hipGraph replays) vs one
     launch per ligand (the latter on the first `per_lig_sample` ligands)."""
     import torch
-    from mmo_amd import ff, synth
+    from mmo_amd import ff, synth, tune
     from mmo_amd import mc as mcmod
-    rng = np.random.default_rng(4)
     flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
     roi = (np.zeros(3), 6.0)
 
-    def make(lig_ids):
+    def make(lig_ids, seed=4):
+        rng = np.random.default_rng(seed)
         cl = np.repeat(np.asarray(lig_ids), chains_per_lig)
         rot0 = synth.random_rotations(rng, len(cl))
         pos0 = rng.uniform(-1.0, 1.0, (len(cl), 3))
         return mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.SHIFTED, total_frames=frames, flags=flags,
                                 roi_center=roi[0], roi_r=roi[1], seed=2024)
 
+    key = tune.key_for("mc", ligands=len(ids), chains=chains_per_lig)
+    if autotune:
+        log("autotune MC residency")
+        tuned = tune.tune_mc(ctx, lambda: make(ids[:256], seed=5), frames=50,
+                             key_sig={"ligands": len(ids), "chains": chains_per_lig})["best"]
+    else:
+        tuned = tune.apply(ctx, key)
     m = make(ids)
     nrep = 4 if frames % 4 == 0 else 1
     torch.cuda.synchronize()
@@ -238,6 +245,7 @@ def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample):
             "accept_rate": float(st["accepted"].sum() / max(steps_b, 1.0)),
             "s_batched": tb, "s_per_ligand": tp,
             "per_ligand_sample": "%d ligands x %d chains" % (len(sub), chains_per_lig),
+            "tuned": tuned, "mc_waves_per_cu": ctx.tuning("mc_waves_per_cu"),
             "launch": "one persistent launch per hipGraph replay, %d replays of %d frames"
                       % (nrep, frames // nrep)}
 
@@ -257,6 +265,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this size first (counter scale reference)")
     ap.add_argument("--tune", default="", help="kernel shape, e.g. cull_k=1,cull_waves=16")
+    ap.add_argument("--autotune", action="store_true",
+                    help="sweep kernel shapes first (untimed) and record them in mmo_amd/tuned.json;"
+                         " without it the recorded shapes are applied")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the scoring path is built and measured on a single "
@@ -271,13 +282,19 @@ def main():
     rots = synth.so3_rotations(args.rotations)
     trans = synth.lattice((0.0, 0.0, 0.0), N_TRANS_SIDE, TRANS_STEP)
     npose = len(rots) * len(trans)
+    from mmo_amd import tune as tuner
     ctx = ff.Context(0)
+    scan_key = tuner.key_for("scan", lig_atoms=len(lig["xs"]), rec_atoms=len(rec["xs"]))
+    tuned = None if args.autotune else tuner.apply(ctx, scan_key)
     tune = dict(kv.split("=") for kv in args.tune.split(",") if kv)
     if "cell_size" in tune:
         ctx.tune(cell_size=float(tune.pop("cell_size")))
     ctx.set_receptor(rec)
     ctx.tune(**{k: float(v) for k, v in tune.items()})
     lid = ctx.add_ligand(lig)
+    if args.autotune:
+        log("autotune scan shapes on %d rotations" % min(256, len(rots)))
+        tuned = tuner.tune_scan(ctx, rec, lid, rots[:256], trans)["best"]
     out = torch.empty(npose, dtype=torch.float64, device="cuda:0")
     if args.copy_probe_mib > 0:
         n = args.copy_probe_mib * (1 << 20) // 4
@@ -320,6 +337,8 @@ def main():
                      "traffic": measured_traffic(kernel + " scan"),
                      "kernel": kernel + " (scan)", "kernel_ms": avg_ms,
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
+        "tuning": {"applied": tuned, "cell_size": ctx.tuning("cell_size"),
+                   "cull_k": ctx.tuning("cull_k"), "cull_waves": ctx.tuning("cull_waves")},
     }
     if not args.only_scan:
         extra = {}
@@ -332,7 +351,7 @@ def main():
         extra["egrad"] = bench_egrad(ctx, ligs, ids)
         log("Monte Carlo, batched vs per ligand")
         extra["mc"] = bench_mc(ctx, ligs, ids, args.mc_chains, args.mc_frames,
-                               args.mc_per_ligand_sample)
+                               args.mc_per_ligand_sample, args.autotune)
         line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lig, rec, rots, trans)

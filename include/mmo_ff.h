@@ -6,7 +6,7 @@
  * arrays (src/mol.ml:17-35), consumed as closures `ene_inter : Mol.t -> float` and
  * `ene_intra : Mol.t -> float` (src/lds.ml:702-739, :1952-1983).  Each entry point
  * below names the reference function it replaces.  An OCaml binding is a thin
- * `external` stub per entry point.
+ * `external` stub per entry point over these symbols.
  *
  * Conventions
  *   - All functions return MMO_OK (0) or a negative MMO_ERR_*; mmo_last_error()
