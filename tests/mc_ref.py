@@ -150,11 +150,11 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
         else:
             conf_step += 1
         if trace is not None:
-            trace.append({"frame": frame, "e_inter": prev_inter, "e_intra": prev_intra,
+            trace.append({"frame": frame, "e_inter": prev_inter, "e_intra": prev_intra, "e_prop": cur,
                           "e": prev_e, "best_e": best_e, "accepted": accepted,
                           "resets": resets, "pos": pos.copy(), "rot": rot.copy(),
                           "conf": conf.copy(), "center": np.array(center), "pose": pose.copy()})
     return {"best_e": best_e, "e": prev_e, "e_inter": prev_inter, "e_intra": prev_intra,
-            "rot": rot, "pos": pos, "conf": conf, "accepted": accepted, "resets": resets,
+            "rot": rot, "pos": pos, "conf": conf, "center": np.array(center), "accepted": accepted, "resets": resets,
             "ooroi": ooroi, "status": status, "max_rot": max_rot, "max_trans": max_trans,
             "best_rot": best_rot, "best_pos": best_pos, "dr": dr}

@@ -3,6 +3,7 @@ tests."""
 import math
 
 import numpy as np
+import pytest
 
 from mmo_amd.mc import frame_uniforms, philox4x32
 
@@ -35,3 +36,46 @@ def test_cpu_restatement_runs_and_accepts_with_hard_roi(config0):
     r2 = simulate(lig, sub, np.eye(3), np.array([9.0, 0.0, 0.0]), 30, seed=3, chain=0)
     assert r2["accepted"] == 0   # no acceptance test without hard ROI (src/lds.ml:908-917)
     assert math.isfinite(r2["best_e"])
+
+
+def _small_system(config0):
+    lig, rec = config0
+    sub = {k: np.asarray(rec[k])[:400] for k in ("xs", "ys", "zs", "q", "anum")}
+    return lig, sub
+
+
+def _energy_of_state(lig, rec, r):
+    import oracle
+    x = (r["conf"] - r["center"]) @ r["rot"].T + r["pos"]
+    m = dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+    c = dict(lig, xs=r["conf"][:, 0], ys=r["conf"][:, 1], zs=r["conf"][:, 2])
+    return oracle.ene_inter(rec, m) + oracle.ene_intra(c)
+
+
+def test_restatement_state_energy_and_best(config0):
+    """The chain's current energy is the energy of its current pose and conformer, and
+    the best energy is the minimum over the start pose and every proposal scored."""
+    from tests.mc_ref import simulate
+    lig, rec = _small_system(config0)
+    tr = []
+    start = np.array([9.0, 0.0, 0.0])
+    r = simulate(lig, rec, np.eye(3), start, 40, seed=11, chain=2, hard_roi=(start, 50.0), trace=tr)
+    assert r["resets"] == 0 and r["accepted"] > 0
+    assert r["e"] == pytest.approx(_energy_of_state(lig, rec, r), rel=1e-12)
+    r0 = simulate(lig, rec, np.eye(3), start, 0, seed=11, chain=2, hard_roi=(start, 50.0))
+    assert r["best_e"] == min([r0["e"]] + [t["e_prop"] for t in tr])
+
+
+def test_restatement_metropolis_limits(config0):
+    """beta -> infinity: accepted energies never increase; beta = 0: every proposal
+    inside the ROI is accepted."""
+    from tests.mc_ref import simulate
+    lig, rec = _small_system(config0)
+    start = np.array([9.0, 0.0, 0.0])
+    tr = []
+    simulate(lig, rec, np.eye(3), start, 40, seed=5, chain=0, hard_roi=(start, 50.0),
+             beta=1e300, trace=tr)
+    e = [t["e"] for t in tr]
+    assert all(b <= a for a, b in zip(e, e[1:]))
+    r = simulate(lig, rec, np.eye(3), start, 40, seed=5, chain=0, hard_roi=(start, 50.0), beta=0.0)
+    assert r["accepted"] == 40 and r["resets"] == 0
