@@ -230,6 +230,30 @@ int mmo_mc_get(mmo_mc* mc, double* best_e, double* cur_e, double* best_rot, doub
 int mmo_mc_coords(mmo_mc* mc, double* conf, double* best_xyz, double* rb_dr);
 void mmo_mc_destroy(mmo_mc* mc);
 
+/* ------------------------------------------------------- local minimisation */
+/* Batched L-BFGS in DOF space.  The reference minimises E_intra + E_inter of
+ * Optim.apply_config(dofs) with NLopt inside box bounds, gradients by central
+ * differences (Lds.minimize_ligand / nlopt_minimize / approx_local_gradient,
+ * src/lds.ml:1014-1036, :1118-1297; bounds src/optim.ml:34-60).  Here every pose of
+ * the batch runs limited-memory BFGS (history 1..16 pairs) on the analytic gradient,
+ * with Armijo backtracking, trial points projected onto [lo, hi] (host arrays laid out
+ * like the batch's DOFs, nullable = unbounded), first trial steps capped at max_step
+ * per component (A or rad; <= 0: 1.0) and convergence at max |dE/dDOF| <= gtol
+ * (<= 0: 1e-6).  flags: one of MMO_F_INTER_SHIFTED / _GLOBAL / _INTERP, and/or
+ * MMO_F_INTRA.  Poses longer than 12 A after their torsions (Mol.Too_long) or with a
+ * non-finite energy count as failed trials (the reference abandons the ligand). */
+typedef struct mmo_min mmo_min;
+int mmo_min_create(mmo_batch* b, int history, int flags, const double* lo, const double* hi,
+                   double max_step, double gtol, mmo_min** out);
+/* Minimises from dofs (device, in/out: start point in, best point out); each of the
+ * iters iterations is one energy + gradient evaluation of every pose followed by one
+ * L-BFGS update.  e_out (device, npose, nullable): best energies.  Enqueued on stream. */
+int mmo_min_run_dev(mmo_min* mz, double* dofs, int iters, double* e_out, void* stream);
+/* After a run (synchronises): info[4p..] = status (0 running, 1 converged, 2 failed
+ * start), evaluations, accepted steps, history restarts; e_start[p]: start energy. */
+int mmo_min_get(mmo_min* mz, int32_t* info, double* e_start);
+void mmo_min_destroy(mmo_min* mz);
+
 /* ------------------------------------------------------------------ tuning */
 /* Kernel time of the last launch of `kernel` ("inter", "interp", "intra", "grid") measured
  * with HIP events on the launch stream, in milliseconds (-1 if none). */

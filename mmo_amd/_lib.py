@@ -57,6 +57,10 @@ SIGNATURES = {
     "mmo_mc_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
     "mmo_mc_coords": (_i, [_p, _p, _p, _p]),
     "mmo_mc_destroy": (None, [_p]),
+    "mmo_min_create": (_i, [_p, _i, _i, _p, _p, _d, _d, _pp]),
+    "mmo_min_run_dev": (_i, [_p, _p, _i, _p, _p]),
+    "mmo_min_get": (_i, [_p, _p, _p]),
+    "mmo_min_destroy": (None, [_p]),
     "mmo_last_kernel_ms": (_d, [_p, ct.c_char_p]),
     "mmo_fixed_point_bits": (_i, [_p]),
     "mmo_tune_set": (_i, [_p, ct.c_char_p, _d]),

@@ -1400,6 +1400,126 @@ void mmo_mc_destroy(mmo_mc* m) {
   delete m;
 }
 
+// ------------------------------------------------------------- L-BFGS minimiser
+struct mmo_min {
+  mmo_batch* b = nullptr;
+  int m = 8, flags = 0;
+  double max_step = 1.0, gtol = 1e-6;
+  DevBuf<double> x, g, d, xt, gt, e, rad, S, Y, rho, lo, hi;
+  DevBuf<MinPose> st;
+};
+
+int mmo_min_create(mmo_batch* b, int history, int flags, const double* lo, const double* hi,
+                   double max_step, double gtol, mmo_min** out) {
+  if (!b || !out) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL batch or output");
+  mmo_ctx* c = b->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (history < 1 || history > kMinMaxHist)
+    return fail(c, MMO_ERR_BAD_ARG, "history %d outside 1..%d", history, kMinMaxHist);
+  const int inter = flags & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL | MMO_F_INTER_INTERP);
+  if ((inter & (inter - 1)) || (flags & ~(inter | MMO_F_INTRA)) || !(flags & (inter | MMO_F_INTRA)))
+    return fail(c, MMO_ERR_BAD_ARG, "flags: at most one E_inter form, optionally MMO_F_INTRA");
+  for (int p = 0; p < b->npose; ++p)
+    if (b->dof_off[p + 1] - b->dof_off[p] > kWave)
+      return fail(c, MMO_ERR_BAD_ARG, "pose %d has more than %d DOFs", p, kWave);
+  mmo_min* z = new mmo_min;
+  z->b = b;
+  z->m = history;
+  z->flags = flags;
+  if (max_step > 0.0) z->max_step = max_step;
+  if (gtol > 0.0) z->gtol = gtol;
+  const size_t nd = std::max(b->ndof, 1), np = std::max(b->npose, 1);
+  hipStream_t s = c->stream;
+  hipError_t e = hipSuccess;
+  for (DevBuf<double>* v : {&z->x, &z->g, &z->d, &z->xt, &z->gt})
+    if (e == hipSuccess) e = v->ensure(nd);
+  if (e == hipSuccess) e = z->e.ensure(4 * np);
+  if (e == hipSuccess) e = z->rad.ensure(np);
+  if (e == hipSuccess) e = z->S.ensure((size_t)history * nd);
+  if (e == hipSuccess) e = z->Y.ensure((size_t)history * nd);
+  if (e == hipSuccess) e = z->rho.ensure((size_t)history * np);
+  if (e == hipSuccess) e = z->st.ensure(np);
+  if (e == hipSuccess && lo) e = z->lo.ensure(nd);
+  if (e == hipSuccess && hi) e = z->hi.ensure(nd);
+  if (e == hipSuccess && lo) e = hipMemcpyAsync(z->lo.p, lo, (size_t)b->ndof * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && hi) e = hipMemcpyAsync(z->hi.p, hi, (size_t)b->ndof * sizeof(double), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    mmo_min_destroy(z);
+    return fail(c, MMO_ERR_NOMEM, "minimiser buffers: %s", hipGetErrorString(e));
+  }
+  *out = z;
+  return MMO_OK;
+}
+
+int mmo_min_run_dev(mmo_min* z, double* dofs, int iters, double* e_out, void* stream) {
+  if (!z) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL minimiser");
+  mmo_batch* b = z->b;
+  mmo_ctx* c = b->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!dofs || iters < 0) return fail(c, MMO_ERR_BAD_ARG, "bad minimiser arguments");
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(c, hipMemcpyAsync(z->xt.p, dofs, (size_t)b->ndof * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(z->x.p, dofs, (size_t)b->ndof * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, launch_min_reset(z->st.p, b->npose, s));
+  MinArgs a{};
+  a.npose = b->npose; a.m = z->m; a.ndof_total = b->ndof; a.dof_off = b->d_dof_off.p;
+  a.x = z->x.p; a.g = z->g.p; a.d = z->d.p; a.xt = z->xt.p; a.gt = z->gt.p; a.et = z->e.p;
+  a.rad = z->rad.p; a.S = z->S.p; a.Y = z->Y.p; a.rho = z->rho.p; a.st = z->st.p;
+  a.lo = z->lo.p; a.hi = z->hi.p;
+  a.max_step = z->max_step; a.min_step = 1e-10; a.gtol = z->gtol; a.c1 = 1e-4;
+  for (int it = 0; it < iters; ++it) {
+    r = mmo_batch_score_dofs_dev(b, z->flags | MMO_F_GRAD, z->xt.p, nullptr, z->e.p, z->gt.p,
+                                 z->rad.p, stream);
+    if (r) return r;
+    HIPCHK(c, launch_lbfgs(a, s));
+  }
+  if (iters > 0) {
+    HIPCHK(c, hipMemcpyAsync(dofs, z->x.p, (size_t)b->ndof * sizeof(double),
+                             hipMemcpyDeviceToDevice, s));
+    if (e_out) HIPCHK(c, launch_min_energy(z->st.p, b->npose, e_out, s));
+  }
+  return MMO_OK;
+}
+
+int mmo_min_get(mmo_min* z, int32_t* info, double* e_start) {
+  if (!z) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL minimiser");
+  mmo_ctx* c = z->b->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  const int np = z->b->npose;
+  std::vector<MinPose> st(np);
+  HIPCHK(c, hipDeviceSynchronize());
+  if (np) HIPCHK(c, hipMemcpy(st.data(), z->st.p, np * sizeof(MinPose), hipMemcpyDeviceToHost));
+  for (int p = 0; p < np; ++p) {
+    if (info) {
+      info[4 * p] = st[p].status;
+      info[4 * p + 1] = st[p].n_eval;
+      info[4 * p + 2] = st[p].n_acc;
+      info[4 * p + 3] = st[p].n_reset;
+    }
+    if (e_start) e_start[p] = st[p].f0;
+  }
+  return MMO_OK;
+}
+
+void mmo_min_destroy(mmo_min* z) {
+  if (!z) return;
+  if (getpid() == z->b->ctx->pid) {
+    (void)hipSetDevice(z->b->ctx->device);
+    (void)hipDeviceSynchronize();
+    for (DevBuf<double>* v : {&z->x, &z->g, &z->d, &z->xt, &z->gt, &z->e, &z->rad, &z->S, &z->Y,
+                              &z->rho, &z->lo, &z->hi})
+      v->release();
+    z->st.release();
+  }
+  delete z;
+}
+
 double mmo_last_kernel_ms(mmo_ctx* c, const char* kernel) {
   if (!c || !kernel) return -1.0;
   auto it = c->last_ms.find(kernel);

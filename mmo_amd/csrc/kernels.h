@@ -110,4 +110,34 @@ hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipSt
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
 
+// ------------------------------------------------------------ L-BFGS minimiser
+constexpr int kMinMaxHist = 16;
+enum { kMinRunning = 0, kMinConverged = 1, kMinFailed = 2 };
+struct MinPose {
+  double f, f0, alpha, gamma;   // energy at x, at the start, step length, H0 scale
+  int init, hist, head, status;
+  int n_eval, n_acc, n_reset, pad;
+};
+struct MinArgs {
+  int npose, m, ndof_total;
+  const int* dof_off;
+  double* x;            // accepted point (the best point)
+  double* g;            // gradient at x
+  double* d;            // search direction
+  double* xt;           // trial point: the input of the next evaluation
+  const double* gt;     // gradient at the trial point
+  const double* et;     // energies at the trial point (npose x 4: inter, intra, ...)
+  const double* rad;    // enclosing radius after the torsions
+  double* S;            // history (m x ndof_total)
+  double* Y;
+  double* rho;          // m x npose
+  MinPose* st;
+  const double* lo;     // box bounds (ndof_total) or null
+  const double* hi;
+  double max_step, min_step, gtol, c1;
+};
+hipError_t launch_lbfgs(const MinArgs& a, hipStream_t s);
+hipError_t launch_min_reset(MinPose* st, int npose, hipStream_t s);
+hipError_t launch_min_energy(const MinPose* st, int npose, double* e_out, hipStream_t s);
+
 }  // namespace mmo

@@ -648,3 +648,98 @@ def test_mc_interp_mode_matches_cpu_restatement():
                        inter_fn=inter)
         assert st["accepted"][k] == ref["accepted"]
         assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-9, abs=1e-9)
+
+
+# ------------------------------------------------------------------ L-BFGS minimiser
+def _one_atom_system(anum=6):
+    ctx = ff.Context(0)
+    rec = {"xs": np.zeros(1), "ys": np.zeros(1), "zs": np.zeros(1), "q": np.zeros(1),
+           "anum": np.array([anum], dtype=np.int32)}
+    ctx.set_receptor(rec)
+    lig = {"xs": np.zeros(1), "ys": np.zeros(1), "zs": np.zeros(1), "q": np.zeros(1),
+           "anum": np.array([anum], dtype=np.int32), "dists": np.zeros((1, 1), dtype=np.int32),
+           "rb_left": [], "rb_right": [], "rg_off": [0], "rg_atoms": []}
+    return ctx, rec, lig
+
+
+def test_minimizer_lj_closed_form():
+    """One C atom against one C atom: the global form's minimum sits at r = x_CC = 3.851 A
+    with E = -D_CC = -0.105 kcal/mol (UFF puts the LJ minimum at x_ij); the shifted
+    form's minimum is the 1-D minimum of S(r) LJ(r), found here by bounded Brent on the
+    oracle's pair energy."""
+    import torch
+    from scipy.optimize import minimize_scalar
+    from mmo_amd.minimize import CONVERGED, RUNNING, Minimizer
+    ctx, rec, lig = _one_atom_system()
+    lid = ctx.add_ligand(lig)
+    npose = 64
+    b = ctx.batch([lid] * npose)
+    rng = np.random.default_rng(2)
+    dirs = rng.normal(size=(npose, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    r0 = rng.uniform(3.3, 6.0, npose)
+    start = np.concatenate([np.concatenate([r0[p] * dirs[p], rng.uniform(-1, 1, 3)]) for p in range(npose)])
+
+    def shifted_pair(r):
+        m = dict(lig, xs=np.array([r]), ys=np.zeros(1), zs=np.zeros(1))
+        return oracle.ene_inter(rec, m, shifted=True)
+    rs = minimize_scalar(shifted_pair, bounds=(3.0, 5.0), method="bounded",
+                         options={"xatol": 1e-10}).x
+    for flags, r_min, e_min in ((ff.F_INTER_GLOBAL, 3.851, -0.105),
+                                (ff.F_INTER_SHIFTED, rs, shifted_pair(rs))):
+        d = torch.tensor(start, device="cuda:0")
+        mz = Minimizer(b, flags=flags, history=8, max_step=0.5, gtol=1e-9)
+        print("minimizer one-atom flags", flags, flush=True)
+        e = mz.run(d, 200).cpu().numpy()
+        x = d.cpu().numpy().reshape(npose, 6)
+        info = mz.info()
+        r = np.linalg.norm(x[:, :3], axis=1)
+        # FP32 pair energies resolve E to ~1e-8 kcal/mol near the minimum, where
+        # E'' = 72 D / x^2 = 0.51 kcal/mol/A^2: positions to ~3e-4 A
+        assert np.all(np.abs(r - r_min) < 1e-3), np.abs(r - r_min).max()
+        assert np.allclose(e, e_min, rtol=1e-5, atol=1e-9)
+        assert np.all(e <= info["e_start"])
+        # near the minimum FP32 energy noise keeps some poses backtracking until the
+        # iteration limit; none may fail
+        assert np.all((info["status"] == CONVERGED) | (info["status"] == RUNNING))
+
+
+def test_minimizer_flexible_ligands(lib_sys):
+    """Library ligands near the receptor: every pose ends at or below its start energy,
+    the reported energy is the oracle's energy of the returned DOFs, and the returned
+    DOFs stay inside the ROI / angle bounds."""
+    import torch
+    from mmo_amd.minimize import Minimizer, roi_bounds
+    ctx, ligs, ids, rec, _ = lib_sys
+    pick = [k for k in range(len(ligs)) if 1 <= len(ligs[k]["rb_left"]) <= 8][:4]
+    pose_lig = np.array([ids[k] for k in pick for _ in range(4)])
+    b = ctx.batch(pose_lig)
+    center, radius = np.array([30.0, 0.0, 0.0]), 3.0
+    nrb = [len(ligs[l]["rb_left"]) for l in pose_lig]
+    lo, hi = roi_bounds(b, center, radius, nrb)
+    rng = np.random.default_rng(21)
+    start = []
+    for p, l in enumerate(pose_lig):
+        start.append(np.concatenate([center + rng.uniform(-2, 2, 3), rng.uniform(-3, 3, 3),
+                                     rng.uniform(-0.3, 0.3, nrb[p])]))
+    d = torch.tensor(np.concatenate(start), device="cuda:0")
+    mz = Minimizer(b, flags=ff.F_INTER_SHIFTED | ff.F_INTRA, history=8, bounds=(lo, hi),
+                   max_step=0.2)
+    print("minimizer flexible", b.npose, flush=True)
+    e = mz.run(d, 150).cpu().numpy()
+    x = d.cpu().numpy()
+    info = mz.info()
+    assert np.all(x >= lo - 1e-12) and np.all(x <= hi + 1e-12)
+    assert np.all(e <= info["e_start"] + 1e-12)
+    assert np.mean(info["e_start"] - e) > 0.0
+    refs, sums = [], []
+    for p, l in enumerate(pose_lig):
+        lg = ligs[l]
+        c = np.stack([lg["xs"], lg["ys"], lg["zs"]], 1)
+        dof = x[b.dof_off[p]:b.dof_off[p + 1]]
+        xyz, rad = oracle.apply_config(c, lg["rb_left"], lg["rb_right"], lg["rg_off"], lg["rg_atoms"], dof)
+        m = dict(lg, xs=xyz[:, 0], ys=xyz[:, 1], zs=xyz[:, 2])
+        refs.append(oracle.ene_inter(rec, m) + oracle.ene_intra(m))
+        sums.append(inter_abs_sum(rec, xyz, m) + intra_abs_sum(m, xyz))
+        assert rad <= 12.0
+    assert_energy(e, refs, sums, what="minimised poses")
