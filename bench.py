@@ -12,6 +12,7 @@ Secondary figures (the "extra" object), each on its own workload:
                64^3 at 0.375 A, precomputed on the GPU (precompute time reported);
   egrad        energy + adjoint gradient in DOF space for 4,096 poses of 1,024 library
                ligands: one batched launch sequence vs one per ligand;
+  min          200 batched L-BFGS iterations (energy + gradient + update) on those poses;
   mc           Monte Carlo chains, 64 per ligand for 1,024 library ligands, 1,000 frames,
                batched persistent launch replayed from a hipGraph vs one launch per ligand.
 
@@ -196,6 +197,41 @@ def bench_egrad(ctx, ligs, ids, poses_per_lig=4, reps=20):
             "dtype": "fp32 pairs, fp64 per-pose sums and DOF chain"}
 
 
+def bench_min(ctx, ligs, ids, poses_per_lig=4, iters=200):
+    """200 L-BFGS iterations (one energy + gradient evaluation and one update each) for
+    4 poses of each library ligand, started near the receptor inside a 4 A ROI box."""
+    import torch
+    from mmo_amd import ff
+    from mmo_amd.minimize import Minimizer, roi_bounds
+    rng = np.random.default_rng(6)
+    pose_lig = np.repeat(np.asarray(ids), poses_per_lig)
+    nrb = [len(ligs[ids.index(l)]["rb_left"]) for l in pose_lig]
+    b = ctx.batch(pose_lig)
+    center = np.array([30.0, 0.0, 0.0])
+    lo, hi = roi_bounds(b, center, 4.0, nrb)
+    d0 = np.concatenate([np.concatenate([center + rng.uniform(-2, 2, 3), rng.uniform(-3, 3, 3),
+                                         rng.uniform(-0.2, 0.2, nrb[p])]) for p in range(len(pose_lig))])
+    d = torch.tensor(d0, device="cuda:0")
+    mz = Minimizer(b, flags=ff.F_INTER_SHIFTED | ff.F_INTRA, history=8, bounds=(lo, hi), max_step=0.2)
+    mz.run(d, 2)                                     # warm-up
+    d.copy_(torch.tensor(d0, device="cuda:0"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = mz.run(d, iters)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    info = mz.info()
+    e = e.cpu().numpy()
+    n = len(pose_lig)
+    ok = np.isfinite(info["e_start"])
+    return {"pose_evals_per_s": n * iters / dt, "unit": "pose energy+gradient evaluations/s",
+            "poses": n, "iterations": iters, "s": dt,
+            "median_e_start": float(np.median(info["e_start"][ok])),
+            "median_e_end": float(np.median(e[ok])),
+            "failed_start": int((~ok).sum()), "converged": int((info["status"] == 1).sum()),
+            "mean_accepted_steps": float(np.mean(info["accepted"]))}
+
+
 def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample, autotune):
     """MC steps/s: all chains of all ligands in one launch (hipGraph replays) vs one
     launch per ligand (the latter on the first `per_lig_sample` ligands)."""
@@ -258,6 +294,8 @@ def main():
     ap.add_argument("--rotations", type=int, default=N_ROT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-scan", action="store_true", help="headline scan only (no extras)")
+    ap.add_argument("--extras", default="interp,egrad,min,mc",
+                    help="comma list of secondary figures to run: interp, egrad, min, mc")
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--mc-chains", type=int, default=64)
     ap.add_argument("--mc-frames", type=int, default=1000)
@@ -340,18 +378,26 @@ def main():
         "tuning": {"applied": tuned, "cell_size": ctx.tuning("cell_size"),
                    "cull_k": ctx.tuning("cull_k"), "cull_waves": ctx.tuning("cull_waves")},
     }
-    if not args.only_scan:
+    want = set(x for x in args.extras.split(",") if x) if not args.only_scan else set()
+    if want:
         extra = {}
-        log("interp scan")
-        extra["scan_interp"] = bench_interp(ctx, lig, rots, trans, args.steps, args.warmup)
-        log("library of %d ligands" % args.ligands)
-        ligs, _ = library_system(args.ligands)
-        ids = [ctx.add_ligand(l) for l in ligs]
-        log("energy+gradient, batched vs per ligand")
-        extra["egrad"] = bench_egrad(ctx, ligs, ids)
-        log("Monte Carlo, batched vs per ligand")
-        extra["mc"] = bench_mc(ctx, ligs, ids, args.mc_chains, args.mc_frames,
-                               args.mc_per_ligand_sample, args.autotune)
+        if "interp" in want:
+            log("interp scan")
+            extra["scan_interp"] = bench_interp(ctx, lig, rots, trans, args.steps, args.warmup)
+        if want & {"egrad", "min", "mc"}:
+            log("library of %d ligands" % args.ligands)
+            ligs, _ = library_system(args.ligands)
+            ids = [ctx.add_ligand(l) for l in ligs]
+        if "egrad" in want:
+            log("energy+gradient, batched vs per ligand")
+            extra["egrad"] = bench_egrad(ctx, ligs, ids)
+        if "min" in want:
+            log("L-BFGS minimisation")
+            extra["min"] = bench_min(ctx, ligs, ids)
+        if "mc" in want:
+            log("Monte Carlo, batched vs per ligand")
+            extra["mc"] = bench_mc(ctx, ligs, ids, args.mc_chains, args.mc_frames,
+                                   args.mc_per_ligand_sample, args.autotune)
         line["extra"] = extra
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lig, rec, rots, trans)

@@ -82,6 +82,7 @@ struct mmo_ctx {
   DevBuf<RecAB> rab;
   // receptor cell list (culled shifted kernel)
   float cell_size = 5.0f;
+  int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
   bool cull_ok = false;
   int scan_cull = 1;
   int cull_k = 1;
@@ -881,7 +882,29 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
   // gradient buffer starts at zero; every producer adds
   if (want_grad) HIPCHK(c, launch_fill(grad, 3 * (size_t)b->natoms, 0.0, s));
   HIPCHK(c, launch_fill(e_out, 4 * (size_t)b->npose, 0.0, s));
-  if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
+  if ((inter & MMO_F_INTER_SHIFTED) && c->pose_cull) {
+    // one wavefront per pose, receptor culled per 8-atom sub-block
+    InterLaunch L = inter_launch(c, MMO_SHIFTED, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    // the receptor goes to LDS when enough poses per workgroup amortise the copy
+    L.cull = (c->cull_ok && b->npose >= 1024) ? 1 : 2;
+    PoseCullArgs a{};
+    a.xyz = xyz; a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
+    a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
+    const bool comp = (flags & MMO_F_COMPONENTS) != 0;
+    const int waves = std::max(1, std::min(c->cull_waves, (b->npose + 255) / 256));
+    timer_begin(c, "inter", s);
+    if (!want_grad || comp) {
+      a.e_comp = comp ? e_out : nullptr;
+      HIPCHK(c, launch_pose_cull(L, a, false, waves, s));
+    }
+    if (want_grad) {
+      a.e_comp = nullptr;
+      a.grad = grad;
+      a.write_e = comp ? 0 : 1;
+      HIPCHK(c, launch_pose_cull(L, a, true, waves, s));
+    }
+    timer_end(c, "inter", s);
+  } else if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
     const int mode = (inter & MMO_F_INTER_SHIFTED) ? MMO_SHIFTED : MMO_GLOBAL;
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
     if (mode == MMO_SHIFTED && b->ntiles >= 1024) {
@@ -1553,6 +1576,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "scan_cull" && (iv == 0 || iv == 1)) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
+  else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
   else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
   return MMO_OK;
 }
@@ -1569,6 +1593,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "cell_size") return c->cell_size;
   if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;
   if (k == "mc_waves_per_cu") return c->mc_waves_per_cu;
+  if (k == "pose_cull") return c->pose_cull;
   return -1.0;
 }
 

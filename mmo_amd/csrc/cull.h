@@ -31,6 +31,34 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, in
   }
 }
 
+// shifted-form energy and dE/dr2 of one pair; the gradient is accumulated as
+// coef * (x_rec - x_lig) in fixed-point units (the caller scales by -2 * 2^-k).
+// F = ke qi qj / r + Aij u^6 - 2 Bij u^3, dF/dr2 = -u (ke qi qj / (2 r) + 6 Aij u^6 - 6 Bij u^3),
+// w = t^2 with t = 1 - r2/144, dw/dr2 = -t/72.
+__device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& ab, int lx, int ly,
+                                                  int lz, float inv_s2, float c1, float aj,
+                                                  float bj, float& e, float& gx, float& gy,
+                                                  float& gz) {
+  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
+  const float r2c = fmaxf(r2, kMinR2);
+  const float ir = __builtin_amdgcn_rsqf(r2c);
+  const float u = ir * ir;
+  const float u3 = u * u * u;
+  const float u6 = u3 * u3;
+  const float eq = p.q * ir, e12 = ab.a * u6, e6 = ab.b * u3;
+  const float F = fmaf(c1, eq, fmaf(aj, e12, -2.0f * bj * e6));
+  const float G = fmaf(0.5f * c1, eq, 6.0f * fmaf(aj, e12, -bj * e6));
+  const float t = shift_t(r2c);
+  const float w = t * t;
+  float coef = fmaf(t * F, -1.0f / 72.0f, -(w * u) * G);
+  coef = (r2 >= kMinR2) ? coef : 0.0f;   // clamped distance: flat
+  e = fmaf(w, F, e);
+  gx = fmaf(coef, dx, gx);
+  gy = fmaf(coef, dy, gy);
+  gz = fmaf(coef, dz, gz);
+}
+
 // ------------------------------------------------------- inter kernel, culled
 // Shifted form only.  The whole receptor sits in LDS sorted by 4 A cells.  For each
 // block of K atoms per lane the wavefront takes the bounding box of its lanes' atom
@@ -210,6 +238,104 @@ __device__ __forceinline__ bool wave_query(const int (&lx)[K], const int (&ly)[K
   const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
   R = uniformf(12.0f + sqrtf(hx * hx + hy * hy + hz * hz) + 0.05f);
   return true;
+}
+
+// ------------------------------------------------- one pose per wavefront, culled
+// The atoms of one pose are taken kSub = 8 at a time; lane = a + 8 r holds atom slot a
+// of the sub-block and walks every 8th receptor atom (slot r) of the cell rows that
+// meet the sphere of radius 12 A + half-diagonal around the sub-block's bounding box.
+// Eight neighbouring atoms have a small box, so the sweep stays close to the pairs
+// inside the cutoff even though a whole pose spans up to 24 A.  The (z, y) cell rows of
+// the sphere's box are enumerated across the lanes, which fetch their atom ranges in
+// parallel (one memory latency per 64 rows, not one per row); the rows are then swept
+// in order, each lane reading the ranges back with readlane.  atom(j, lx, ly, lz, cq,
+// ca, cb) loads atom j; with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1 units,
+// FP64) once per atom.  Sums: FP32 per 64 rows, FP64 across them.
+constexpr int kSub = 8;
+
+template <bool GRAD, class AtomFn, class GradOut>
+__device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
+                                           const RecPos* __restrict__ s_pos,
+                                           const RecAB* __restrict__ s_ab, const Frame& fr,
+                                           int n_atoms, AtomFn&& atom, GradOut&& gout,
+                                           double& e_el, double& e_vd) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int aslot = lane & (kSub - 1), rslot = lane >> 3;
+  for (int j0 = 0; j0 < n_atoms; j0 += kSub) {
+    const int j = j0 + aslot;
+    int lx[1], ly[1], lz[1];
+    bool valid[1];
+    double cq = 0.0, ca = 0.0, cb = 0.0;
+    valid[0] = j < n_atoms;
+    lx[0] = ly[0] = lz[0] = 0;
+    if (valid[0]) atom(j, lx[0], ly[0], lz[0], cq, ca, cb);
+    float qx, qy, qz, R;
+    if (!wave_query<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
+    const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    const float R2 = R * R;
+    const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
+    const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
+    const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
+    const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
+    const int nyc = cy1 - cy0 + 1;
+    const int nrows = (cz1 >= cz0 && nyc > 0) ? (cz1 - cz0 + 1) * nyc : 0;
+    for (int base = 0; base < nrows; base += kWave) {
+      // lane k: row base + k -> its atom range [r0, r1) (empty if the row misses the sphere)
+      int r0 = 0, r1 = 0;
+      const int idx = base + lane;
+      if (idx < nrows) {
+        const int cz = cz0 + idx / nyc, cy = cy0 + idx % nyc;
+        const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
+        const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
+        const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
+        const float rx2 = R2 - dzb * dzb - dyb * dyb;
+        if (rx2 >= 0.0f) {
+          const float rx = sqrtf(rx2);
+          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
+          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+          if (cx0 <= cx1) {
+            const int rowb = (cz * cg.ny + cy) * cg.nx;
+            r0 = s_cell[rowb + cx0];
+            r1 = s_cell[rowb + cx1 + 1];
+          }
+        }
+      }
+      float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      const int nk = min(kWave, nrows - base);
+      for (int k = 0; k < nk; ++k) {
+        const int a0 = __builtin_amdgcn_readlane(r0, k);
+        const int a1 = __builtin_amdgcn_readlane(r1, k);
+        for (int i = a0 + rslot; i < a1; i += kSub) {
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
+          if (GRAD)
+            pair_grad_shifted(p, ab, lx[0], ly[0], lz[0], fr.inv_s2, c1, fa, fb, fe, fx, fy, fz);
+          else
+            pair_energy<MMO_SHIFTED>(p, ab, lx[0], ly[0], lz[0], fr.inv_s2, sq, s12, s6);
+        }
+      }
+      if (GRAD) {
+        e_vd += (double)fe;
+        gx += (double)fx; gy += (double)fy; gz += (double)fz;
+      } else {
+        e_el = fma((double)sq, cq, e_el);
+        e_vd = fma((double)s12, ca, e_vd);
+        e_vd = fma(-2.0 * (double)s6, cb, e_vd);
+      }
+    }
+    if (GRAD) {
+      // sum over the receptor slots of each atom slot
+      for (int off = kSub; off < kWave; off <<= 1) {
+        gx += __shfl_xor(gx, off, kWave);
+        gy += __shfl_xor(gy, off, kWave);
+        gz += __shfl_xor(gz, off, kWave);
+      }
+      // dE/dx_lig = -2 * sum coef * (x_rec - x_lig), differences in fixed-point units
+      const double gs = -2.0 * (double)fr.inv_s;
+      if (rslot == 0 && valid[0]) gout(j, gs * gx, gs * gy, gs * gz);
+    }
+  }
 }
 
 }  // namespace mmo

@@ -12,6 +12,8 @@
 //   k_interp  trilinear interpolation in per-type FP32 grids, src/G3D.ml:97-157 and
 //             src/mol.ml:1012-1020, FP64 weights.
 // Every kernel has an adjoint (GRAD) variant giving dE/dxyz of the ligand atoms.
+#include <algorithm>
+
 #include "common.h"
 #include "cull.h"
 #include "kernels.h"
@@ -335,6 +337,84 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
       a.e_comp[o + 3] = e_vd;
     }
   }
+}
+
+// ----------------------------------------------------- inter kernel, pose-culled
+// Shifted form for arbitrary poses (batches, gradients, minimisation): one wavefront per
+// pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep), so the
+// cost follows the pairs inside the cutoff wherever the poses are.
+template <bool GRAD, bool LDS>
+__global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, const RecPos* rpos,
+                                                    const RecAB* rab, int nrec, CellGrid cg) {
+  extern __shared__ __align__(16) char smem[];
+  const RecPos* s_pos;
+  const RecAB* s_ab;
+  const int* s_cell;
+  load_receptor<LDS>(rpos, rab, nrec, cg, s_pos, s_ab, s_cell, smem);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int p = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  if (p >= a.npose) return;
+  const LigInfo li = a.ligs[a.pose_lig[p]];
+  const size_t row0 = (size_t)a.atom_off[p];
+  const double* x = a.xyz + 3 * row0;
+  int sat = 0;
+  double e_el = 0.0, e_vd = 0.0;
+  pose_sweep<GRAD>(
+      cg, s_cell, s_pos, s_ab, fr, li.n_atoms,
+      [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
+        lx = to_fixed(x[3 * j], fr.ox, fr.scale, fr.lim, &sat);
+        ly = to_fixed(x[3 * j + 1], fr.oy, fr.scale, fr.lim, &sat);
+        lz = to_fixed(x[3 * j + 2], fr.oz, fr.scale, fr.lim, &sat);
+        const LigAtom la = a.latoms[li.atom_base + j];
+        cq = la.keq; ca = la.a; cb = la.b;
+      },
+      [&](int j, double gx, double gy, double gz) {
+        double* g = a.grad + 3 * (row0 + j);
+        g[0] += gx; g[1] += gy; g[2] += gz;
+      },
+      e_el, e_vd);
+  (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
+  // every lane of an atom slot holds part of the sums: the wavefront sum counts each
+  // pair once (lanes of one slot share the atom, split the receptor atoms)
+  e_el = wave_sum(e_el);
+  e_vd = wave_sum(e_vd);
+  if (lane == 0) {
+    const size_t o = (size_t)p * a.e_stride;
+    if (!GRAD || a.write_e) a.e_out[o] = e_el + e_vd;
+    if (!GRAD && a.e_comp) {
+      a.e_comp[o + 2] = e_el;
+      a.e_comp[o + 3] = e_vd;
+    }
+  }
+}
+
+hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
+                            hipStream_t s) {
+  if (a.npose == 0) return hipSuccess;
+  waves = std::max(1, std::min(16, waves));
+  const int blocks = (a.npose + waves - 1) / waves;
+  const bool in_lds = L.cull == 1;
+  const size_t lds = in_lds ? (size_t)L.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+                                  (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int)
+                            : 0;
+#define MMO_POSE_CULL(G)                                                                       \
+  if (in_lds) {                                                                                \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pose_cull<G, true>),   \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \
+    if (e != hipSuccess) return e;                                                             \
+    hipLaunchKernelGGL((k_pose_cull<G, true>), dim3(blocks), dim3(waves * kWave), lds, s, a,   \
+                       L.fr, L.rpos, L.rab, L.nrec, L.cg);                                     \
+  } else {                                                                                     \
+    hipLaunchKernelGGL((k_pose_cull<G, false>), dim3(blocks), dim3(waves * kWave), 0, s, a,    \
+                       L.fr, L.rpos, L.rab, L.nrec, L.cg);                                     \
+  }
+  if (grad) {
+    MMO_POSE_CULL(true)
+  } else {
+    MMO_POSE_CULL(false)
+  }
+#undef MMO_POSE_CULL
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------- grid precompute

@@ -107,6 +107,23 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
                               hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s);
+// one pose per wavefront, shifted form, receptor culled per 8-atom sub-block
+struct PoseCullArgs {
+  const double* xyz;        // rows (x, y, z), pose-major
+  const int* atom_off;
+  const int* pose_lig;
+  int npose;
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  double* e_out;            // e_out[pose * e_stride] = E_inter
+  int e_stride;
+  double* e_comp;           // nullable: [pose*e_stride + 2] elec, [+3] vdW
+  double* grad;             // GRAD: dE/dxyz rows, added to what the buffer holds
+  int write_e;
+};
+hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
+                            hipStream_t s);
+
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
 

@@ -129,13 +129,30 @@ __device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const Pose
   return bad ? __builtin_nan("") : e;
 }
 
-// shifted direct sum, lanes over atoms, receptor cell rows culled by the pose's box
+// shifted direct sum of the pose: FP32 pairs swept per 8-atom sub-block (pose_sweep);
+// the FP64 parity mode sweeps with FP64 pair arithmetic, two atoms per lane
 __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v) {
+  const Frame fr = a.fr;
+  int sat = 0;
+  if (!(a.flags & kMcFp64)) {
+    double e_el = 0.0, e_vd = 0.0;
+    pose_sweep<false>(
+        a.cg, a.cg.cell_start, a.rpos, a.rab, fr, li.n_atoms,
+        [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
+          double x, y, z;
+          v.at(j, x, y, z);
+          lx = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
+          ly = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
+          lz = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+          const LigAtom la = a.latoms[li.atom_base + j];
+          cq = la.keq; ca = la.a; cb = la.b;
+        },
+        [](int, double, double, double) {}, e_el, e_vd);
+    return wave_sum(e_el + e_vd);
+  }
   constexpr int K = 2;
   const int lane = threadIdx.x & (kWave - 1);
-  const Frame fr = a.fr;
-  double e_el = 0.0, e_vd = 0.0;
-  int sat = 0;
+  double e_vd = 0.0;
   const int nblk = (li.n_atoms + kWave * K - 1) / (kWave * K);
   for (int blk = 0; blk < nblk; ++blk) {
     int lx[K], ly[K], lz[K];
@@ -160,22 +177,10 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
     }
     float qx, qy, qz, R;
     if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
-    if (a.flags & kMcFp64) {
-      e_vd += cull_sweep_f64<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, 1.0 / fr.scale, qx, qy, qz,
-                                R, lx, ly, lz, cq, ca, cb);
-      continue;
-    }
-    cull_sweep<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
-                      [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
-#pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                          e_el = fma((double)sq[k], cq[k], e_el);
-                          e_vd = fma((double)s12[k], ca[k], e_vd);
-                          e_vd = fma(-2.0 * (double)s6[k], cb[k], e_vd);
-                        }
-                      });
+    e_vd += cull_sweep_f64<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, 1.0 / fr.scale, qx, qy, qz,
+                              R, lx, ly, lz, cq, ca, cb);
   }
-  return wave_sum(e_el + e_vd);
+  return wave_sum(e_vd);
 }
 
 // E_intra of the conformer in LDS (rigid-motion invariant, so the pose is not needed);

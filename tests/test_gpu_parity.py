@@ -372,6 +372,41 @@ def test_autograd_pose_and_dof_energy(lib_sys):
                     what="autograd dof pose %d" % p)
 
 
+def test_pose_cull_matches_unculled(lib_sys):
+    """The pose-culled batch kernel (one wavefront per pose, receptor swept per 8-atom
+    sub-block) drops only pairs beyond 12 A: energies, components and gradients agree
+    with the unculled kernel, on poses inside, at and away from the receptor."""
+    ctx, ligs, ids, rec, _ = lib_sys
+    rng = np.random.default_rng(17)
+    pose_lig = np.array([ids[k % len(ids)] for k in range(300)])
+    b = ctx.batch(pose_lig)
+    xyz = np.zeros((b.natoms, 3))
+    for p, l in enumerate(pose_lig):
+        r = rng.uniform(0, 40)
+        d = rng.normal(size=3)
+        _, x = _place(ligs[l], synth.random_rotations(rng, 1)[0], r * d / np.linalg.norm(d))
+        xyz[b.atom_off[p]:b.atom_off[p + 1]] = x
+    flags = ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD
+    print("pose cull vs unculled", b.npose, flush=True)
+    out = {}
+    for pc in (1, 0):
+        ctx.tune(pose_cull=pc)
+        e, g = b.score(xyz, flags)
+        ec, _ = b.score(xyz, ff.F_INTER_SHIFTED | ff.F_COMPONENTS)
+        out[pc] = (e, g, ec)
+    ctx.tune(pose_cull=1)
+    (e1, g1, c1), (e0, g0, c0) = out[1], out[0]
+    # both kernels sum FP32 pair terms in different orders: errors scale with sum |e_pair|
+    scale = np.array([inter_abs_sum(rec, xyz[b.atom_off[p]:b.atom_off[p + 1]], ligs[l])
+                      for p, l in enumerate(pose_lig)]) + 1e-3
+    assert np.all(np.abs(e1[:, 0] - e0[:, 0]) <= 2e-6 * scale)
+    assert np.all(np.abs(c1[:, 2:] - c0[:, 2:]) <= 2e-6 * scale[:, None])
+    assert np.array_equal(e1[:, 1], e0[:, 1])
+    for p in range(b.npose):
+        ga, gb = g1[b.atom_off[p]:b.atom_off[p + 1]], g0[b.atom_off[p]:b.atom_off[p + 1]]
+        assert np.abs(ga - gb).max() <= 2e-5 * max(np.abs(gb).max(), 1e-3), p
+
+
 def test_rigid_scan_culled_equals_unculled_and_oracle(sys0):
     """The culled kernel (cell list in LDS) drops only pairs beyond 12 A."""
     import torch
