@@ -1257,6 +1257,7 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
     return r;
   }
   hipError_t e = m->d_queue.ensure(1);
+  if (e == hipSuccess) e = mc_prepare();
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
     delete m;
@@ -1282,15 +1283,29 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.max_rot0 = 15.0 * deg; a.max_trans0 = 0.15;
   a.roi_x = m->roi[0]; a.roi_y = m->roi[1]; a.roi_z = m->roi[2]; a.roi_r = m->roi_r;
   a.inter_mode = m->inter_mode; a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p;
+  a.nrec = c->nrec;
   a.cg = c->cg; a.g = c->g;
   return a;
 }
 
-static int mc_blocks(mmo_mc* m, int nwork) {
+// launch shape: up to 8 chains (wavefronts) per workgroup; in the direct form the
+// workgroup stages the receptor in LDS when it fits next to the conformers
+static hipError_t mc_launch(mmo_mc* m, const McArgs& a, int nwork, hipStream_t s) {
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  return std::max(1, std::min(nwork, ncu * m->ctx->mc_waves_per_cu));
+  const size_t cap = 160 * 1024;
+  bool rec_lds = m->inter_mode != MMO_INTERP && m->ctx->cull_ok;
+  int waves = 8;
+  while (waves > 1 && mc_lds_bytes(a, waves, rec_lds) > cap) --waves;
+  if (rec_lds && (waves < 4 || mc_lds_bytes(a, waves, rec_lds) > cap)) {
+    rec_lds = false;
+    waves = 8;
+    while (waves > 1 && mc_lds_bytes(a, waves, false) > cap) --waves;
+  }
+  const int resident = (ncu * m->ctx->mc_waves_per_cu + waves - 1) / waves;
+  const int blocks = std::max(1, std::min((nwork + waves - 1) / waves, resident));
+  return launch_mc(a, blocks, waves, rec_lds, s);
 }
 
 int mmo_mc_run(mmo_mc* m, int nsteps, void* stream) {
@@ -1302,7 +1317,7 @@ int mmo_mc_run(mmo_mc* m, int nsteps, void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   McArgs a = mc_args(m, nsteps, m->nchain, m->d_work.p);
   timer_begin(c, "mc", s);
-  HIPCHK(c, launch_mc(a, mc_blocks(m, m->nchain), s));
+  HIPCHK(c, mc_launch(m, a, m->nchain, s));
   timer_end(c, "mc", s);
   return MMO_OK;
 }
@@ -1324,7 +1339,7 @@ int mmo_mc_run_graph(mmo_mc* m, int nsteps, int nreplay, void* stream) {
     // capture queue reset + persistent launch on the context's own stream
     McArgs a = mc_args(m, nsteps, m->nchain, m->d_work.p);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_mc(a, mc_blocks(m, m->nchain), c->stream);
+    hipError_t e = mc_launch(m, a, m->nchain, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess)
@@ -1364,7 +1379,7 @@ int mmo_mc_run_per_ligand(mmo_mc* m, int nsteps, void* stream) {
   timer_begin(c, "mc", s);
   for (const auto& gr : m->groups) {
     McArgs a = mc_args(m, nsteps, gr.second, m->d_work_by_lig.p + gr.first);
-    HIPCHK(c, launch_mc(a, mc_blocks(m, gr.second), s));
+    HIPCHK(c, mc_launch(m, a, gr.second, s));
   }
   timer_end(c, "mc", s);
   return MMO_OK;

@@ -36,6 +36,7 @@ struct McState {
   int status, initialized;
   int accepted, resets, ooroi;
 };
+static_assert(sizeof(McState) % 8 == 0, "McState is copied as 32-bit words into 8-aligned LDS slots");
 
 struct McArgs {
   // ligands
@@ -70,10 +71,15 @@ struct McArgs {
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
+  int nrec;
   CellGrid cg;
   GridDev g;
 };
 
-hipError_t launch_mc(const McArgs& a, int nblocks, hipStream_t s);
+// waves: chains per workgroup (one wavefront each); rec_lds: the workgroup first copies
+// the receptor and its cell table to LDS (they must fit next to the conformers)
+hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s);
+size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds);
+hipError_t mc_prepare();
 
 }  // namespace mmo

@@ -131,13 +131,14 @@ __device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const Pose
 
 // shifted direct sum of the pose: FP32 pairs swept per 8-atom sub-block (pose_sweep);
 // the FP64 parity mode sweeps with FP64 pair arithmetic, two atoms per lane
-__device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v) {
+__device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v,
+                                  const RecPos* s_pos, const RecAB* s_ab, const int* s_cell) {
   const Frame fr = a.fr;
   int sat = 0;
   if (!(a.flags & kMcFp64)) {
     double e_el = 0.0, e_vd = 0.0;
     pose_sweep<false>(
-        a.cg, a.cg.cell_start, a.rpos, a.rab, fr, li.n_atoms,
+        a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms,
         [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
           double x, y, z;
           v.at(j, x, y, z);
@@ -177,8 +178,8 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
     }
     float qx, qy, qz, R;
     if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
-    e_vd += cull_sweep_f64<K>(a.cg, a.cg.cell_start, a.rpos, a.rab, 1.0 / fr.scale, qx, qy, qz,
-                              R, lx, ly, lz, cq, ca, cb);
+    e_vd += cull_sweep_f64<K>(a.cg, s_cell, s_pos, s_ab, 1.0 / fr.scale, qx, qy, qz, R, lx, ly,
+                              lz, cq, ca, cb);
   }
   return wave_sum(e_vd);
 }
@@ -202,31 +203,60 @@ __device__ double mc_intra(const McArgs& a, const LigInfo& li, const double* sx,
   return wave_sum(e);
 }
 
-__device__ double mc_inter(const McArgs& a, const LigInfo& li, const PoseView& v) {
-  return a.inter_mode == MMO_INTERP ? mc_inter_interp(a, li, v) : mc_inter_direct(a, li, v);
+struct RecView {
+  const RecPos* pos;
+  const RecAB* ab;
+  const int* cell;
+};
+
+__device__ double mc_inter(const McArgs& a, const LigInfo& li, const PoseView& v, const RecView& r) {
+  return a.inter_mode == MMO_INTERP ? mc_inter_interp(a, li, v)
+                                    : mc_inter_direct(a, li, v, r.pos, r.ab, r.cell);
+}
+
+// LDS writes of one wavefront made visible to its other lanes (the chain's conformer
+// buffers belong to one wavefront; the workgroup's other wavefronts run other chains)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // --------------------------------------------------------------------- the kernel
-__global__ void __launch_bounds__(64) k_mc(McArgs a) {
-  extern __shared__ __align__(16) double mc_smem[];
-  const int lane = threadIdx.x;
-  double* cx_ = mc_smem;                       // current conformer
-  double* cy_ = cx_ + a.max_atoms;
+// A workgroup of `waves` wavefronts, each running its own chains; with REC_LDS the
+// workgroup first stages the receptor and its cell table in LDS, shared by all chains.
+template <bool REC_LDS>
+__global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
+  extern __shared__ __align__(16) char mc_lds[];
+  RecView rv;
+  load_receptor<REC_LDS>(a.rpos, a.rab, a.nrec, a.cg, rv.pos, rv.ab, rv.cell, mc_lds);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  double* cx_ = reinterpret_cast<double*>(mc_lds + rec_bytes) + (size_t)wv * 6 * a.max_atoms;
+  double* cy_ = cx_ + a.max_atoms;                // current conformer
   double* cz_ = cy_ + a.max_atoms;
-  double* px_ = cz_ + a.max_atoms;             // proposed conformer
+  double* px_ = cz_ + a.max_atoms;                // proposed conformer
   double* py_ = px_ + a.max_atoms;
   double* pz_ = py_ + a.max_atoms;
-  __shared__ int s_chain;
-  const double pi = 3.14159265358979311600;    // 4 atan 1 (src/math.ml:13)
+  McState* s_st = reinterpret_cast<McState*>(
+      mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 6 * a.max_atoms * sizeof(double) + 15) &
+                            ~(size_t)15));
+  const double pi = 3.14159265358979311600;       // 4 atan 1 (src/math.ml:13)
 
   for (;;) {
-    if (lane == 0) s_chain = atomicAdd(a.queue, 1);
-    __syncthreads();
-    const int q = s_chain;
-    __syncthreads();
+    int qv = 0;
+    if (lane == 0) qv = atomicAdd(a.queue, 1);
+    const int q = __shfl(qv, 0, kWave);
     if (q >= a.nwork) break;
     const int chain = a.work[q];
-    McState st = a.st[chain];
+    // the chain's state in this wavefront's LDS slot (wave-uniform values)
+    McState& st = s_st[wv];
+    {
+      const int* src = reinterpret_cast<const int*>(a.st + chain);
+      int* dst = reinterpret_cast<int*>(&st);
+      for (int k = lane; k < (int)(sizeof(McState) / 4); k += kWave) dst[k] = src[k];
+    }
+    wave_sync();
     if (st.status != kMcRunning) continue;
     const int lig = a.chain_lig[chain];
     const LigInfo li = a.ligs[lig];
@@ -238,7 +268,7 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
     for (int j = lane; j < L; j += kWave) {
       cx_[j] = gconf[3 * j]; cy_[j] = gconf[3 * j + 1]; cz_[j] = gconf[3 * j + 2];
     }
-    __syncthreads();
+    wave_sync();
     const bool flex = (a.flags & kMcTweak) && nrb > 0;
     const bool use_intra = (a.flags & kMcIntraUff) != 0;
 
@@ -248,7 +278,7 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
       PoseView v{cx_, cy_, cz_, {}, 0.0, 0.0, 0.0, st.pos0[0], st.pos0[1], st.pos0[2]};
       for (int k = 0; k < 9; ++k) v.m[k] = st.rot0[k];
       st.prev_e_intra = use_intra ? mc_intra(a, li, cx_, cy_, cz_) : 0.0;
-      st.prev_e_inter = mc_inter(a, li, v);
+      st.prev_e_inter = mc_inter(a, li, v, rv);
       st.prev_e = st.prev_e_inter + st.prev_e_intra;
       st.best_e = st.prev_e;
       for (int k = 0; k < 9; ++k) { st.rot[k] = st.rot0[k]; st.best_rot[k] = st.rot0[k]; }
@@ -278,8 +308,6 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
       const double* sy = cy_;
       const double* sz = cz_;
       int just_rotated = -1;
-      double ndr[kMcMaxRb];
-      for (int k = 0; k < nrb; ++k) ndr[k] = dr[k];
       ccx = st.center[0]; ccy = st.center[1]; ccz = st.center[2];
       if (!rigid) {
         for (int k = 0; k < 9; ++k) nrot[k] = st.rot[k];
@@ -289,17 +317,17 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
           int i = (int)(u0 * nrb);
           i = i < nrb ? i : nrb - 1;
           const bool flip = st.conf_step > 0 && st.conf_step % kMcBlock == 0 && !(a.flags & kMcNoFlip);
-          const double amp = flip ? pi : ndr[i];
+          const double amp = flip ? pi : dr[i];
           const double ang = u1 * (2.0 * amp) - amp;   // Move.rand_angle, src/move.ml:14-16
           for (int j = lane; j < L; j += kWave) { px_[j] = cx_[j]; py_[j] = cy_[j]; pz_[j] = cz_[j]; }
-          __syncthreads();
+          wave_sync();
           const int left = a.rb_left[li.rb_base + i], right = a.rb_right[li.rb_base + i];
           const double ox = px_[right], oy = py_[right], oz = pz_[right];
           const double vx = ox - px_[left], vy = oy - py_[left], vz = oz - pz_[left];
           const double mag = sqrt(vx * vx + vy * vy + vz * vz);
           double Rm[9];
           axis_angle(vx / mag, vy / mag, vz / mag, ang, Rm);
-          __syncthreads();
+          wave_sync();
           const int g0 = a.rg_off[li.rg_base + i], g1 = a.rg_off[li.rg_base + i + 1];
           for (int m = g0 + lane; m < g1; m += kWave) {
             const int at = a.rg_atoms[m];
@@ -308,7 +336,7 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
             py_[at] = (Rm[3] * zx + Rm[4] * zy + Rm[5] * zz) + oy;
             pz_[at] = (Rm[6] * zx + Rm[7] * zy + Rm[8] * zz) + oz;
           }
-          __syncthreads();
+          wave_sync();
           // update_center (A.favg) and enclosing radius (src/mol.ml:353-356, :576-591)
           double s0 = 0.0, s1 = 0.0, s2 = 0.0;
           for (int j = lane; j < L; j += kWave) { s0 += px_[j]; s1 += py_[j]; s2 += pz_[j]; }
@@ -357,7 +385,7 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
       PoseView v{sx, sy, sz, {}, ccx, ccy, ccz, npos[0], npos[1], npos[2]};
       for (int k = 0; k < 9; ++k) v.m[k] = nrot[k];
       if (!rigid && use_intra && flex) st.prev_e_intra = mc_intra(a, li, sx, sy, sz);
-      st.prev_e_inter = mc_inter(a, li, v);
+      st.prev_e_inter = mc_inter(a, li, v, rv);
       const double curr_e = st.prev_e_inter + st.prev_e_intra;
       const double dxr = a.roi_x - npos[0], dyr = a.roi_y - npos[1], dzr = a.roi_z - npos[2];
       const double dist_roi = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
@@ -392,7 +420,7 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
           if (sx != cx_) {   // conf := conf'
             for (int j = lane; j < L; j += kWave) { cx_[j] = px_[j]; cy_[j] = py_[j]; cz_[j] = pz_[j]; }
             st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
-            __syncthreads();
+            wave_sync();
             sx = cx_; sy = cy_; sz = cz_;
             v.sx = cx_; v.sy = cy_; v.sz = cz_;
           }
@@ -415,14 +443,12 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
         }
         // adaptive torsion steps on the proposal's copy (src/lds.ml:976-979, :603-621):
         // kept only when the proposal was accepted (conf := conf')
-        if (flex && !rigid && st.conf_step > 0 && st.conf_step % (kMcBlock * nrb) == 0) {
+        if (accept && flex && !rigid && st.conf_step > 0 && st.conf_step % (kMcBlock * nrb) == 0) {
           for (int k = 0; k < nrb; ++k) {
             const double ar = sw_ratio(rsw[k]);
-            if (ar <= 0.45) ndr[k] = 0.95 * ndr[k];
-            else if (ar >= 0.55) ndr[k] = fmin(pi, 1.05 * ndr[k]);
+            if (ar <= 0.45) dr[k] = 0.95 * dr[k];
+            else if (ar >= 0.55) dr[k] = fmin(pi, 1.05 * dr[k]);
           }
-          if (accept)
-            for (int k = 0; k < nrb; ++k) dr[k] = ndr[k];
         }
       }
       if (rigid) ++st.rigid_step;
@@ -433,8 +459,13 @@ __global__ void __launch_bounds__(64) k_mc(McArgs a) {
     for (int j = lane; j < L; j += kWave) {
       gconf[3 * j] = cx_[j]; gconf[3 * j + 1] = cy_[j]; gconf[3 * j + 2] = cz_[j];
     }
-    if (lane == 0) a.st[chain] = st;
-    __syncthreads();
+    wave_sync();
+    {
+      const int* src = reinterpret_cast<const int*>(&st);
+      int* dst = reinterpret_cast<int*>(a.st + chain);
+      for (int k = lane; k < (int)(sizeof(McState) / 4); k += kWave) dst[k] = src[k];
+    }
+    wave_sync();
   }
 }
 
@@ -442,12 +473,36 @@ __global__ void k_mc_reset_queue(int* q) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *q = 0;
 }
 
-hipError_t launch_mc(const McArgs& a, int nblocks, hipStream_t s) {
+size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
+  size_t rec = 0;
+  if (rec_lds)
+    rec = ((size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+           (size_t)(a.cg.nx * a.cg.ny * a.cg.nz + 1) * sizeof(int) + 15) & ~(size_t)15;
+  const size_t conf = ((size_t)waves * 6 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
+  return rec + conf + (size_t)waves * sizeof(McState);
+}
+
+hipError_t mc_prepare() {
+  // dynamic LDS above 64 KiB must be allowed per kernel (gfx950: 160 KiB per CU); set
+  // once, outside any stream capture
+  const int cap = 160 * 1024;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<false>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+}
+
+hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s) {
   hipLaunchKernelGGL(k_mc_reset_queue, dim3(1), dim3(1), 0, s, a.queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
-  hipLaunchKernelGGL(k_mc, dim3(nblocks), dim3(kWave), lds, s, a);
+  const size_t lds = mc_lds_bytes(a, waves, rec_lds);
+  const int rec_bytes = (int)(mc_lds_bytes(a, waves, rec_lds) - mc_lds_bytes(a, waves, false));
+  if (rec_lds)
+    hipLaunchKernelGGL(k_mc<true>, dim3(nblocks), dim3(waves * kWave), lds, s, a, rec_bytes);
+  else
+    hipLaunchKernelGGL(k_mc<false>, dim3(nblocks), dim3(waves * kWave), lds, s, a, 0);
   return hipGetLastError();
 }
 
