@@ -186,6 +186,40 @@ int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double
 int mmo_scan_dev(mmo_ctx* ctx, int lig_id, int mode, const double* rots, int nrot,
                  const double* trans, int ntrans, double* e_out, void* stream);
 
+/* ------------------------------------------- rigid-scan filters and top-k */
+/* Protein vdW volume bitmask (Lds.vdW_volume / atom_bitmask_set, src/lds.ml:164-197):
+ * voxel (i, j, k) of the grid origin + (i, j, k) * step is set when it lies strictly
+ * inside the vdW sphere of a receptor atom.  radii (nullable) per receptor atom; NULL:
+ * Ptable.vdW_radii by element (src/ptable.ml:43-53).  origin NULL = (0, 0, 0), the
+ * reference's grid corner.  Needs the receptor; replaced by the next receptor. */
+int mmo_set_vdw_mask(mmo_ctx* ctx, const double* radii, double step, int nx, int ny, int nz,
+                     const double* origin);
+/* Copies the mask's 32-bit words (bit v of word v/32 = voxel v, v = i + j*nx + k*nx*ny)
+ * to words (nullable) and returns the number of words. */
+int mmo_get_vdw_mask(const mmo_ctx* ctx, uint32_t* words);
+enum {
+  MMO_SCAN_ROI = 1,           /* skip translations outside the ROI sphere (ROI.is_inside) */
+  MMO_SCAN_CENTER_CLASH = 2,  /* ligands whose centre lies inside one of their atoms: skip
+                                 translations whose 8 voxel corners are all protein
+                                 volume (G3D.vdW_clash_AND, src/G3D.ml:188-213)        */
+  MMO_SCAN_ATOM_CLASH = 4     /* skip poses with an atom that has a protein-volume corner
+                                 (Mol.protein_ligand_clash, src/mol.ml:1195-1206)     */
+};
+/* The clash filters of Lds.exhaustive_rigid_ligand_docking (src/lds.ml:1040-1114) on the
+ * energies of a finished mmo_scan_dev over the same rotations and translations: the
+ * skipped poses' energies become +inf (they are never best and never in the top-k).
+ * lig_radii: the ligand's atom radii (for MMO_SCAN_CENTER_CLASH).  Corners outside the
+ * mask's grid count as free.  e_inout: device (nrot * ntrans).  Returns after the
+ * filter is enqueued and its host inputs are consumed. */
+int mmo_scan_filter_dev(mmo_ctx* ctx, int lig_id, const double* rots, int nrot, const double* trans,
+                        int ntrans, const double* roi_center, double roi_r,
+                        const double* lig_radii, int flags, double* e_inout, void* stream);
+/* The k lowest energies of e (device, n) with their indices, ascending, ties by index
+ * (the TopKeeper of src/lds.ml:1055-1064, :1110-1113); NaN sorts last; k <= 2048
+ * (k > n gives n).  e_out, idx_out: device.  Enqueued on stream. */
+int mmo_topk_dev(mmo_ctx* ctx, const double* e, int64_t n, int k, double* e_out, int64_t* idx_out,
+                 void* stream);
+
 /* -------------------------------------------------------------- Monte Carlo */
 /* Batched MC chains (Lds.simulate_lig, src/lds.ml:741-1000) on the GPU: chain k
  * docks ligand chain_lig[k] from the start pose (rot0[9k..], pos0[3k..]) with rigid

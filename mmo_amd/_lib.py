@@ -16,6 +16,7 @@ ERR_NO_RECEPTOR, ERR_NO_GRID, ERR_FORK, ERR_RANGE = -5, -6, -7, -8
 GLOBAL, SHIFTED, INTERP = 0, 1, 2
 F_INTER_SHIFTED, F_INTER_GLOBAL, F_INTER_INTERP = 1, 2, 4
 F_INTRA, F_GRAD, F_COMPONENTS = 8, 16, 32
+SCAN_ROI, SCAN_CENTER_CLASH, SCAN_ATOM_CLASH = 1, 2, 4
 MC_TWEAK_RBONDS, MC_HARD_ROI, MC_INTRA_UFF, MC_METROPOLIS_ALWAYS, MC_NO_FLIP, MC_FP64 = 1, 2, 4, 8, 16, 32
 
 _d, _i, _p = ct.c_double, ct.c_int, ct.c_void_p
@@ -49,6 +50,10 @@ SIGNATURES = {
     "mmo_batch_score": (_i, [_p, _i, _p, _p, _p]),
     "mmo_batch_score_dofs_dev": (_i, [_p, _i, _p, _p, _p, _p, _p, _p]),
     "mmo_scan_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _p]),
+    "mmo_set_vdw_mask": (_i, [_p, _p, _d, _i, _i, _i, _p]),
+    "mmo_get_vdw_mask": (_i, [_p, _p]),
+    "mmo_scan_filter_dev": (_i, [_p, _i, _p, _i, _p, _i, _p, _d, _p, _i, _p, _p]),
+    "mmo_topk_dev": (_i, [_p, _p, ct.c_int64, _i, _p, _p, _p]),
     "mmo_mc_create": (_i, [_p, _p, _i, _p, _p, _i, _i, _p, _d, _d, ct.c_uint64, _i, _pp]),
     "mmo_mc_run": (_i, [_p, _i, _p]),
     "mmo_mc_run_graph": (_i, [_p, _i, _i, _p]),

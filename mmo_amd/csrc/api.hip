@@ -115,6 +115,18 @@ struct mmo_ctx {
   DevBuf<Tile> s_tile;
   DevBuf<double> s_scan;   // rotations + translations of the last scan
   DevBuf<int> s_tperm;
+  // receptor on the host (rows x, y, z; atomic numbers) for host-side set-ups
+  std::vector<double> rec_host;
+  std::vector<int> rec_anum;
+  // protein vdW volume bitmask (Lds.vdW_volume) for the rigid scan's clash filters
+  bool has_mask = false;
+  double mask_step = 0.0, mask_o[3] = {0.0, 0.0, 0.0};
+  int mask_n[3] = {0, 0, 0};
+  std::vector<uint32_t> mask_host;
+  DevBuf<uint32_t> d_mask;
+  DevBuf<uint8_t> s_skip;
+  DevBuf<uint64_t> s_tk_st;
+  DevBuf<unsigned> s_tk_u;
   // tuning
   int inter_k = 4, inter_waves = 4, inter_rtile = 2048;
   std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -530,6 +542,12 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   c->fp_bits = k;
   c->nrec = n;
   c->has_rec = true;
+  c->rec_host.resize(3 * (size_t)n);
+  c->rec_anum.assign(anum, anum + n);
+  for (int i = 0; i < n; ++i) {
+    c->rec_host[3 * i] = xs[i]; c->rec_host[3 * i + 1] = ys[i]; c->rec_host[3 * i + 2] = zs[i];
+  }
+  c->has_mask = false;   // a mask belongs to the receptor it was computed from
   return MMO_OK;
 }
 
@@ -1436,6 +1454,176 @@ void mmo_mc_destroy(mmo_mc* m) {
     m->d_conf.release(); m->d_best.release(); m->d_dr.release(); m->d_sw.release();
   }
   delete m;
+}
+
+// ------------------------------------------------- rigid-scan filters and top-k
+namespace {
+// Ptable.vdW_radii (src/ptable.ml:43-53)
+double bondi_radius(int anum) {
+  switch (anum) {
+    case 1: return 1.2;
+    case 6: return 1.7;
+    case 7: return 1.6;
+    case 8: return 1.55;
+    case 9: return 1.5;
+    case 12: return 2.2;
+    case 15: return 1.95;
+    case 16: return 1.8;
+    case 17: return 1.8;
+    case 35: return 1.9;
+    case 53: return 2.1;
+    default: return std::nan("");
+  }
+}
+
+bool mask_get(const mmo_ctx* c, long i, long j, long k) {
+  if (i < 0 || j < 0 || k < 0 || i >= c->mask_n[0] || j >= c->mask_n[1] || k >= c->mask_n[2])
+    return false;
+  const size_t v = (size_t)i + (size_t)j * c->mask_n[0] + (size_t)k * c->mask_n[0] * c->mask_n[1];
+  return (c->mask_host[v >> 5] >> (v & 31)) & 1u;
+}
+}  // namespace
+
+int mmo_set_vdw_mask(mmo_ctx* c, const double* radii, double step, int nx, int ny, int nz,
+                     const double* origin) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (!(step > 0.0) || nx < 2 || ny < 2 || nz < 2 || (double)nx * ny * nz > 2147483647.0)
+    return fail(c, MMO_ERR_BAD_ARG, "bad mask grid (step %g, %d x %d x %d)", step, nx, ny, nz);
+  const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+  const size_t nvox = (size_t)nx * ny * nz;
+  std::vector<uint32_t> mask((nvox + 31) / 32, 0u);
+  // atom_bitmask_set grid mask xyz r_vdW true (src/lds.ml:164-184) for every atom
+  for (int a = 0; a < c->nrec; ++a) {
+    const double rad = radii ? radii[a] : bondi_radius(c->rec_anum[a]);
+    if (!(rad > 0.0) || !std::isfinite(rad))
+      return fail(c, MMO_ERR_BAD_ARG, "receptor atom %d: no vdW radius (element %d)", a,
+                  c->rec_anum[a]);
+    const double* p = &c->rec_host[3 * (size_t)a];
+    // Grid.coord_of_point (src/grid.ml:87-90): truncation of (x - x0) / step
+    const long ci = (long)((p[0] - o[0]) / step), cj = (long)((p[1] - o[1]) / step),
+               ck = (long)((p[2] - o[2]) / step);
+    const long rs = (long)std::ceil(rad / step);
+    const double r2 = rad * rad;
+    for (long i = ci - rs; i <= ci + rs; ++i) {
+      if (i < 0 || i >= nx) continue;
+      const double dx = p[0] - (o[0] + i * step);
+      for (long j = cj - rs; j <= cj + rs; ++j) {
+        if (j < 0 || j >= ny) continue;
+        const double dy = p[1] - (o[1] + j * step);
+        for (long k = ck - rs; k <= ck + rs; ++k) {
+          if (k < 0 || k >= nz) continue;
+          const double dz = p[2] - (o[2] + k * step);
+          if (dx * dx + dy * dy + dz * dz < r2) {
+            const size_t v = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
+            mask[v >> 5] |= 1u << (v & 31);
+          }
+        }
+      }
+    }
+  }
+  if ((r = upload(c, c->d_mask, mask.data(), mask.size(), c->stream))) return r;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mask_host.swap(mask);
+  c->mask_step = step;
+  c->mask_o[0] = o[0]; c->mask_o[1] = o[1]; c->mask_o[2] = o[2];
+  c->mask_n[0] = nx; c->mask_n[1] = ny; c->mask_n[2] = nz;
+  c->has_mask = true;
+  return MMO_OK;
+}
+
+int mmo_get_vdw_mask(const mmo_ctx* c, uint32_t* words) {
+  if (!c || !c->has_mask) return MMO_ERR_BAD_ARG;
+  if (words) std::memcpy(words, c->mask_host.data(), c->mask_host.size() * sizeof(uint32_t));
+  return (int)c->mask_host.size();
+}
+
+int mmo_scan_filter_dev(mmo_ctx* c, int lig_id, const double* rots, int nrot, const double* trans,
+                        int ntrans, const double* roi_center, double roi_r,
+                        const double* lig_radii, int flags, double* e_inout, void* stream) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (lig_id < 0 || lig_id >= (int)c->ligs.size())
+    return fail(c, MMO_ERR_BAD_ARG, "unknown ligand %d", lig_id);
+  if (!rots || !trans || nrot <= 0 || ntrans <= 0 || !e_inout)
+    return fail(c, MMO_ERR_BAD_ARG, "bad scan filter arguments");
+  if ((long long)nrot * ntrans > 2147483647LL)
+    return fail(c, MMO_ERR_BAD_ARG, "too many poses (%d x %d)", nrot, ntrans);
+  if ((flags & MMO_SCAN_ROI) && (!roi_center || !(roi_r > 0.0)))
+    return fail(c, MMO_ERR_BAD_ARG, "ROI filter without an ROI");
+  if ((flags & (MMO_SCAN_CENTER_CLASH | MMO_SCAN_ATOM_CLASH)) && !c->has_mask)
+    return fail(c, MMO_ERR_BAD_ARG, "clash filters need mmo_set_vdw_mask first");
+  if ((flags & MMO_SCAN_CENTER_CLASH) && !lig_radii)
+    return fail(c, MMO_ERR_BAD_ARG, "the centre-clash filter needs the ligand's radii");
+  if ((r = sync_ligands(c))) return r;
+  const LigHost& h = c->ligs[lig_id];
+  // Mol.is_ligand_center_vdW_occuppied (src/mol.ml:1208-1218)
+  bool centre_occupied = false;
+  if (flags & MMO_SCAN_CENTER_CLASH) {
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int i = 0; i < h.n; ++i) { cx += h.xyz[3 * i]; cy += h.xyz[3 * i + 1]; cz += h.xyz[3 * i + 2]; }
+    cx /= h.n; cy /= h.n; cz /= h.n;
+    for (int i = 0; i < h.n && !centre_occupied; ++i) {
+      const double dx = cx - h.xyz[3 * i], dy = cy - h.xyz[3 * i + 1], dz = cz - h.xyz[3 * i + 2];
+      centre_occupied = std::sqrt(dx * dx + dy * dy + dz * dz) < lig_radii[i];
+    }
+  }
+  std::vector<uint8_t> skip(ntrans, 0);
+  for (int t = 0; t < ntrans; ++t) {
+    const double* p = trans + 3 * (size_t)t;
+    if (flags & MMO_SCAN_ROI) {   // ROI.is_inside: dist2 < out_r^2 (src/ROI.ml:65-66)
+      const double dx = roi_center[0] - p[0], dy = roi_center[1] - p[1], dz = roi_center[2] - p[2];
+      if (!(dx * dx + dy * dy + dz * dz < roi_r * roi_r)) { skip[t] = 1; continue; }
+    }
+    if (centre_occupied) {   // G3D.vdW_clash_AND: all 8 corners in the protein volume
+      const double inv = 1.0 / c->mask_step;
+      const long i0 = (long)((p[0] - c->mask_o[0]) * inv), j0 = (long)((p[1] - c->mask_o[1]) * inv),
+                 k0 = (long)((p[2] - c->mask_o[2]) * inv);
+      bool all = true;
+      for (int q = 0; q < 8 && all; ++q)
+        all = mask_get(c, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2));
+      if (all) skip[t] = 1;
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(c, c->s_skip.ensure(ntrans));
+  HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
+  HIPCHK(c, hipMemcpyAsync(c->s_skip.p, skip.data(), ntrans, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p + 9 * (size_t)nrot, trans, 3 * (size_t)ntrans * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  ScanFilterArgs a{};
+  a.rots = c->s_scan.p; a.trans = c->s_scan.p + 9 * (size_t)nrot; a.nrot = nrot; a.ntrans = ntrans;
+  size_t base = 0;
+  for (int l = 0; l < lig_id; ++l) base += (size_t)c->ligs[l].n;   // LigInfo.atom_base
+  a.lig = c->d_lig_xyz.p + 3 * base; a.n_atoms = h.n;
+  a.mask = c->has_mask ? c->d_mask.p : nullptr;
+  a.ox = c->mask_o[0]; a.oy = c->mask_o[1]; a.oz = c->mask_o[2];
+  a.inv_step = c->has_mask ? 1.0 / c->mask_step : 0.0;
+  a.nx = c->mask_n[0]; a.ny = c->mask_n[1]; a.nz = c->mask_n[2];
+  a.skip = c->s_skip.p; a.atom_clash = (flags & MMO_SCAN_ATOM_CLASH) ? 1 : 0; a.e = e_inout;
+  HIPCHK(c, launch_scan_filter(a, s));
+  // the host arrays are read by the copies above: wait before returning
+  HIPCHK(c, hipStreamSynchronize(s));
+  return MMO_OK;
+}
+
+int mmo_topk_dev(mmo_ctx* c, const double* e, int64_t n, int k, double* e_out, int64_t* idx_out,
+                 void* stream) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!e || n <= 0 || k <= 0 || k > kTopkMax || !e_out || !idx_out)
+    return fail(c, MMO_ERR_BAD_ARG, "bad top-k arguments (n %lld, k %d, k <= %d)", (long long)n,
+                k, kTopkMax);
+  const int kk = (int)std::min<int64_t>(k, n);
+  const size_t chunk = 4096;
+  const size_t nb = ((size_t)n + chunk - 1) / chunk;
+  HIPCHK(c, c->s_tk_st.ensure(3));
+  HIPCHK(c, c->s_tk_u.ensure(256 + 1 + nb));
+  TopkScratch w{c->s_tk_st.p, c->s_tk_u.p, c->s_tk_u.p + 256, c->s_tk_u.p + 257, chunk};
+  HIPCHK(c, launch_topk(e, (size_t)n, kk, e_out, idx_out, w, (hipStream_t)stream));
+  return MMO_OK;
 }
 
 // ------------------------------------------------------------- L-BFGS minimiser

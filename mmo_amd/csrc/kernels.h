@@ -127,6 +127,33 @@ hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool gr
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
 
+// -------------------------------------------------- rigid-scan filters, top-k
+struct ScanFilterArgs {
+  const double* rots;         // nrot x 9
+  const double* trans;        // ntrans x 3
+  int nrot, ntrans;
+  const double* lig;          // centred conformer rows (x, y, z)
+  int n_atoms;
+  const uint32_t* mask;       // protein vdW volume bitmask, voxel i + j*nx + k*nx*ny
+  double ox, oy, oz, inv_step;
+  int nx, ny, nz;
+  const uint8_t* skip;        // per translation: 1 = skipped (ROI / centre clash)
+  int atom_clash;             // test every atom's 8 corners
+  double* e;                  // scan energies, set to +inf for skipped poses
+};
+hipError_t launch_scan_filter(const ScanFilterArgs& a, hipStream_t s);
+
+constexpr int kTopkMax = 2048;
+struct TopkScratch {
+  uint64_t* st;               // 3
+  unsigned* hist;             // 256
+  unsigned* cnt;              // 1
+  unsigned* bcnt;             // one per chunk
+  size_t chunk;
+};
+hipError_t launch_topk(const double* e, size_t n, int k, double* out_e, int64_t* out_i,
+                       TopkScratch w, hipStream_t s);
+
 // ------------------------------------------------------------ L-BFGS minimiser
 constexpr int kMinMaxHist = 16;
 enum { kMinRunning = 0, kMinConverged = 1, kMinFailed = 2 };

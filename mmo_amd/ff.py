@@ -229,6 +229,52 @@ class Context:
                                      ct.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
 
+    def set_vdw_mask(self, step, dims, origin=None, radii=None):
+        """Protein vdW volume bitmask on the grid origin + (i, j, k) * step (Lds.vdW_volume);
+        radii default to the reference's per-element vdW radii."""
+        r = _f64(radii) if radii is not None else None
+        o = _f64(origin) if origin is not None else None
+        nx, ny, nz = (int(v) for v in dims)
+        self._check(lib.mmo_set_vdw_mask(self.h, _ptr(r), float(step), nx, ny, nz, _ptr(o)))
+
+    def vdw_mask(self):
+        """The mask's 32-bit words (bit v of word v // 32 is voxel v = i + j*nx + k*nx*ny)."""
+        n = lib.mmo_get_vdw_mask(self.h, None)
+        if n < 0:
+            raise MMOError(n, "no vdW mask")
+        w = np.zeros(n, dtype=np.uint32)
+        lib.mmo_get_vdw_mask(self.h, _ptr(w))
+        return w
+
+    def scan_filter(self, lig_id, rots, trans, e, roi=None, lig_radii=None, flags=None,
+                    stream=None):
+        """Clash filters of the exhaustive scan on the energies `e` of ctx.scan(lig_id, ...,
+        rots, trans): skipped poses become +inf.  roi = (center, radius)."""
+        r = _f64(np.asarray(rots).reshape(-1, 9))
+        t = _f64(np.asarray(trans).reshape(-1, 3))
+        if flags is None:
+            flags = (_lib.SCAN_ROI if roi is not None else 0) | _lib.SCAN_ATOM_CLASH | \
+                (_lib.SCAN_CENTER_CLASH if lig_radii is not None else 0)
+        c = _f64(roi[0]) if roi is not None else None
+        rad = _f64(lig_radii) if lig_radii is not None else None
+        self._check(lib.mmo_scan_filter_dev(self.h, int(lig_id), _ptr(r), len(r), _ptr(t), len(t),
+                                            _ptr(c), float(roi[1]) if roi is not None else 0.0,
+                                            _ptr(rad), int(flags), ct.c_void_p(e.data_ptr()),
+                                            _stream_ptr(stream)))
+        return e
+
+    def topk(self, e, k, stream=None):
+        """The k lowest values of the CUDA FP64 tensor e and their indices (ascending)."""
+        import torch
+        n = e.numel()
+        kk = min(int(k), n)
+        vals = torch.empty(kk, dtype=torch.float64, device=e.device)
+        idx = torch.empty(kk, dtype=torch.int64, device=e.device)
+        self._check(lib.mmo_topk_dev(self.h, ct.c_void_p(e.data_ptr()), ct.c_int64(n), int(k),
+                                     ct.c_void_p(vals.data_ptr()), ct.c_void_p(idx.data_ptr()),
+                                     _stream_ptr(stream)))
+        return vals, idx
+
     def tune(self, **kv):
         for k, v in kv.items():
             self._check(lib.mmo_tune_set(self.h, k.encode(), float(v)))

@@ -778,3 +778,68 @@ def test_minimizer_flexible_ligands(lib_sys):
         sums.append(inter_abs_sum(rec, xyz, m) + intra_abs_sum(m, xyz))
         assert rad <= 12.0
     assert_energy(e, refs, sums, what="minimised poses")
+
+
+# ------------------------------------------------ exhaustive-scan filters, top-k
+def _bondi(anum):
+    from oracle.scan_ref import VDW_RADII
+    return np.array([VDW_RADII[int(a)] for a in anum])
+
+
+def test_vdw_mask_and_scan_filters(sys0):
+    """Protein vdW bitmask bit-exact against the CPU restatement; the scan's ROI,
+    centre-clash and atom-clash filters skip exactly the poses the restatement skips."""
+    import torch
+    from oracle import scan_ref
+    ctx, lig, rec = sys0
+    rxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    step, dims = 0.5, (48, 44, 40)
+    origin = rxyz.min(axis=0) - 1.0
+    print("vdw mask", dims, flush=True)
+    ctx.set_vdw_mask(step, dims, origin)
+    ref = scan_ref.vdw_mask(rxyz, _bondi(rec["anum"]), step, dims, origin)
+    assert np.array_equal(ctx.vdw_mask(), scan_ref.mask_words(ref))
+    # a ligand whose centre sits inside one of its atoms (a 3-atom line), and the
+    # 75-atom ligand; a 16 x 8000 scan around the cavity, ROI radius 4 A
+    line = {"xs": np.array([-1.5, 0.0, 1.5]), "ys": np.zeros(3), "zs": np.zeros(3),
+            "q": np.zeros(3), "anum": np.array([6, 6, 6], dtype=np.int32),
+            "dists": np.array([[0, 1, 2], [1, 0, 1], [2, 1, 0]], dtype=np.int32)}
+    rots = synth.so3_rotations(4608)[::288]
+    trans = synth.lattice((0.0, 0.0, 0.0), 20, 0.5)
+    roi = (np.array([0.3, -0.2, 0.1]), 4.0)
+    rng = np.random.default_rng(3)
+    for L in (line, lig):
+        lid = ctx.add_ligand(L)
+        radii = _bondi(L["anum"])
+        e = ctx.scan(lid, ff.SHIFTED, rots, trans)
+        print("scan filter atoms", len(L["xs"]), flush=True)
+        ctx.scan_filter(lid, rots, trans, e, roi=roi, lig_radii=radii)
+        got = np.isinf(e.cpu().numpy())
+        c = np.stack([L["xs"], L["ys"], L["zs"]], 1)
+        sample = rng.choice(len(got), 600, replace=False)
+        for s in sample:
+            r, t = divmod(int(s), len(trans))
+            want = scan_ref.pose_skipped(ref, step, origin, c, radii, rots[r], trans[t], roi=roi)
+            assert got[s] == want, (len(L["xs"]), r, t)
+        assert 0 < got.sum() < len(got)
+    torch.cuda.synchronize()
+
+
+def test_topk_matches_numpy(sys0):
+    import torch
+    from oracle import scan_ref
+    ctx, _, _ = sys0
+    rng = np.random.default_rng(11)
+    for n, k in ((1_000_003, 100), (1_000_003, 2048), (500, 2048), (70_000, 1)):
+        e = rng.normal(0, 10, n)
+        e[rng.choice(n, n // 10, replace=False)] = np.round(e[: n // 10], 1)   # ties
+        e[rng.choice(n, n // 50, replace=False)] = np.inf
+        e[rng.choice(n, 3, replace=False)] = np.nan
+        print("topk", n, k, flush=True)
+        v, i = ctx.topk(torch.tensor(e, device="cuda:0"), k)
+        rv, ri = scan_ref.topk(e, k)
+        assert np.array_equal(i.cpu().numpy(), ri)
+        assert np.array_equal(v.cpu().numpy(), rv, equal_nan=True)
+    # all equal: the lowest indices win
+    v, i = ctx.topk(torch.full((10_000,), 2.5, dtype=torch.float64, device="cuda:0"), 7)
+    assert i.cpu().numpy().tolist() == list(range(7))
