@@ -89,7 +89,7 @@ class Mol:
     @property
     def types_id(self):
         if self.t_a is None:
-            raise KeyError("t_a")
+            raise AttributeError("t_a: grid types not assigned")
         return self.t_a
 
     def num_atoms(self):
@@ -177,7 +177,12 @@ def _read_one(lines, pos, ligand):
 
 def ligands_of_pqrs_file(fn):
     """Mol.ligands_of_pqrs_file, src/mol.ml:420-440 (duplicate names are an error)."""
-    lines = [l for l in open(fn).read().splitlines() if l.strip()]
+    with open(fn) as f:
+        return ligands_of_pqrs_text(f.read(), fn)
+
+
+def ligands_of_pqrs_text(text, fn="<text>"):
+    lines = [l for l in text.splitlines() if l.strip()]
     out, pos, seen = [], 0, set()
     while pos < len(lines):
         m, pos = _read_one(lines, pos, True)

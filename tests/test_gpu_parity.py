@@ -843,3 +843,49 @@ def test_topk_matches_numpy(sys0):
     # all equal: the lowest indices win
     v, i = ctx.topk(torch.full((10_000,), 2.5, dtype=torch.float64, device="cuda:0"), 7)
     assert i.cpu().numpy().tolist() == list(range(7))
+
+
+# ------------------------------------------------------------ .mol2 input path
+def test_mol2_ligands_score_like_the_oracle(sys0, tmp_path):
+    """Ligands read from .mol2 (topology, rotatable bonds and groups computed here)
+    flow into the DOF kernels: coordinates and energies agree with the oracle."""
+    import torch
+    from mmo_amd import mol2
+    from tests.test_mol2 import BUTANE, RING
+    ctx, _, rec = sys0
+    fn = str(tmp_path / "ligs.mol2")
+    with open(fn, "w") as f:
+        f.write(BUTANE + RING)
+    ligs = []
+    for m in mol2.ligands_of_mol2_file(fn):
+        m = m.with_coords(m.coords() - m.coords().mean(axis=0))   # Lds.preprocess_ligand
+        n = m.num_atoms()
+        d = {"xs": m.xs, "ys": m.ys, "zs": m.zs, "q": m.q_a, "anum": m.elt_a,
+             "dists": np.asarray(m.dists_a).reshape(n, n).T, "rb_left": m.rb_left,
+             "rb_right": m.rb_right, "rg_off": m.rg_off, "rg_atoms": m.rg_atoms}
+        ligs.append((ctx.add_ligand(m), d))
+    rng = np.random.default_rng(2)
+    pose_lig = [l for l, _ in ligs for _ in range(3)]
+    b = ctx.batch(pose_lig)
+    dofs = []
+    for lid in pose_lig:
+        d = dict(ligs)[lid]
+        dofs.append(np.concatenate([[26.0, 0, 0] + rng.uniform(-2, 2, 3),
+                                    rng.uniform(-np.pi, np.pi, 3 + len(d["rb_left"]))]))
+    dt = torch.tensor(np.concatenate(dofs), device="cuda:0")
+    x = torch.zeros((b.natoms, 3), dtype=torch.float64, device="cuda:0")
+    e = torch.zeros((b.npose, 4), dtype=torch.float64, device="cuda:0")
+    print("mol2 ligands dof batch", b.npose, flush=True)
+    b.score_dofs_dev(dt, e, xyz_out=x, flags=ff.F_INTER_SHIFTED | ff.F_INTRA)
+    torch.cuda.synchronize()
+    x, e = x.cpu().numpy(), e.cpu().numpy()
+    refs, sums = [], []
+    for p, lid in enumerate(pose_lig):
+        d = dict(ligs)[lid]
+        c = np.stack([d["xs"], d["ys"], d["zs"]], 1)
+        xyz, _ = oracle.apply_config(c, d["rb_left"], d["rb_right"], d["rg_off"], d["rg_atoms"], dofs[p])
+        assert np.allclose(x[b.atom_off[p]:b.atom_off[p + 1]], xyz, atol=1e-10)
+        m = dict(d, xs=xyz[:, 0], ys=xyz[:, 1], zs=xyz[:, 2])
+        refs.append(oracle.ene_inter(rec, m) + oracle.ene_intra(m))
+        sums.append(inter_abs_sum(rec, xyz, m) + intra_abs_sum(m, xyz))
+    assert_energy(e[:, 0] + e[:, 1], refs, sums, what="mol2 ligands")
