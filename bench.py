@@ -14,7 +14,10 @@ Secondary figures (the "extra" object), each on its own workload:
                ligands: one batched launch sequence vs one per ligand;
   min          200 batched L-BFGS iterations (energy + gradient + update) on those poses;
   mc           Monte Carlo chains, 64 per ligand for 1,024 library ligands, 1,000 frames,
-               batched persistent launch replayed from a hipGraph vs one launch per ligand.
+               batched persistent launch replayed from a hipGraph vs one launch per ligand,
+               direct sum and grid interpolation;
+  scan_select  the scan's clash filters (ROI, centre and atom clashes against the protein
+               vdW bitmask) and the top-100 of the filtered energies.
 
 Prints one JSON line with "roofline" (the scan kernel's fraction of the FP32 peak) and
 "cpu_baseline" (the FP64 C restatement timed on this host's cores).
@@ -137,6 +140,39 @@ def bench_interp(ctx, lig, rots, trans, steps, warmup):
             "poses_inside_grid": inside, "poses_out_of_grid": npose - inside,
             "gather_GBs": 32.0 * lookups / (avg * 1e-3) / 1e9,
             "note": "poses with an atom outside the grid return NaN and are counted"}
+
+
+def bench_select(ctx, lig, rec, lid, rots, trans, out, k=100):
+    """The exhaustive scan's clash filters (ROI sphere, centre and atom clashes against
+    the protein vdW bitmask) and the top-k of the filtered energies, on the energies of
+    the direct scan still in `out`."""
+    import torch
+    from mmo_amd import mol2
+    rxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    step = 0.375
+    origin = rxyz.min(axis=0) - 2.0
+    dims = tuple(int(v) for v in np.ceil((rxyz.max(axis=0) + 2.0 - origin) / step) + 1)
+    t0 = time.perf_counter()
+    ctx.set_vdw_mask(step, dims, origin)
+    mask_s = time.perf_counter() - t0
+    radii = [mol2.VDW_RADII[int(a)] for a in lig["anum"]]
+    roi = (np.zeros(3), 4.75 + 1e-9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.scan_filter(lid, rots, trans, out, roi=roi, lig_radii=radii)
+    torch.cuda.synchronize()
+    filt_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vals, idx = ctx.topk(out, k)
+    torch.cuda.synchronize()
+    topk_s = time.perf_counter() - t0
+    kept = int(torch.isfinite(out).sum().item())
+    best = int(idx[0].item())
+    return {"poses": out.numel(), "kept_after_filters": kept, "filter_s": filt_s,
+            "topk_s": topk_s, "k": k, "best_e": float(vals[0].item()),
+            "best_rotation": best // len(trans), "best_translation": best % len(trans),
+            "mask": "%d x %d x %d at %.3f A, built in %.2f s on the host" % (dims + (step, mask_s)),
+            "filters": "ROI sphere r = 4.75 A, centre clash (AND), atom clash (OR)"}
 
 
 def library_system(n_lig):
@@ -274,7 +310,25 @@ def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample, autotune):
     torch.cuda.synchronize()
     tp = time.perf_counter() - t0
     steps_p = float(m2.state()["frame"].sum())
+    # the reference's default energy: trilinear interpolation in per-type grids
+    probe_a, probe_q, types = synth.grid_types(ligs, 10)
+    origin, dims, gstep = synth.grid_spec((0.0, 0.0, 0.0), 100, 0.375)
+    ctx.compute_grids(probe_a, probe_q, gstep, dims, origin, copy_out=False)
+    tids = [ctx.add_ligand(l, types=t) for l, t in zip(ligs, types)]
+    rng_i = np.random.default_rng(4)
+    cl = np.repeat(np.asarray(tids), chains_per_lig)
+    mi = mcmod.MonteCarlo(ctx, cl, synth.random_rotations(rng_i, len(cl)),
+                          rng_i.uniform(-1.0, 1.0, (len(cl), 3)), ff.INTERP, total_frames=frames,
+                          flags=flags, roi_center=roi[0], roi_r=roi[1], seed=2024)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mi.run_graph(frames // nrep, nreplay=nrep)
+    torch.cuda.synchronize()
+    ti = time.perf_counter() - t0
+    steps_i = float(mi.state()["frame"].sum())
     return {"batched": steps_b / tb, "per_ligand": steps_p / tp,
+            "batched_interp": steps_i / ti,
+            "interp_grids": "10 x 100^3 FP32 at 0.375 A (%.0f MB)" % (10 * 100 ** 3 * 4 / 1e6),
             "speedup": (steps_b / tb) / (steps_p / tp), "unit": "MC steps/s",
             "chains": m.nchain, "frames": frames, "chains_finished": done,
             "too_long_chains": int((st["status"] == 2).sum()),
@@ -294,8 +348,8 @@ def main():
     ap.add_argument("--rotations", type=int, default=N_ROT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-scan", action="store_true", help="headline scan only (no extras)")
-    ap.add_argument("--extras", default="interp,egrad,min,mc",
-                    help="comma list of secondary figures to run: interp, egrad, min, mc")
+    ap.add_argument("--extras", default="select,interp,egrad,min,mc",
+                    help="comma list of secondary figures to run: select, interp, egrad, min, mc")
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--mc-chains", type=int, default=64)
     ap.add_argument("--mc-frames", type=int, default=1000)
@@ -381,6 +435,9 @@ def main():
     want = set(x for x in args.extras.split(",") if x) if not args.only_scan else set()
     if want:
         extra = {}
+        if "select" in want:
+            log("scan filters and top-k")
+            extra["scan_select"] = bench_select(ctx, lig, rec, lid, rots, trans, out)
         if "interp" in want:
             log("interp scan")
             extra["scan_interp"] = bench_interp(ctx, lig, rots, trans, args.steps, args.warmup)

@@ -149,10 +149,25 @@ __global__ void __launch_bounds__(64) k_topk_eq_count(const double* e, size_t n,
   if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
 }
 
-__global__ void k_topk_eq_scan(unsigned* bcnt, int nb) {
-  if (threadIdx.x != 0) return;
-  unsigned run = 0;
-  for (int b = 0; b < nb; ++b) {
+// exclusive prefix sum of the per-chunk counts: each thread scans a contiguous run of
+// chunks, the run totals are scanned across the workgroup
+__global__ void __launch_bounds__(1024) k_topk_eq_scan(unsigned* bcnt, int nb) {
+  __shared__ unsigned part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  unsigned sum = 0;
+  for (int b = b0; b < b1; ++b) sum += bcnt[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = part[t] - sum;
+  for (int b = b0; b < b1; ++b) {
     const unsigned c = bcnt[b];
     bcnt[b] = run;
     run += c;
@@ -230,7 +245,7 @@ hipError_t launch_topk(const double* e, size_t n, int k, double* out_e, int64_t*
   hipLaunchKernelGGL(k_topk_less, dim3(blocks), dim3(256), 0, s, e, n, w.st, w.cnt, out_e, out_i);
   const int nb = (int)((n + w.chunk - 1) / w.chunk);
   hipLaunchKernelGGL(k_topk_eq_count, dim3(nb), dim3(64), 0, s, e, n, w.chunk, w.st, w.bcnt);
-  hipLaunchKernelGGL(k_topk_eq_scan, dim3(1), dim3(64), 0, s, w.bcnt, nb);
+  hipLaunchKernelGGL(k_topk_eq_scan, dim3(1), dim3(1024), 0, s, w.bcnt, nb);
   hipLaunchKernelGGL(k_topk_eq_write, dim3(nb), dim3(64), 0, s, e, n, w.chunk, w.st, w.bcnt,
                      (uint64_t)k, out_e, out_i);
   hipLaunchKernelGGL(k_topk_sort, dim3(1), dim3(1024), 0, s, out_e, out_i, k);
