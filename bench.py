@@ -268,6 +268,70 @@ def bench_min(ctx, ligs, ids, poses_per_lig=4, iters=200):
             "mean_accepted_steps": float(np.mean(info["accepted"]))}
 
 
+def bench_config4(n_lig=4096, chains=32, frames=500, min_iters=50):
+    """At scale: the 15,000-atom receptor in a 60 A box, 10 per-type grids of 128^3 at
+    0.375 A computed on the GPU, 4,096 library ligands x 32 chains x 500 Monte Carlo frames
+    (grid interpolation, one persistent launch replayed from a hipGraph), then 50 batched
+    L-BFGS iterations (direct shifted energy + gradient) for one pose per chain."""
+    import torch
+    from mmo_amd import ff, synth
+    from mmo_amd import mc as mcmod
+    from mmo_amd.minimize import Minimizer, roi_bounds
+    t0 = time.perf_counter()
+    rec = synth.config4_receptor()
+    ligs = synth.library(n_lig, processes=min(16, cpu_threads()))
+    setup_s = time.perf_counter() - t0
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    probe_a, probe_q, types = synth.grid_types(ligs, 10)
+    origin, dims, step = synth.grid_spec((0.0, 0.0, 0.0), 128, 0.375)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.compute_grids(probe_a, probe_q, step, dims, origin, copy_out=False)
+    torch.cuda.synchronize()
+    grid_s = time.perf_counter() - t0
+    ids = [ctx.add_ligand(l, types=t) for l, t in zip(ligs, types)]
+    rng = np.random.default_rng(44)
+    cl = np.repeat(np.asarray(ids), chains)
+    roi = (np.zeros(3), 10.0)
+    m = mcmod.MonteCarlo(ctx, cl, synth.random_rotations(rng, len(cl)),
+                         rng.uniform(-8.0, 8.0, (len(cl), 3)), ff.INTERP, total_frames=frames,
+                         flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI,
+                         roi_center=roi[0], roi_r=roi[1], seed=4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.run_graph(frames // 4, nreplay=4)
+    torch.cuda.synchronize()
+    mc_s = time.perf_counter() - t0
+    st = m.state()
+    steps = float(st["frame"].sum())
+    atoms_per_step = float(np.mean([len(ligs[ids.index(l)]["xs"]) for l in cl]))
+    del m
+    # one pose per chain through the minimiser, direct shifted energy + gradient
+    pose_lig = cl
+    nrb = [len(ligs[ids.index(l)]["rb_left"]) for l in pose_lig]
+    b = ctx.batch(pose_lig)
+    lo, hi = roi_bounds(b, roi[0], roi[1], nrb)
+    d0 = np.concatenate([np.concatenate([rng.uniform(-8, 8, 3), rng.uniform(-3, 3, 3),
+                                         rng.uniform(-0.2, 0.2, nrb[p])]) for p in range(len(pose_lig))])
+    d = torch.tensor(d0, device="cuda:0")
+    mz = Minimizer(b, flags=ff.F_INTER_SHIFTED | ff.F_INTRA, history=8, bounds=(lo, hi), max_step=0.2)
+    mz.run(d, 1)
+    d.copy_(torch.tensor(d0, device="cuda:0"))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mz.run(d, min_iters)
+    torch.cuda.synchronize()
+    min_s = time.perf_counter() - t0
+    return {"receptor_atoms": len(rec["xs"]), "receptor": "60 A box, min separation 2.0 A",
+            "ligands": n_lig, "chains": len(cl), "frames": frames,
+            "grids": "10 x 128^3 FP32 at 0.375 A (%.0f MB)" % (10 * 128 ** 3 * 4 / 1e6),
+            "grid_precompute_s": grid_s, "mc_steps_per_s": steps / mc_s, "mc_s": mc_s,
+            "mc_gather_GBs": 32.0 * atoms_per_step * steps / mc_s / 1e9,
+            "min_pose_evals_per_s": len(pose_lig) * min_iters / min_s, "min_iterations": min_iters,
+            "min_poses": len(pose_lig), "min_s": min_s, "host_setup_s": setup_s}
+
+
 def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample, autotune):
     """MC steps/s: all chains of all ligands in one launch (hipGraph replays) vs one
     launch per ligand (the latter on the first `per_lig_sample` ligands)."""
@@ -349,7 +413,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only-scan", action="store_true", help="headline scan only (no extras)")
     ap.add_argument("--extras", default="select,interp,egrad,min,mc",
-                    help="comma list of secondary figures to run: select, interp, egrad, min, mc")
+                    help="comma list of secondary figures to run: select, interp, egrad, min, mc,"
+                         " config4 (the 15,000-atom workload; not in the default list)")
     ap.add_argument("--ligands", type=int, default=1024)
     ap.add_argument("--mc-chains", type=int, default=64)
     ap.add_argument("--mc-frames", type=int, default=1000)
@@ -451,6 +516,9 @@ def main():
         if "min" in want:
             log("L-BFGS minimisation")
             extra["min"] = bench_min(ctx, ligs, ids)
+        if "config4" in want:
+            log("at scale: 15,000-atom receptor, 128^3 grids, 131,072 chains")
+            extra["config4"] = bench_config4()
         if "mc" in want:
             log("Monte Carlo, batched vs per ligand")
             extra["mc"] = bench_mc(ctx, ligs, ids, args.mc_chains, args.mc_frames,
