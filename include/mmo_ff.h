@@ -17,6 +17,9 @@
  *   - Coordinates are FP64 in the caller's frame.  The library re-origins them (in
  *     FP64) and keeps receptor/ligand positions as 32-bit fixed point, so pair
  *     differences are exact; pair arithmetic is FP32, per-pose sums are FP64.
+ *     Host-buffer entry points reject non-finite ligand input (MMO_ERR_BAD_ARG); in
+ *     device batches a non-finite atom is treated like an atom beyond the fixed-point
+ *     range (no contribution to the shifted form, NaN energy in the global form).
  *   - A context belongs to the process that created it: calls from a forked child
  *     fail with MMO_ERR_FORK (HIP state does not survive fork()).
  *   - A context must be used by one thread at a time.

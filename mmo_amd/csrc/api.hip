@@ -358,6 +358,9 @@ int plain_setup(mmo_ctx* c, const double* xs, const double* ys, const double* zs
                 const double* q, const int32_t* anum, int n, bool need_params) {
   std::vector<double> xyz(3 * (size_t)n);
   for (int j = 0; j < n; ++j) {
+    if (!std::isfinite(xs[j]) || !std::isfinite(ys[j]) || !std::isfinite(zs[j]) ||
+        (q && !std::isfinite(q[j])))
+      return fail(c, MMO_ERR_BAD_ARG, "ligand atom %d: coordinates or charge not finite", j);
     xyz[3 * j] = xs[j]; xyz[3 * j + 1] = ys[j]; xyz[3 * j + 2] = zs[j];
   }
   std::vector<LigAtom> la(n);
@@ -818,23 +821,27 @@ int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch**
   b->pose_lig.assign(pose_lig, pose_lig + npose);
   b->atom_off.resize(npose + 1);
   b->dof_off.resize(npose + 1);
-  int na = 0, nd = 0;
+  long long na = 0, nd = 0;
   for (int p = 0; p < npose; ++p) {
     const int l = pose_lig[p];
     if (l < 0 || l >= (int)c->ligs.size()) {
       delete b;
       return fail(c, MMO_ERR_BAD_ARG, "pose %d: unknown ligand %d", p, l);
     }
-    b->atom_off[p] = na;
-    b->dof_off[p] = nd;
+    b->atom_off[p] = (int)na;
+    b->dof_off[p] = (int)nd;
     na += c->ligs[l].n;
-    nd += 6 + (int)c->ligs[l].rb_left.size();
+    nd += 6 + (long long)c->ligs[l].rb_left.size();
+    if (na > 2147483647LL || nd > 2147483647LL) {
+      delete b;
+      return fail(c, MMO_ERR_BAD_ARG, "batch too large: more than 2^31 - 1 atom rows");
+    }
     b->max_atoms = std::max(b->max_atoms, c->ligs[l].n);
   }
-  b->atom_off[npose] = na;
-  b->dof_off[npose] = nd;
-  b->natoms = na;
-  b->ndof = nd;
+  b->atom_off[npose] = (int)na;
+  b->dof_off[npose] = (int)nd;
+  b->natoms = (int)na;
+  b->ndof = (int)nd;
   plan_tiles(b->pose_lig, &c->ligs, nullptr, c->inter_k, b->perm, b->tiles);
   b->ntiles = (int)b->tiles.size();
   hipStream_t s = c->stream;

@@ -64,12 +64,13 @@ struct CellGrid {
 // contiguous after planning); each pose is handled by `split` lanes.
 struct Tile { int lig, pose0, npose, split; };
 
+// non-finite coordinates saturate like far-away ones (beyond every cutoff)
 __device__ __forceinline__ int to_fixed(double v, double o, double s, int lim, int* sat) {
   double t = (v - o) * s;
   const double dl = (double)lim;
   if (!(t >= -dl && t <= dl)) {   // also catches NaN
     *sat = 1;
-    t = (t != t) ? 0.0 : fmin(fmax(t, -dl), dl);
+    t = (t != t) ? dl : fmin(fmax(t, -dl), dl);
   }
   return __double2int_rn(t);
 }

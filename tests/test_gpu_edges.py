@@ -1,0 +1,153 @@
+"""Edge cases and error paths through the C ABI: empty and single-atom inputs, rigid
+ligands (no rotatable bonds) in every caller, saturated and out-of-grid atoms, size
+limits, and calls made in the wrong order.  Errors must come back as status codes with
+a message, never as a fault."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from mmo_amd import ff, synth
+from mmo_amd._lib import (ERR_BAD_ARG, ERR_NO_GRID, ERR_NO_RECEPTOR, ERR_OUT_OF_GRID, ERR_RANGE,
+                          MMOError)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def system():
+    lig, rec = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    return ctx, lig, rec
+
+
+def _code(fn):
+    with pytest.raises(MMOError) as e:
+        fn()
+    return e.value.code
+
+
+def test_empty_and_wrong_order_calls():
+    ctx = ff.Context(0)
+    lig, _ = synth.config0()
+    print("edge: calls before set-up", flush=True)
+    assert _code(lambda: ctx.ene_inter(lig)) == ERR_NO_RECEPTOR
+    assert _code(lambda: ctx.ene_inter_interp(lig, types=np.zeros(75, dtype=np.int32))) == ERR_NO_GRID
+    empty = {"xs": np.zeros(0), "ys": np.zeros(0), "zs": np.zeros(0), "q": np.zeros(0),
+             "anum": np.zeros(0, dtype=np.int32)}
+    assert _code(lambda: ctx.set_receptor(empty)) == ERR_BAD_ARG
+    assert _code(lambda: ctx.batch([])) == ERR_BAD_ARG
+    assert _code(lambda: ctx.batch([0])) == ERR_BAD_ARG          # no ligand registered
+    assert _code(lambda: ctx.scan(0, ff.SHIFTED, np.eye(3)[None], np.zeros((1, 3)))) == ERR_BAD_ARG
+
+
+def test_single_atom_ligand_every_form(system):
+    ctx, _, rec = system
+    one = {"xs": np.array([30.0]), "ys": np.zeros(1), "zs": np.zeros(1), "q": np.array([0.3]),
+           "anum": np.array([8], dtype=np.int32), "dists": np.zeros((1, 1), dtype=np.int32)}
+    print("edge: one-atom ligand", flush=True)
+    for mode, shifted in ((ff.SHIFTED, True), (ff.GLOBAL, False)):
+        assert ctx.ene_inter(one, mode) == pytest.approx(oracle.ene_inter(rec, one, shifted),
+                                                         rel=1e-5, abs=1e-7)
+    assert ctx.ene_intra(one) == 0.0
+    lid = ctx.add_ligand(dict(one, xs=np.zeros(1)))
+    b = ctx.batch([lid, lid, lid])
+    xyz = np.array([[30.0, 0, 0], [0.0, 29.0, 0], [200.0, 0, 0]])
+    e, g = b.score(xyz, ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD)
+    for p in range(3):
+        m = dict(one, xs=xyz[p:p + 1, 0], ys=xyz[p:p + 1, 1], zs=xyz[p:p + 1, 2])
+        assert e[p, 0] == pytest.approx(oracle.ene_inter(rec, m), rel=1e-5, abs=1e-7)
+        assert e[p, 1] == 0.0
+    assert e[2, 0] == 0.0 and np.all(g[2] == 0.0)                 # beyond every cutoff
+
+
+def test_saturated_atoms(system):
+    """Atoms beyond the fixed-point range: exactly 0 in the shifted form, an error in
+    the global form (whose sum has no cutoff)."""
+    ctx, lig, _ = system
+    far = dict(lig, xs=np.asarray(lig["xs"]) + 1e6)
+    print("edge: saturated coordinates", flush=True)
+    assert ctx.ene_inter(far, ff.SHIFTED) == 0.0
+    assert _code(lambda: ctx.ene_inter(far, ff.GLOBAL)) == ERR_RANGE
+    # non-finite input: rejected by the plain forms; in batches it counts as out of range
+    nan = dict(lig, xs=np.full(len(lig["xs"]), np.nan))
+    assert _code(lambda: ctx.ene_inter(nan, ff.SHIFTED)) == ERR_BAD_ARG
+    lid = ctx.add_ligand(lig)
+    b = ctx.batch([lid])
+    e, _ = b.score(np.full((len(lig["xs"]), 3), np.nan), ff.F_INTER_SHIFTED)
+    assert e[0, 0] == 0.0
+
+
+def test_out_of_grid(system):
+    ctx, lig, _ = system
+    rng = np.random.default_rng(0)
+    grids = rng.normal(size=(2, 8, 8, 8)).astype(np.float32)
+    ctx.set_grids(grids, 0.5, (0.0, 0.0, 0.0))
+    probe = {"xs": np.array([10.0]), "ys": np.array([1.0]), "zs": np.array([1.0])}
+    print("edge: out of grid", flush=True)
+    assert _code(lambda: ctx.ene_inter_interp(probe, types=[0])) == ERR_OUT_OF_GRID
+    assert _code(lambda: ctx.ene_inter_interp(dict(probe, xs=np.array([1.0])), types=[5])) == ERR_BAD_ARG
+
+
+def test_rigid_ligand_in_dof_mc_and_minimiser(system):
+    """A ligand with no rotatable bond: 6 DOFs, torsion frames are no-ops in MC."""
+    import torch
+    from mmo_amd import mc as mcmod
+    from mmo_amd.minimize import Minimizer
+    ctx, lig, rec = system
+    rigid = dict(lig, rb_left=[], rb_right=[], rg_off=[0], rg_atoms=[])
+    lid = ctx.add_ligand(rigid)
+    b = ctx.batch([lid] * 4)
+    assert int(b.dof_off[-1]) == 24
+    d = torch.tensor(np.tile([30.0, 0, 0, 0.1, 0.2, 0.3], 4), device="cuda:0")
+    print("edge: rigid ligand minimiser", flush=True)
+    e = Minimizer(b, max_step=0.2).run(d, 20).cpu().numpy()
+    assert np.all(np.isfinite(e))
+    print("edge: rigid ligand MC", flush=True)
+    m = mcmod.MonteCarlo(ctx, [lid] * 3, np.tile(np.eye(3), (3, 1, 1)),
+                         np.array([[30.0, 0, 0]] * 3), ff.SHIFTED, total_frames=20,
+                         flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI,
+                         roi_center=np.array([30.0, 0, 0]), roi_r=4.0, seed=2)
+    m.run(20)
+    st = m.state()
+    assert (st["frame"] == 20).all() and (st["status"] == 1).all()
+    # torsion frames leave the conformer as registered
+    conf, _ = m.coords()
+    c = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1)
+    assert np.array_equal(conf[0], c)
+
+
+def test_size_limits(system):
+    import torch
+    from mmo_amd import mc as mcmod
+    from mmo_amd.minimize import Minimizer
+    ctx, lig, _ = system
+    print("edge: size limits", flush=True)
+    # more than 32 rotatable bonds: rejected by the chains, accepted by the batches
+    n = 40
+    chain = {"xs": np.arange(n) * 1.5, "ys": np.zeros(n), "zs": np.zeros(n), "q": np.zeros(n),
+             "anum": np.full(n, 6, dtype=np.int32),
+             "dists": np.abs(np.arange(n)[:, None] - np.arange(n)[None, :]).astype(np.int32),
+             "rb_left": list(range(0, 36)), "rb_right": list(range(1, 37)),
+             "rg_off": [0], "rg_atoms": []}
+    groups = [list(range(k + 2, n)) for k in range(36)]
+    chain["rg_off"] = np.concatenate([[0], np.cumsum([len(g) for g in groups])])
+    chain["rg_atoms"] = np.concatenate(groups)
+    chain["xs"] = chain["xs"] - chain["xs"].mean()
+    lid = ctx.add_ligand(chain)
+    assert _code(lambda: mcmod.MonteCarlo(ctx, [lid], np.eye(3)[None], np.zeros((1, 3)),
+                                          ff.SHIFTED, total_frames=1)) == ERR_BAD_ARG
+    b = ctx.batch([lid])
+    assert int(b.dof_off[-1]) == 42
+    d = torch.zeros(42, dtype=torch.float64, device="cuda:0")
+    e = torch.zeros((1, 4), dtype=torch.float64, device="cuda:0")
+    b.score_dofs_dev(d, e, flags=ff.F_INTER_SHIFTED | ff.F_INTRA)
+    # top-k bounds
+    with pytest.raises(MMOError):
+        ctx.topk(torch.zeros(10, dtype=torch.float64, device="cuda:0"), 5000)
+    # minimiser history bounds
+    assert _code(lambda: Minimizer(b, history=0)) == ERR_BAD_ARG
+    assert _code(lambda: Minimizer(b, history=17)) == ERR_BAD_ARG
+    torch.cuda.synchronize()
