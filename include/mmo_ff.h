@@ -300,9 +300,10 @@ int mmo_fixed_point_bits(const mmo_ctx* ctx);
 /* Kernel shape parameters, set by the autotuner (mmo_amd/tune.py): "inter_k",
  * "inter_waves", "inter_rtile", "cull_k", "cull_waves", "scan_cull", "cell_size"
  * (applies to the next mmo_set_receptor), "mc_waves_per_cu", "pose_cull" (1: batches in
- * the shifted form run one wavefront per pose with the receptor culled per 8-atom
- * sub-block; 0: tiles of poses over the whole receptor).  No counterpart in the
- * reference. */
+ * the shifted form run one wavefront per pose with a culled receptor sweep; 0: tiles of
+ * poses over the whole receptor), "pose_sweep" (that sweep: 1 lanes over the receptor
+ * atoms near each ligand atom, 0 per 8-atom sub-block), "mc_sweep" (the same choice for
+ * the Monte Carlo direct sum).  No counterpart in the reference. */
 int mmo_tune_set(mmo_ctx* ctx, const char* key, double value);
 double mmo_tune_get(mmo_ctx* ctx, const char* key);
 

@@ -83,6 +83,8 @@ struct mmo_ctx {
   // receptor cell list (culled shifted kernel)
   float cell_size = 5.0f;
   int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
+  int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
+  int mc_sweep = 0;           // the same choice for the Monte Carlo direct sum
   bool cull_ok = false;
   int scan_cull = 1;
   int cull_k = 1;
@@ -920,13 +922,13 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     timer_begin(c, "inter", s);
     if (!want_grad || comp) {
       a.e_comp = comp ? e_out : nullptr;
-      HIPCHK(c, launch_pose_cull(L, a, false, waves, s));
+      HIPCHK(c, launch_pose_cull(L, a, false, waves, c->pose_sweep == 1, s));
     }
     if (want_grad) {
       a.e_comp = nullptr;
       a.grad = grad;
       a.write_e = comp ? 0 : 1;
-      HIPCHK(c, launch_pose_cull(L, a, true, waves, s));
+      HIPCHK(c, launch_pose_cull(L, a, true, waves, c->pose_sweep == 1, s));
     }
     timer_end(c, "inter", s);
   } else if (inter & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL)) {
@@ -1308,7 +1310,7 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.max_rot0 = 15.0 * deg; a.max_trans0 = 0.15;
   a.roi_x = m->roi[0]; a.roi_y = m->roi[1]; a.roi_z = m->roi[2]; a.roi_r = m->roi_r;
   a.inter_mode = m->inter_mode; a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p;
-  a.nrec = c->nrec;
+  a.nrec = c->nrec; a.sweep = c->mc_sweep;
   a.cg = c->cg; a.g = c->g;
   return a;
 }
@@ -1787,6 +1789,8 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
+  else if (k == "pose_sweep" && (iv == 0 || iv == 1)) c->pose_sweep = iv;
+  else if (k == "mc_sweep" && (iv == 0 || iv == 1)) c->mc_sweep = iv;
   else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
   return MMO_OK;
 }
@@ -1804,6 +1808,8 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;
   if (k == "mc_waves_per_cu") return c->mc_waves_per_cu;
   if (k == "pose_cull") return c->pose_cull;
+  if (k == "pose_sweep") return c->pose_sweep;
+  if (k == "mc_sweep") return c->mc_sweep;
   return -1.0;
 }
 

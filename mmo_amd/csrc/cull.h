@@ -338,4 +338,103 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
   }
 }
 
+// ------------------------------------------ one pose per wavefront, lanes on receptor
+// For each atom j of the pose in turn, the wavefront visits the receptor atoms of the
+// cell rows that meet the 12 A sphere around atom j itself (no bounding-box growth):
+// lane k fetches the atom range of row k, a wavefront prefix sum lays the ranges end
+// to end, and lane l takes positions l, l + 64, ... of that concatenation, finding its
+// row by binary search over the inclusive prefix sums (cross-lane reads).  Work per
+// atom follows the receptor atoms near it, spread evenly over the 64 lanes.
+// atom(j, lx, ly, lz, cq, ca, cb) gives atom j (wave-uniform); with GRAD,
+// gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from lane 0.  Sums: FP32 per atom
+// and per 64 rows, FP64 across atoms.
+template <bool GRAD, class AtomFn, class GradOut>
+__device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
+                                           const RecPos* __restrict__ s_pos,
+                                           const RecAB* __restrict__ s_ab, const Frame& fr,
+                                           int n_atoms, AtomFn&& atom, GradOut&& gout,
+                                           double& e_el, double& e_vd) {
+  const int lane = threadIdx.x & (kWave - 1);
+  constexpr float R = 12.05f;          // cutoff + FP32 rounding margin
+  constexpr float R2 = R * R;
+  for (int j = 0; j < n_atoms; ++j) {
+    int lx, ly, lz;
+    double cq, ca, cb;
+    atom(j, lx, ly, lz, cq, ca, cb);
+    const float qx = uniformf((float)lx * fr.inv_s), qy = uniformf((float)ly * fr.inv_s),
+                qz = uniformf((float)lz * fr.inv_s);
+    const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
+    const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
+    const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
+    const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
+    const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
+    const int nyc = cy1 - cy0 + 1;
+    const int nrows = (cz1 >= cz0 && nyc > 0) ? (cz1 - cz0 + 1) * nyc : 0;
+    float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    for (int base = 0; base < nrows; base += kWave) {
+      int r0 = 0, len = 0;
+      const int idx = base + lane;
+      if (idx < nrows) {
+        const int cz = cz0 + idx / nyc, cy = cy0 + idx % nyc;
+        const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
+        const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
+        const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
+        const float rx2 = R2 - dzb * dzb - dyb * dyb;
+        if (rx2 >= 0.0f) {
+          const float rx = sqrtf(rx2);
+          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
+          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+          if (cx0 <= cx1) {
+            const int rowb = (cz * cg.ny + cy) * cg.nx;
+            r0 = s_cell[rowb + cx0];
+            len = s_cell[rowb + cx1 + 1] - r0;
+          }
+        }
+      }
+      // inclusive prefix sum of the row lengths across the wavefront
+      int incl = len;
+      for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+      }
+      const int total = uniformi(__shfl(incl, kWave - 1, kWave));
+      // whole-wavefront iterations: every lane takes part in the cross-lane reads (a
+      // read from an inactive lane is undefined); lanes past the end only skip the pair
+      for (int t0 = 0; t0 < total; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool valid = t < total;
+        // the row holding position t: the first k with incl_k > t
+        int lo = 0, hi = kWave - 1;
+#pragma unroll
+        for (int step = 0; step < 6; ++step) {
+          const int mid = (lo + hi) >> 1;
+          const int v = __shfl(incl, mid, kWave);
+          if (v > t) hi = mid; else lo = mid + 1;
+        }
+        const int k = lo;
+        const int rk = __shfl(r0, k, kWave), ik = __shfl(incl, k, kWave), nk = __shfl(len, k, kWave);
+        if (valid) {
+          const int i = rk + t - (ik - nk);
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
+          if (GRAD)
+            pair_grad_shifted(p, ab, lx, ly, lz, fr.inv_s2, c1, fa, fb, fe, fx, fy, fz);
+          else
+            pair_energy<MMO_SHIFTED>(p, ab, lx, ly, lz, fr.inv_s2, sq, s12, s6);
+        }
+      }
+    }
+    if (GRAD) {
+      e_vd += (double)fe;
+      const double gs = -2.0 * (double)fr.inv_s;
+      const double gx = wave_sum((double)fx), gy = wave_sum((double)fy), gz = wave_sum((double)fz);
+      if (lane == 0) gout(j, gs * gx, gs * gy, gs * gz);
+    } else {
+      e_el = fma((double)sq, cq, e_el);
+      e_vd = fma((double)s12, ca, e_vd);
+      e_vd = fma(-2.0 * (double)s6, cb, e_vd);
+    }
+  }
+}
+
 }  // namespace mmo

@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
 // Shifted form for arbitrary poses (batches, gradients, minimisation): one wavefront per
 // pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep), so the
 // cost follows the pairs inside the cutoff wherever the poses are.
-template <bool GRAD, bool LDS>
+template <bool GRAD, bool LDS, bool ATOMWISE>
 __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, const RecPos* rpos,
                                                     const RecAB* rab, int nrec, CellGrid cg) {
   extern __shared__ __align__(16) char smem[];
@@ -359,20 +359,21 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
   const double* x = a.xyz + 3 * row0;
   int sat = 0;
   double e_el = 0.0, e_vd = 0.0;
-  pose_sweep<GRAD>(
-      cg, s_cell, s_pos, s_ab, fr, li.n_atoms,
-      [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
-        lx = to_fixed(x[3 * j], fr.ox, fr.scale, fr.lim, &sat);
-        ly = to_fixed(x[3 * j + 1], fr.oy, fr.scale, fr.lim, &sat);
-        lz = to_fixed(x[3 * j + 2], fr.oz, fr.scale, fr.lim, &sat);
-        const LigAtom la = a.latoms[li.atom_base + j];
-        cq = la.keq; ca = la.a; cb = la.b;
-      },
-      [&](int j, double gx, double gy, double gz) {
-        double* g = a.grad + 3 * (row0 + j);
-        g[0] += gx; g[1] += gy; g[2] += gz;
-      },
-      e_el, e_vd);
+  auto atom = [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
+    lx = to_fixed(x[3 * j], fr.ox, fr.scale, fr.lim, &sat);
+    ly = to_fixed(x[3 * j + 1], fr.oy, fr.scale, fr.lim, &sat);
+    lz = to_fixed(x[3 * j + 2], fr.oz, fr.scale, fr.lim, &sat);
+    const LigAtom la = a.latoms[li.atom_base + j];
+    cq = la.keq; ca = la.a; cb = la.b;
+  };
+  auto gout = [&](int j, double gx, double gy, double gz) {
+    double* g = a.grad + 3 * (row0 + j);
+    g[0] += gx; g[1] += gy; g[2] += gz;
+  };
+  if (ATOMWISE)
+    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, gout, e_el, e_vd);
+  else
+    pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, gout, e_el, e_vd);
   (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
   // every lane of an atom slot holds part of the sums: the wavefront sum counts each
   // pair once (lanes of one slot share the atom, split the receptor atoms)
@@ -389,7 +390,7 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
 }
 
 hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
-                            hipStream_t s) {
+                            bool atomwise, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
   waves = std::max(1, std::min(16, waves));
   const int blocks = (a.npose + waves - 1) / waves;
@@ -397,21 +398,25 @@ hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool gr
   const size_t lds = in_lds ? (size_t)L.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
                                   (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int)
                             : 0;
-#define MMO_POSE_CULL(G)                                                                       \
+#define MMO_POSE_CULL(G, A)                                                                    \
   if (in_lds) {                                                                                \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pose_cull<G, true>),   \
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pose_cull<G, true, A>), \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);  \
     if (e != hipSuccess) return e;                                                             \
-    hipLaunchKernelGGL((k_pose_cull<G, true>), dim3(blocks), dim3(waves * kWave), lds, s, a,   \
-                       L.fr, L.rpos, L.rab, L.nrec, L.cg);                                     \
+    hipLaunchKernelGGL((k_pose_cull<G, true, A>), dim3(blocks), dim3(waves * kWave), lds, s,   \
+                       a, L.fr, L.rpos, L.rab, L.nrec, L.cg);                                  \
   } else {                                                                                     \
-    hipLaunchKernelGGL((k_pose_cull<G, false>), dim3(blocks), dim3(waves * kWave), 0, s, a,    \
+    hipLaunchKernelGGL((k_pose_cull<G, false, A>), dim3(blocks), dim3(waves * kWave), 0, s, a, \
                        L.fr, L.rpos, L.rab, L.nrec, L.cg);                                     \
   }
-  if (grad) {
-    MMO_POSE_CULL(true)
+  if (grad && atomwise) {
+    MMO_POSE_CULL(true, true)
+  } else if (grad) {
+    MMO_POSE_CULL(true, false)
+  } else if (atomwise) {
+    MMO_POSE_CULL(false, true)
   } else {
-    MMO_POSE_CULL(false)
+    MMO_POSE_CULL(false, false)
   }
 #undef MMO_POSE_CULL
   return hipGetLastError();

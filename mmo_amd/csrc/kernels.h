@@ -121,8 +121,10 @@ struct PoseCullArgs {
   double* grad;             // GRAD: dE/dxyz rows, added to what the buffer holds
   int write_e;
 };
+// atomwise: lanes over the receptor atoms near each ligand atom (atom_sweep) instead of
+// 8-atom sub-blocks (pose_sweep)
 hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
-                            hipStream_t s);
+                            bool atomwise, hipStream_t s);
 
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);

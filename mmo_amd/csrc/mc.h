@@ -72,6 +72,7 @@ struct McArgs {
   const RecPos* rpos;
   const RecAB* rab;
   int nrec;
+  int sweep;                 // 1: atom_sweep, 0: pose_sweep (direct sum, FP32)
   CellGrid cg;
   GridDev g;
 };

@@ -137,18 +137,20 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
   int sat = 0;
   if (!(a.flags & kMcFp64)) {
     double e_el = 0.0, e_vd = 0.0;
-    pose_sweep<false>(
-        a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms,
-        [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
-          double x, y, z;
-          v.at(j, x, y, z);
-          lx = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
-          ly = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
-          lz = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
-          const LigAtom la = a.latoms[li.atom_base + j];
-          cq = la.keq; ca = la.a; cb = la.b;
-        },
-        [](int, double, double, double) {}, e_el, e_vd);
+    auto atom = [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
+      double x, y, z;
+      v.at(j, x, y, z);
+      lx = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
+      ly = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
+      lz = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+      const LigAtom la = a.latoms[li.atom_base + j];
+      cq = la.keq; ca = la.a; cb = la.b;
+    };
+    auto none = [](int, double, double, double) {};
+    if (a.sweep == 1)
+      atom_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
+    else
+      pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
     return wave_sum(e_el + e_vd);
   }
   constexpr int K = 2;

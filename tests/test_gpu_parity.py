@@ -387,24 +387,26 @@ def test_pose_cull_matches_unculled(lib_sys):
         _, x = _place(ligs[l], synth.random_rotations(rng, 1)[0], r * d / np.linalg.norm(d))
         xyz[b.atom_off[p]:b.atom_off[p + 1]] = x
     flags = ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD
-    print("pose cull vs unculled", b.npose, flush=True)
     out = {}
-    for pc in (1, 0):
-        ctx.tune(pose_cull=pc)
+    for pc, sweep in ((0, 1), (1, 1), (1, 0)):
+        print("pose cull", pc, "sweep", sweep, b.npose, flush=True)
+        ctx.tune(pose_cull=pc, pose_sweep=sweep)
         e, g = b.score(xyz, flags)
         ec, _ = b.score(xyz, ff.F_INTER_SHIFTED | ff.F_COMPONENTS)
-        out[pc] = (e, g, ec)
-    ctx.tune(pose_cull=1)
-    (e1, g1, c1), (e0, g0, c0) = out[1], out[0]
-    # both kernels sum FP32 pair terms in different orders: errors scale with sum |e_pair|
+        out[(pc, sweep)] = (e, g, ec)
+    ctx.tune(pose_cull=1, pose_sweep=1)
+    # the kernels sum FP32 pair terms in different orders: errors scale with sum |e_pair|
     scale = np.array([inter_abs_sum(rec, xyz[b.atom_off[p]:b.atom_off[p + 1]], ligs[l])
                       for p, l in enumerate(pose_lig)]) + 1e-3
-    assert np.all(np.abs(e1[:, 0] - e0[:, 0]) <= 2e-6 * scale)
-    assert np.all(np.abs(c1[:, 2:] - c0[:, 2:]) <= 2e-6 * scale[:, None])
-    assert np.array_equal(e1[:, 1], e0[:, 1])
-    for p in range(b.npose):
-        ga, gb = g1[b.atom_off[p]:b.atom_off[p + 1]], g0[b.atom_off[p]:b.atom_off[p + 1]]
-        assert np.abs(ga - gb).max() <= 2e-5 * max(np.abs(gb).max(), 1e-3), p
+    e0, g0, c0 = out[(0, 1)]
+    for key in ((1, 1), (1, 0)):
+        e1, g1, c1 = out[key]
+        assert np.all(np.abs(e1[:, 0] - e0[:, 0]) <= 2e-6 * scale), key
+        assert np.all(np.abs(c1[:, 2:] - c0[:, 2:]) <= 2e-6 * scale[:, None]), key
+        assert np.array_equal(e1[:, 1], e0[:, 1])
+        for p in range(b.npose):
+            ga, gb = g1[b.atom_off[p]:b.atom_off[p + 1]], g0[b.atom_off[p]:b.atom_off[p + 1]]
+            assert np.abs(ga - gb).max() <= 2e-5 * max(np.abs(gb).max(), 1e-3), (key, p)
 
 
 def test_rigid_scan_culled_equals_unculled_and_oracle(sys0):
@@ -605,15 +607,20 @@ def test_mc_fp32_matches_cpu_restatement():
     mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=19)
     roi = (SURFACE, 4.0)
     flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
-    m = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=60, flags=flags,
-                         roi_center=roi[0], roi_r=roi[1], seed=3)
-    m.run(60)
-    st = m.state()
-    for k, l in enumerate(chain_lig):
-        ref = simulate(ligs[ids.index(l)], rec, rot0[k], pos0[k], 60, seed=3, chain=k, hard_roi=roi)
-        assert st["accepted"][k] == ref["accepted"]
-        assert st["resets"][k] == ref["resets"]
-        assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-5, abs=1e-3)
+    refs = [simulate(ligs[ids.index(l)], rec, rot0[k], pos0[k], 60, seed=3, chain=k, hard_roi=roi)
+            for k, l in enumerate(chain_lig)]
+    for sweep in (0, 1):          # 8-atom sub-blocks, lanes over receptor atoms
+        ctx.tune(mc_sweep=sweep)
+        print("mc fp32 sweep", sweep, flush=True)
+        m = mcmod.MonteCarlo(ctx, chain_lig, rot0, pos0, ff.SHIFTED, total_frames=60, flags=flags,
+                             roi_center=roi[0], roi_r=roi[1], seed=3)
+        m.run(60)
+        st = m.state()
+        for k, ref in enumerate(refs):
+            assert st["accepted"][k] == ref["accepted"]
+            assert st["resets"][k] == ref["resets"]
+            assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-5, abs=1e-3)
+    ctx.tune(mc_sweep=0)
 
 
 def test_mc_launch_forms_agree():
