@@ -84,7 +84,7 @@ struct mmo_ctx {
   float cell_size = 5.0f;
   int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
   int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
-  int mc_sweep = 0;           // the same choice for the Monte Carlo direct sum
+  int mc_sweep = 1;           // the same choice for the Monte Carlo direct sum
   bool cull_ok = false;
   int scan_cull = 1;
   int cull_k = 1;

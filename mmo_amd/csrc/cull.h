@@ -186,6 +186,15 @@ __device__ __forceinline__ double cull_sweep_f64(const CellGrid& cg, const int* 
   return e;
 }
 
+// bytes of LDS that load_receptor<LDS> uses (rounded up to 16)
+template <bool LDS>
+__host__ __device__ __forceinline__ size_t receptor_lds_bytes(int nrec, const CellGrid& cg) {
+  if (!LDS) return 0;
+  const size_t b = (size_t)nrec * (sizeof(RecPos) + sizeof(RecAB)) +
+                   (size_t)(cg.nx * cg.ny * cg.nz + 1) * sizeof(int);
+  return (b + 15) & ~(size_t)15;
+}
+
 // whole receptor + cell table into LDS (all threads of the block); receptors too large
 // for LDS (LDS = false) are read in place from global memory (L2), by the same
 // uniform row ranges
@@ -341,22 +350,38 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // ------------------------------------------ one pose per wavefront, lanes on receptor
 // For each atom j of the pose in turn, the wavefront visits the receptor atoms of the
 // cell rows that meet the 12 A sphere around atom j itself (no bounding-box growth):
-// lane k fetches the atom range of row k, a wavefront prefix sum lays the ranges end
-// to end, and lane l takes positions l, l + 64, ... of that concatenation, finding its
-// row by binary search over the inclusive prefix sums (cross-lane reads).  Work per
-// atom follows the receptor atoms near it, spread evenly over the 64 lanes.
-// atom(j, lx, ly, lz, cq, ca, cb) gives atom j (wave-uniform); with GRAD,
-// gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from lane 0.  Sums: FP32 per atom
-// and per 64 rows, FP64 across atoms.
+// lane k fetches the atom range of row k, the non-empty rows are compacted into LDS
+// with a wavefront prefix sum that lays their ranges end to end, and the concatenation
+// is walked 64 positions at a time.  Each lane finds the row of its position from a
+// ballot of the row ends falling in the current 64 positions (one LDS scatter + one
+// ballot per 64 pairs), so every lane works on every step.  wl: kAtomSweepInts ints of
+// this wavefront's LDS.  atom(j, lx, ly, lz, cq, ca, cb) gives atom j (wave-uniform);
+// with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from lane 0.  Sums: FP32
+// per atom and per 64 rows, FP64 across atoms.
+constexpr int kAtomSweepInts = 3 * kWave;
+
+// LDS writes of this wavefront made visible to its own lanes
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <bool GRAD, class AtomFn, class GradOut>
 __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab, const Frame& fr,
-                                           int n_atoms, AtomFn&& atom, GradOut&& gout,
+                                           int n_atoms, int* wl, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
   const int lane = threadIdx.x & (kWave - 1);
-  constexpr float R = 12.05f;          // cutoff + FP32 rounding margin
+  int* s_r0 = wl;                 // compacted rows: first receptor atom
+  int* s_beg = wl + kWave;        //                 first position in the concatenation
+  int* s_flag = wl + 2 * kWave;   // row ends inside the current 64 positions
+  const uint64_t below = lane ? ((1ull << lane) - 1ull) : 0ull;
+  const uint64_t upto = (lane == kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
+  constexpr float R = 12.05f;     // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
+  s_flag[lane] = 0;
   for (int j = 0; j < n_atoms; ++j) {
     int lx, ly, lz;
     double cq, ca, cb;
@@ -371,9 +396,9 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
     const int nyc = cy1 - cy0 + 1;
     const int nrows = (cz1 >= cz0 && nyc > 0) ? (cz1 - cz0 + 1) * nyc : 0;
     float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int base = 0; base < nrows; base += kWave) {
+    for (int base_row = 0; base_row < nrows; base_row += kWave) {
       int r0 = 0, len = 0;
-      const int idx = base + lane;
+      const int idx = base_row + lane;
       if (idx < nrows) {
         const int cz = cz0 + idx / nyc, cy = cy0 + idx % nyc;
         const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
@@ -391,30 +416,40 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
           }
         }
       }
-      // inclusive prefix sum of the row lengths across the wavefront
+      // inclusive prefix sum of the row lengths: row k covers [incl - len, incl)
       int incl = len;
       for (int off = 1; off < kWave; off <<= 1) {
         const int v = __shfl_up(incl, off, kWave);
         if (lane >= off) incl += v;
       }
       const int total = uniformi(__shfl(incl, kWave - 1, kWave));
-      // whole-wavefront iterations: every lane takes part in the cross-lane reads (a
-      // read from an inactive lane is undefined); lanes past the end only skip the pair
+      if (total == 0) continue;
+      const uint64_t nonempty = __ballot(len > 0);
+      wave_lds_sync();            // the previous rows' readers are done
+      if (len > 0) {
+        const int c = __popcll(nonempty & below);
+        s_r0[c] = r0;
+        s_beg[c] = incl - len;
+      }
+      int done = 0;               // compacted rows that end before the current positions
       for (int t0 = 0; t0 < total; t0 += kWave) {
-        const int t = t0 + lane;
-        const bool valid = t < total;
-        // the row holding position t: the first k with incl_k > t
-        int lo = 0, hi = kWave - 1;
-#pragma unroll
-        for (int step = 0; step < 6; ++step) {
-          const int mid = (lo + hi) >> 1;
-          const int v = __shfl(incl, mid, kWave);
-          if (v > t) hi = mid; else lo = mid + 1;
+        // rows ending inside [t0, t0 + 63] (ends of non-empty rows are distinct)
+        if (len > 0) {
+          const int b = incl - t0;
+          if (b >= 0 && b < kWave) s_flag[b] = 1;
         }
-        const int k = lo;
-        const int rk = __shfl(r0, k, kWave), ik = __shfl(incl, k, kWave), nk = __shfl(len, k, kWave);
+        wave_lds_sync();
+        const uint64_t ends = __ballot(s_flag[lane] != 0);
+        const int t = t0 + lane;
+        const int k = done + __popcll(ends & upto);   // row holding position t
+        const bool valid = t < total;
+        const int rk = s_r0[valid ? k : 0], bk = s_beg[valid ? k : 0];
+        wave_lds_sync();
+        s_flag[lane] = 0;
+        wave_lds_sync();
+        done += __popcll(ends);
         if (valid) {
-          const int i = rk + t - (ik - nk);
+          const int i = rk + (t - bk);
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
           if (GRAD)

@@ -354,6 +354,9 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
   const int lane = threadIdx.x & (kWave - 1);
   const int p = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
   if (p >= a.npose) return;
+  // this wavefront's scratch for atom_sweep, after the receptor copy
+  int* wl = reinterpret_cast<int*>(smem + receptor_lds_bytes<LDS>(nrec, cg)) +
+            (threadIdx.x / kWave) * kAtomSweepInts;
   const LigInfo li = a.ligs[a.pose_lig[p]];
   const size_t row0 = (size_t)a.atom_off[p];
   const double* x = a.xyz + 3 * row0;
@@ -371,7 +374,7 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
     g[0] += gx; g[1] += gy; g[2] += gz;
   };
   if (ATOMWISE)
-    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, gout, e_el, e_vd);
+    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
   else
     pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, gout, e_el, e_vd);
   (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
@@ -395,9 +398,8 @@ hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool gr
   waves = std::max(1, std::min(16, waves));
   const int blocks = (a.npose + waves - 1) / waves;
   const bool in_lds = L.cull == 1;
-  const size_t lds = in_lds ? (size_t)L.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
-                                  (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int)
-                            : 0;
+  const size_t lds = (in_lds ? receptor_lds_bytes<true>(L.nrec, L.cg) : 0) +
+                     (size_t)waves * kAtomSweepInts * sizeof(int);
 #define MMO_POSE_CULL(G, A)                                                                    \
   if (in_lds) {                                                                                \
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pose_cull<G, true, A>), \
@@ -406,8 +408,8 @@ hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool gr
     hipLaunchKernelGGL((k_pose_cull<G, true, A>), dim3(blocks), dim3(waves * kWave), lds, s,   \
                        a, L.fr, L.rpos, L.rab, L.nrec, L.cg);                                  \
   } else {                                                                                     \
-    hipLaunchKernelGGL((k_pose_cull<G, false, A>), dim3(blocks), dim3(waves * kWave), 0, s, a, \
-                       L.fr, L.rpos, L.rab, L.nrec, L.cg);                                     \
+    hipLaunchKernelGGL((k_pose_cull<G, false, A>), dim3(blocks), dim3(waves * kWave), lds, s,  \
+                       a, L.fr, L.rpos, L.rab, L.nrec, L.cg);                                  \
   }
   if (grad && atomwise) {
     MMO_POSE_CULL(true, true)

@@ -132,7 +132,8 @@ __device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const Pose
 // shifted direct sum of the pose: FP32 pairs swept per 8-atom sub-block (pose_sweep);
 // the FP64 parity mode sweeps with FP64 pair arithmetic, two atoms per lane
 __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v,
-                                  const RecPos* s_pos, const RecAB* s_ab, const int* s_cell) {
+                                  const RecPos* s_pos, const RecAB* s_ab, const int* s_cell,
+                                  int* wl) {
   const Frame fr = a.fr;
   int sat = 0;
   if (!(a.flags & kMcFp64)) {
@@ -148,7 +149,7 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
     };
     auto none = [](int, double, double, double) {};
     if (a.sweep == 1)
-      atom_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
+      atom_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, wl, atom, none, e_el, e_vd);
     else
       pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
     return wave_sum(e_el + e_vd);
@@ -209,11 +210,12 @@ struct RecView {
   const RecPos* pos;
   const RecAB* ab;
   const int* cell;
+  int* wl;                        // this wavefront's atom_sweep scratch (LDS)
 };
 
 __device__ double mc_inter(const McArgs& a, const LigInfo& li, const PoseView& v, const RecView& r) {
   return a.inter_mode == MMO_INTERP ? mc_inter_interp(a, li, v)
-                                    : mc_inter_direct(a, li, v, r.pos, r.ab, r.cell);
+                                    : mc_inter_direct(a, li, v, r.pos, r.ab, r.cell, r.wl);
 }
 
 // LDS writes of one wavefront made visible to its other lanes (the chain's conformer
@@ -243,6 +245,7 @@ __global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
   McState* s_st = reinterpret_cast<McState*>(
       mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 6 * a.max_atoms * sizeof(double) + 15) &
                             ~(size_t)15));
+  rv.wl = reinterpret_cast<int*>(s_st + blockDim.x / kWave) + wv * kAtomSweepInts;
   const double pi = 3.14159265358979311600;       // 4 atan 1 (src/math.ml:13)
 
   for (;;) {
@@ -481,7 +484,7 @@ size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
     rec = ((size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
            (size_t)(a.cg.nx * a.cg.ny * a.cg.nz + 1) * sizeof(int) + 15) & ~(size_t)15;
   const size_t conf = ((size_t)waves * 6 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
-  return rec + conf + (size_t)waves * sizeof(McState);
+  return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int));
 }
 
 hipError_t mc_prepare() {

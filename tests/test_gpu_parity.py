@@ -620,7 +620,7 @@ def test_mc_fp32_matches_cpu_restatement():
             assert st["accepted"][k] == ref["accepted"]
             assert st["resets"][k] == ref["resets"]
             assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-5, abs=1e-3)
-    ctx.tune(mc_sweep=0)
+    ctx.tune(mc_sweep=1)
 
 
 def test_mc_launch_forms_agree():
