@@ -7,6 +7,8 @@
 namespace mmo {
 
 constexpr int kMcBlock = 100;    // Params.block_size, src/params.ml:29
+// energy form of a chain launch (one kernel instantiation each)
+enum { kMcInterGrid = 0, kMcInterAtom = 1, kMcInterSub = 2, kMcInterF64 = 3 };
 constexpr int kMcMaxRb = 32;     // rotatable bonds per ligand
 enum { kMcRunning = 0, kMcDone = 1, kMcTooLong = 2 };
 enum {

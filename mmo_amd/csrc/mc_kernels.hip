@@ -131,12 +131,14 @@ __device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const Pose
 
 // shifted direct sum of the pose: FP32 pairs swept per 8-atom sub-block (pose_sweep);
 // the FP64 parity mode sweeps with FP64 pair arithmetic, two atoms per lane
+// INTER: kMcInterAtom / kMcInterSub (FP32, atom_sweep / pose_sweep) or kMcInterF64
+template <int INTER>
 __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const PoseView& v,
                                   const RecPos* s_pos, const RecAB* s_ab, const int* s_cell,
                                   int* wl) {
   const Frame fr = a.fr;
   int sat = 0;
-  if (!(a.flags & kMcFp64)) {
+  if (INTER != kMcInterF64) {
     double e_el = 0.0, e_vd = 0.0;
     auto atom = [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
       double x, y, z;
@@ -148,7 +150,7 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
       cq = la.keq; ca = la.a; cb = la.b;
     };
     auto none = [](int, double, double, double) {};
-    if (a.sweep == 1)
+    if (INTER == kMcInterAtom)
       atom_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, wl, atom, none, e_el, e_vd);
     else
       pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
@@ -213,9 +215,10 @@ struct RecView {
   int* wl;                        // this wavefront's atom_sweep scratch (LDS)
 };
 
+template <int INTER>
 __device__ double mc_inter(const McArgs& a, const LigInfo& li, const PoseView& v, const RecView& r) {
-  return a.inter_mode == MMO_INTERP ? mc_inter_interp(a, li, v)
-                                    : mc_inter_direct(a, li, v, r.pos, r.ab, r.cell, r.wl);
+  if (INTER == kMcInterGrid) return mc_inter_interp(a, li, v);
+  return mc_inter_direct<INTER>(a, li, v, r.pos, r.ab, r.cell, r.wl);
 }
 
 // LDS writes of one wavefront made visible to its other lanes (the chain's conformer
@@ -229,7 +232,8 @@ __device__ __forceinline__ void wave_sync() {
 // --------------------------------------------------------------------- the kernel
 // A workgroup of `waves` wavefronts, each running its own chains; with REC_LDS the
 // workgroup first stages the receptor and its cell table in LDS, shared by all chains.
-template <bool REC_LDS>
+// one instantiation per energy form, so that each carries only its own registers
+template <bool REC_LDS, int INTER>
 __global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
   extern __shared__ __align__(16) char mc_lds[];
   RecView rv;
@@ -283,7 +287,7 @@ __global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
       PoseView v{cx_, cy_, cz_, {}, 0.0, 0.0, 0.0, st.pos0[0], st.pos0[1], st.pos0[2]};
       for (int k = 0; k < 9; ++k) v.m[k] = st.rot0[k];
       st.prev_e_intra = use_intra ? mc_intra(a, li, cx_, cy_, cz_) : 0.0;
-      st.prev_e_inter = mc_inter(a, li, v, rv);
+      st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
       st.prev_e = st.prev_e_inter + st.prev_e_intra;
       st.best_e = st.prev_e;
       for (int k = 0; k < 9; ++k) { st.rot[k] = st.rot0[k]; st.best_rot[k] = st.rot0[k]; }
@@ -390,7 +394,7 @@ __global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
       PoseView v{sx, sy, sz, {}, ccx, ccy, ccz, npos[0], npos[1], npos[2]};
       for (int k = 0; k < 9; ++k) v.m[k] = nrot[k];
       if (!rigid && use_intra && flex) st.prev_e_intra = mc_intra(a, li, sx, sy, sz);
-      st.prev_e_inter = mc_inter(a, li, v, rv);
+      st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
       const double curr_e = st.prev_e_inter + st.prev_e_intra;
       const double dxr = a.roi_x - npos[0], dyr = a.roi_y - npos[1], dzr = a.roi_z - npos[2];
       const double dist_roi = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
@@ -487,15 +491,31 @@ size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
   return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int));
 }
 
+template <bool R, int I>
+static hipError_t mc_allow_lds(int cap) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<R, I>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+}
+
 hipError_t mc_prepare() {
   // dynamic LDS above 64 KiB must be allowed per kernel (gfx950: 160 KiB per CU); set
   // once, outside any stream capture
   const int cap = 160 * 1024;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<true>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, cap);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<false>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = mc_allow_lds<true, kMcInterAtom>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<true, kMcInterSub>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<true, kMcInterF64>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<false, kMcInterAtom>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<false, kMcInterSub>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<false, kMcInterF64>(cap);
+  if (e == hipSuccess) e = mc_allow_lds<false, kMcInterGrid>(cap);
+  return e;
+}
+
+int mc_inter_kind(const McArgs& a) {
+  if (a.inter_mode == MMO_INTERP) return kMcInterGrid;
+  if (a.flags & kMcFp64) return kMcInterF64;
+  return a.sweep == 1 ? kMcInterAtom : kMcInterSub;
 }
 
 hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s) {
@@ -504,10 +524,17 @@ hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipS
   if (e != hipSuccess) return e;
   const size_t lds = mc_lds_bytes(a, waves, rec_lds);
   const int rec_bytes = (int)(mc_lds_bytes(a, waves, rec_lds) - mc_lds_bytes(a, waves, false));
-  if (rec_lds)
-    hipLaunchKernelGGL(k_mc<true>, dim3(nblocks), dim3(waves * kWave), lds, s, a, rec_bytes);
-  else
-    hipLaunchKernelGGL(k_mc<false>, dim3(nblocks), dim3(waves * kWave), lds, s, a, 0);
+  const dim3 g(nblocks), b(waves * kWave);
+  switch (mc_inter_kind(a) + (rec_lds ? 8 : 0)) {
+    case kMcInterAtom + 8: hipLaunchKernelGGL((k_mc<true, kMcInterAtom>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterSub + 8: hipLaunchKernelGGL((k_mc<true, kMcInterSub>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterF64 + 8: hipLaunchKernelGGL((k_mc<true, kMcInterF64>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterAtom: hipLaunchKernelGGL((k_mc<false, kMcInterAtom>), g, b, lds, s, a, 0); break;
+    case kMcInterSub: hipLaunchKernelGGL((k_mc<false, kMcInterSub>), g, b, lds, s, a, 0); break;
+    case kMcInterF64: hipLaunchKernelGGL((k_mc<false, kMcInterF64>), g, b, lds, s, a, 0); break;
+    case kMcInterGrid: hipLaunchKernelGGL((k_mc<false, kMcInterGrid>), g, b, lds, s, a, 0); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
