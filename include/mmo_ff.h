@@ -303,7 +303,10 @@ int mmo_fixed_point_bits(const mmo_ctx* ctx);
  * the shifted form run one wavefront per pose with a culled receptor sweep; 0: tiles of
  * poses over the whole receptor), "pose_sweep" (that sweep: 1 lanes over the receptor
  * atoms near each ligand atom, 0 per 8-atom sub-block), "mc_sweep" (the same choice for
- * the Monte Carlo direct sum).  No counterpart in the reference. */
+ * the Monte Carlo direct sum), "row_clip" (scan and grid sweeps clip cell rows longer
+ * than this many atoms to the cutoff sphere's x chord), "cell_xsub" (the pose and Monte
+ * Carlo sweeps use cells this many times narrower along x; applies to the next
+ * mmo_set_receptor).  No counterpart in the reference. */
 int mmo_tune_set(mmo_ctx* ctx, const char* key, double value);
 double mmo_tune_get(mmo_ctx* ctx, const char* key);
 

@@ -82,6 +82,8 @@ struct mmo_ctx {
   DevBuf<RecAB> rab;
   // receptor cell list (culled shifted kernel)
   float cell_size = 5.0f;
+  int row_clip = 4;           // scan/grid sweeps: clip rows longer than this to the x chord
+  int cell_xsub = 4;          // pose/MC sweeps: cells cut n-fold along x (rows clipped finer)
   int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
   int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
   int mc_sweep = 1;           // the same choice for the Monte Carlo direct sum
@@ -90,8 +92,11 @@ struct mmo_ctx {
   int cull_k = 1;
   int mc_waves_per_cu = 8;
   int cull_waves = 16;
-  CellGrid cg{};
+  CellGrid cg{};              // scan and grid kernels: cubic cells
   DevBuf<int> d_cell;
+  CellGrid cgp{};             // pose and Monte Carlo sweeps: cells cut along x, same atom order
+  DevBuf<int> d_cellp;
+  bool cullp_ok = false;
   // grids
   bool has_grid = false;
   int ntypes = 0;
@@ -512,10 +517,30 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
     cell[i] = (cz * gny + cy) * gnx + cx;
     order[i] = i;
   }
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cell[a] < cell[b]; });
+  // within a cell by x, so that every row of cells (cz, cy) is sorted by x: the sweeps
+  // clip a row to an x interval with one ballot
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return cell[a] < cell[b] || (cell[a] == cell[b] && xs[a] < xs[b]);
+  });
   std::vector<int> cstart(ncell + 1, 0);
   for (int i = 0; i < n; ++i) cstart[cell[i] + 1]++;
   for (size_t k = 0; k < ncell; ++k) cstart[k + 1] += cstart[k];
+  // the pose sweeps' grid: the same rows, cells cell_xsub times narrower along x; the
+  // atom order (by row, then x) is sorted for both grids
+  const double csx = cs / c->cell_xsub;
+  const int gnxp = (int)std::floor((hi[0] - fr.ox - gx0) / csx) + 1;
+  const size_t ncellp = (size_t)gnxp * gny * gnz;
+  std::vector<int> cstartp(ncellp + 1, 0);
+  size_t prev = 0;
+  for (int k = 0; k < n; ++k) {
+    const int i = order[k];
+    const int cxp = std::min(gnxp - 1, std::max(0, (int)std::floor((xs[i] - fr.ox - gx0) / csx)));
+    const size_t cp = (size_t)(cell[i] / gnx) * gnxp + cxp;
+    if (cp < prev) return fail(c, MMO_ERR_DEVICE, "receptor cell order inconsistent along x");
+    prev = cp;
+    cstartp[cp + 1]++;
+  }
+  for (size_t k = 0; k < ncellp; ++k) cstartp[k + 1] += cstartp[k];
   std::vector<RecPos> pos(n);
   std::vector<RecAB> ab(n);
   for (int k = 0; k < n; ++k) {
@@ -533,16 +558,25 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   if ((r = upload(c, c->rpos, pos.data(), pos.size(), c->stream))) return r;
   if ((r = upload(c, c->rab, ab.data(), ab.size(), c->stream))) return r;
   if ((r = upload(c, c->d_cell, cstart.data(), cstart.size(), c->stream))) return r;
+  if ((r = upload(c, c->d_cellp, cstartp.data(), cstartp.size(), c->stream))) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   CellGrid cg{};
   cg.cell_start = c->d_cell.p;
   cg.x0 = (float)gx0; cg.y0 = (float)gy0; cg.z0 = (float)gz0;
   cg.c = (float)cs; cg.inv_c = (float)(1.0 / cs);
+  cg.cx = cg.c; cg.inv_cx = cg.inv_c;
   cg.nx = gnx; cg.ny = gny; cg.nz = gnz;
+  cg.clip_min = c->row_clip;
   c->cg = cg;
+  CellGrid cgp = cg;
+  cgp.cell_start = c->d_cellp.p;
+  cgp.cx = (float)csx; cgp.inv_cx = (float)(1.0 / csx);
+  cgp.nx = gnxp;
+  c->cgp = cgp;
   // the culled kernel keeps the whole receptor and the cell table in LDS (160 KiB/CU)
   const size_t lds = (size_t)n * (sizeof(RecPos) + sizeof(RecAB)) + (ncell + 1) * sizeof(int);
   c->cull_ok = lds <= 150 * 1024;
+  c->cullp_ok = (size_t)n * (sizeof(RecPos) + sizeof(RecAB)) + (ncellp + 1) * sizeof(int) <= 150 * 1024;
   c->fr = fr;
   c->fp_bits = k;
   c->nrec = n;
@@ -912,8 +946,9 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
   if ((inter & MMO_F_INTER_SHIFTED) && c->pose_cull) {
     // one wavefront per pose, receptor culled per 8-atom sub-block
     InterLaunch L = inter_launch(c, MMO_SHIFTED, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+    L.cg = c->cgp;
     // the receptor goes to LDS when enough poses per workgroup amortise the copy
-    L.cull = (c->cull_ok && b->npose >= 1024) ? 1 : 2;
+    L.cull = (c->cullp_ok && b->npose >= 1024) ? 1 : 2;
     PoseCullArgs a{};
     a.xyz = xyz; a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
     a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
@@ -1117,7 +1152,8 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     timer_end(c, "interp", s);
   } else {
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
-    L.cull = (mode == MMO_SHIFTED && c->scan_cull) ? (c->cull_ok ? 1 : 2) : 0;
+    // scan_cull 1: receptor staged in LDS when it fits; 2: read through the scalar cache
+    L.cull = (mode == MMO_SHIFTED && c->scan_cull) ? ((c->cull_ok && c->scan_cull == 1) ? 1 : 2) : 0;
     timer_begin(c, "inter", s);
     HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "inter", s);
@@ -1311,7 +1347,7 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.roi_x = m->roi[0]; a.roi_y = m->roi[1]; a.roi_z = m->roi[2]; a.roi_r = m->roi_r;
   a.inter_mode = m->inter_mode; a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p;
   a.nrec = c->nrec; a.sweep = c->mc_sweep;
-  a.cg = c->cg; a.g = c->g;
+  a.cg = c->cgp; a.g = c->g;
   return a;
 }
 
@@ -1322,7 +1358,7 @@ static hipError_t mc_launch(mmo_mc* m, const McArgs& a, int nwork, hipStream_t s
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t cap = 160 * 1024;
-  bool rec_lds = m->inter_mode != MMO_INTERP && m->ctx->cull_ok;
+  bool rec_lds = m->inter_mode != MMO_INTERP && m->ctx->cullp_ok;
   int waves = 8;
   while (waves > 1 && mc_lds_bytes(a, waves, rec_lds) > cap) --waves;
   if (rec_lds && (waves < 4 || mc_lds_bytes(a, waves, rec_lds) > cap)) {
@@ -1785,12 +1821,14 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "inter_rtile" && iv >= 64 && iv <= 6144) c->inter_rtile = iv;
   else if (k == "cull_k" && pow2_in(iv, 1, 4)) c->cull_k = iv;
   else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
-  else if (k == "scan_cull" && (iv == 0 || iv == 1)) c->scan_cull = iv;
+  else if (k == "scan_cull" && iv >= 0 && iv <= 2) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
   else if (k == "pose_sweep" && (iv == 0 || iv == 1)) c->pose_sweep = iv;
   else if (k == "mc_sweep" && (iv == 0 || iv == 1)) c->mc_sweep = iv;
+  else if (k == "row_clip" && iv >= 0 && iv <= (1 << 20)) c->row_clip = c->cg.clip_min = iv;
+  else if (k == "cell_xsub" && iv >= 1 && iv <= 16) c->cell_xsub = iv;
   else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
   return MMO_OK;
 }
@@ -1810,6 +1848,8 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "pose_cull") return c->pose_cull;
   if (k == "pose_sweep") return c->pose_sweep;
   if (k == "mc_sweep") return c->mc_sweep;
+  if (k == "row_clip") return c->row_clip;
+  if (k == "cell_xsub") return c->cell_xsub;
   return -1.0;
 }
 

@@ -56,8 +56,10 @@ struct Frame {
 struct CellGrid {
   const int* cell_start;   // ncell + 1
   float x0, y0, z0;        // corner of cell (0,0,0)
-  float c, inv_c;          // cell edge
+  float c, inv_c;          // cell edge along y and z
+  float cx, inv_cx;        // cell edge along x (c / n for the pose sweeps' grid)
   int nx, ny, nz;
+  int clip_min;            // rows longer than this are clipped to the sphere's x chord
 };
 
 // A pose tile: npose poses of ligand `lig` starting at pose0 (poses of one ligand are

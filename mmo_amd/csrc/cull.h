@@ -16,17 +16,18 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, in
   const float r2c = fmaxf(r2, kMinR2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = ir * ir;
-  const float u3 = u * u * u;
-  const float u6 = u3 * u3;
+  const float u3 = u * (u * u);
   if (MODE == MMO_SHIFTED) {
+    // w u^3 serves both LJ terms: 9 multiplies and FMAs after u^3 instead of 10
     const float t = shift_t(r2c);
     const float w = t * t;
-    sq = fmaf(w * p.q, ir, sq);
-    s12 = fmaf(w * ab.a, u6, s12);
-    s6 = fmaf(w * ab.b, u3, s6);
+    const float v = w * u3;
+    sq = fmaf(p.q, w * ir, sq);
+    s6 = fmaf(ab.b, v, s6);
+    s12 = fmaf(ab.a, v * u3, s12);
   } else {
     sq = fmaf(p.q, ir, sq);
-    s12 = fmaf(ab.a, u6, s12);
+    s12 = fmaf(ab.a, u3 * u3, s12);
     s6 = fmaf(ab.b, u3, s6);
   }
 }
@@ -81,12 +82,15 @@ __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfir
 
 // Visits every receptor atom within R of (qx, qy, qz) (plus some beyond it) through
 // the LDS cell list, accumulating the receptor-side sums for K ligand atoms per lane;
-// fold(sq, s12, s6) is called once per z slab of cells (FP32 partial sums).
+// fold(sq, s12, s6) is called once per z slab of cells (FP32 partial sums).  A row of
+// cells holds its atoms sorted by x, so a row longer than cg.clip_min atoms is clipped
+// to the chord [qx - rx, qx + rx] of the sphere: one LDS read per lane of the row's first
+// (and, past 64 atoms, last) 64 atoms and a ballot give the atoms below and above it.
 template <int K, class Fold>
 __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
-                                           const RecAB* __restrict__ s_ab, float inv_s2,
-                                           float qx, float qy, float qz, float R,
+                                           const RecAB* __restrict__ s_ab, float inv_s,
+                                           float inv_s2, float qx, float qy, float qz, float R,
                                            const int (&lx)[K], const int (&ly)[K],
                                            const int (&lz)[K], Fold&& fold) {
   const float R2 = R * R;
@@ -109,13 +113,30 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
       const float rx2 = rz2 - dy * dy;
       if (rx2 < 0.0f) continue;
       const float rx = sqrtf(rx2);
-      const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
-      const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+      const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+      const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
       const int rowb = (cz * cg.ny + cy) * cg.nx;
-      const int a0 = uniformi(s_cell[rowb + cx0]);
-      const int a1 = uniformi(s_cell[rowb + cx1 + 1]);
-#pragma unroll 2
+      int a0 = uniformi(s_cell[rowb + cx0]);
+      int a1 = uniformi(s_cell[rowb + cx1 + 1]);
+      const int len = a1 - a0;
+      if (len > cg.clip_min) {
+        const int lane = threadIdx.x & (kWave - 1);
+        const float xlo = qx - rx, xhi = qx + rx;
+        const bool in0 = lane < len;
+        const float x0 = in0 ? (float)s_pos[a0 + lane].x * inv_s : 0.0f;
+        const int below = __popcll(__ballot(in0 && x0 < xlo));
+        int upto;   // atoms with x <= xhi: a prefix of the row
+        if (len <= kWave) {
+          upto = a0 + __popcll(__ballot(in0 && x0 <= xhi));
+        } else {
+          const float x1 = (float)s_pos[a1 - kWave + lane].x * inv_s;
+          upto = a1 - kWave + __popcll(__ballot(x1 <= xhi));
+        }
+        a0 = uniformi(a0 + below);
+        a1 = uniformi(upto);
+      }
+#pragma unroll 4
       for (int i = a0; i < a1; ++i) {
         const RecPos p = s_pos[i];
         const RecAB ab = s_ab[i];
@@ -157,8 +178,8 @@ __device__ __forceinline__ double cull_sweep_f64(const CellGrid& cg, const int* 
       const float rx2 = rz2 - dy * dy;
       if (rx2 < 0.0f) continue;
       const float rx = sqrtf(rx2);
-      const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
-      const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+      const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+      const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
       const int rowb = (cz * cg.ny + cy) * cg.nx;
       const int a0 = uniformi(s_cell[rowb + cx0]);
@@ -301,8 +322,8 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
         const float rx2 = R2 - dzb * dzb - dyb * dyb;
         if (rx2 >= 0.0f) {
           const float rx = sqrtf(rx2);
-          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
-          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
           if (cx0 <= cx1) {
             const int rowb = (cz * cg.ny + cy) * cg.nx;
             r0 = s_cell[rowb + cx0];
@@ -407,8 +428,8 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
         const float rx2 = R2 - dzb * dzb - dyb * dyb;
         if (rx2 >= 0.0f) {
           const float rx = sqrtf(rx2);
-          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_c));
-          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_c));
+          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
           if (cx0 <= cx1) {
             const int rowb = (cz * cg.ny + cy) * cg.nx;
             r0 = s_cell[rowb + cx0];

@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
     }
     float qx, qy, qz, R;
     if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
-    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
                   [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
   double Sq = 0.0, S12 = 0.0, S6 = 0.0;
   float qx, qy, qz, R;
   if (wave_query<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) {
-    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
                   [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
                     Sq += (double)sq[0];
                     S12 += (double)s12[0];

@@ -84,6 +84,37 @@ class MonteCarlo:
         split = lambda a: [a[self.chain_atom[k]:self.chain_atom[k + 1]] for k in range(self.nchain)]
         return split(conf), split(best)
 
+    def rescore(self):
+        """E_inter of every chain's best pose by the direct shifted sum, whatever form the
+        chains ran with: the reference's final-pose rescoring, whose ene_rescore is
+        Mol.ene_inter_UFF_shifted_bst (src/lds.ml:681-687, :1983).  One batched launch."""
+        from .ff import F_INTER_SHIFTED
+        conf = np.zeros((self.natoms, 3))
+        best = np.zeros((self.natoms, 3))
+        self.ctx._check(lib.mmo_mc_coords(self.h, _ptr(conf), _ptr(best), None))
+        e, _ = self.ctx.batch(self.chain_lig).score(best, F_INTER_SHIFTED)
+        return e[:, 0].copy()
+
+
+DETAILED_SCORES_HEADER = "#name\tE_inter\tRB\tHA\n"
+
+
+def detailed_scores_line(name, e_inter, num_rbonds, num_heavy_atoms):
+    """One line of the reference's detailed_scores.tsv (Lds.rescore, src/lds.ml:687; header
+    src/lds.ml:1806): name, exact E_inter (%g), rotatable bonds, heavy atoms."""
+    return "%s\t%g\t%d\t%d\n" % (name, e_inter, num_rbonds, num_heavy_atoms)
+
+
+def write_detailed_scores(fn, names, e_inter, ligs):
+    """detailed_scores.tsv for chains whose ligands are `ligs` (dicts or Mol records with
+    anum and rb_left): the header, then one line per chain."""
+    with open(fn, "w") as f:
+        f.write(DETAILED_SCORES_HEADER)
+        for name, e, lig in zip(names, e_inter, ligs):
+            anum = lig["anum"] if isinstance(lig, dict) else lig.anum
+            rb = lig["rb_left"] if isinstance(lig, dict) else lig.rb_left
+            f.write(detailed_scores_line(name, float(e), len(rb), int(np.sum(np.asarray(anum) > 1))))
+
 
 def philox4x32(counter, key):
     """Philox4x32-10 (Salmon et al., SC 2011) on the host, the same stream as the kernel:

@@ -690,6 +690,14 @@ def test_mc_interp_mode_matches_cpu_restatement():
                        inter_fn=inter)
         assert st["accepted"][k] == ref["accepted"]
         assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-9, abs=1e-9)
+    # final-pose rescoring: the direct shifted E_inter of each chain's best pose
+    e_rs = m.rescore()
+    _, best = m.coords()
+    for k, l in enumerate(chain_lig):
+        lg = dict(ligs[ids.index(l)])
+        lg["xs"], lg["ys"], lg["zs"] = (np.ascontiguousarray(best[k][:, a]) for a in range(3))
+        assert_energy(e_rs[k], oracle.ene_inter(rec, lg, shifted=True),
+                      inter_abs_sum(rec, best[k], lg), what="rescore chain %d" % k)
 
 
 # ------------------------------------------------------------------ L-BFGS minimiser

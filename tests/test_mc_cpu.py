@@ -79,3 +79,16 @@ def test_restatement_metropolis_limits(config0):
     assert all(b <= a for a, b in zip(e, e[1:]))
     r = simulate(lig, rec, np.eye(3), start, 40, seed=5, chain=0, hard_roi=(start, 50.0), beta=0.0)
     assert r["accepted"] == 40 and r["resets"] == 0
+
+
+def test_detailed_scores_lines(tmp_path):
+    """Lds.rescore's line (src/lds.ml:687: "%s\t%g\t%d\t%d\n") and header (src/lds.ml:1806);
+    heavy atoms are anum > 1 (Mol.num_heavy_atoms, src/mol.ml:67-68)."""
+    from mmo_amd import mc as mcmod
+    assert mcmod.detailed_scores_line("lig_1", -12.3456789, 3, 20) == "lig_1\t-12.3457\t3\t20\n"
+    assert mcmod.detailed_scores_line("x", 1.0e-5, 0, 1) == "x\t1e-05\t0\t1\n"
+    assert mcmod.detailed_scores_line("y", 123456789.0, 15, 70) == "y\t1.23457e+08\t15\t70\n"
+    lig = {"anum": np.array([6, 1, 8, 1, 0, 17]), "rb_left": [0, 2]}
+    fn = tmp_path / "detailed_scores.tsv"
+    mcmod.write_detailed_scores(str(fn), ["a"], [-1.5], [lig])
+    assert fn.read_text() == "#name\tE_inter\tRB\tHA\na\t-1.5\t2\t3\n"
