@@ -65,25 +65,33 @@ def cpu_threads():
     return max(1, min(n, 16))
 
 
-def cpu_baseline(lig, rec, rots, trans, target_s=12.0):
+def cpu_baseline(lig, rec, rots, trans, target_s=12.0, single_s=4.0):
     """FP64 C restatement of the reference loop (oracle, OpenMP over poses) on a random
-    sample of the same scan's poses."""
+    sample of the same scan's poses: all host threads (the reported value), and one
+    thread (the reference scores one pose at a time per process)."""
     import oracle
-    threads = cpu_threads()
     rng = np.random.default_rng(1)
-    n = 64
-    while True:
-        pr = rng.integers(0, len(rots), n).astype(np.int32)
-        pt = rng.integers(0, len(trans), n).astype(np.int32)
-        t0 = time.perf_counter()
-        oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= target_s or n >= 1 << 20:
-            break
-        n = int(n * min(16.0, max(2.0, 1.1 * target_s / max(dt, 1e-3))))
+
+    def run(threads, target):
+        n = 16
+        while True:
+            pr = rng.integers(0, len(rots), n).astype(np.int32)
+            pt = rng.integers(0, len(trans), n).astype(np.int32)
+            t0 = time.perf_counter()
+            oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=threads)
+            dt = time.perf_counter() - t0
+            if dt >= target or n >= 1 << 20:
+                return n, dt
+            n = int(n * min(16.0, max(2.0, 1.1 * target / max(dt, 1e-3))))
+
+    threads = cpu_threads()
+    n, dt = run(threads, target_s)
+    n1, dt1 = run(1, single_s)
     return {"value": n / dt, "unit": "poses/s", "cores": threads, "kind": "port",
             "sample": "%d random poses of the scan (shifted direct E_inter, FP64, OpenMP), %.1f s"
-                      % (n, dt)}
+                      % (n, dt),
+            "single_thread": {"value": n1 / dt1, "unit": "poses/s", "cores": 1,
+                              "sample": "%d random poses, %.1f s" % (n1, dt1)}}
 
 
 def measured_traffic(kernel):

@@ -1152,8 +1152,13 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     timer_end(c, "interp", s);
   } else {
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
-    // scan_cull 1: receptor staged in LDS when it fits; 2: read through the scalar cache
+    // scan_cull 1: receptor staged in LDS when it fits; 2: read through the scalar cache;
+    // 3: read through the scalar cache, rows cut along x by the pose sweeps' cell table
     L.cull = (mode == MMO_SHIFTED && c->scan_cull) ? ((c->cull_ok && c->scan_cull == 1) ? 1 : 2) : 0;
+    if (L.cull && c->scan_cull == 3) {
+      L.cg = c->cgp;
+      L.cg.clip_min = 1 << 30;
+    }
     timer_begin(c, "inter", s);
     HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "inter", s);
@@ -1821,7 +1826,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "inter_rtile" && iv >= 64 && iv <= 6144) c->inter_rtile = iv;
   else if (k == "cull_k" && pow2_in(iv, 1, 4)) c->cull_k = iv;
   else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
-  else if (k == "scan_cull" && iv >= 0 && iv <= 2) c->scan_cull = iv;
+  else if (k == "scan_cull" && iv >= 0 && iv <= 3) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;

@@ -101,4 +101,24 @@ __device__ __forceinline__ float shift_t(float r2c) {
   return __builtin_amdgcn_fmed3f(fmaf(r2c, -1.0f / 144.0f, 1.0f), 0.0f, 1.0f);
 }
 
+// Pair-loop constants kept in VGPRs.  On gfx950 a VALU instruction that reads an SGPR
+// issues at a quarter of the rate of an all-VGPR one and a literal operand also costs
+// issue rate (measured with single-form loops); the pair loops are bound by VALU issue,
+// so the constants they use on every pair live in VGPRs.
+__device__ __forceinline__ float in_vgpr(float x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+struct PairK {
+  float inv_s2;   // 2^-2k: fixed-point r^2 to A^2
+  float min_r2;   // clamp r >= 0.01 A
+  float m144;     // -1/144: shift t = 1 - r^2/144
+};
+__device__ __forceinline__ PairK pair_k(float inv_s2) {
+  return {in_vgpr(inv_s2), in_vgpr(kMinR2), in_vgpr(-1.0f / 144.0f)};
+}
+__device__ __forceinline__ float shift_t(float r2c, const PairK& k) {
+  return __builtin_amdgcn_fmed3f(fmaf(r2c, k.m144, 1.0f), 0.0f, 1.0f);
+}
+
 }  // namespace mmo

@@ -9,17 +9,17 @@ namespace mmo {
 // --------------------------------------------------------------- pair arithmetic
 template <int MODE>
 __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                            int lz, float inv_s2, float& sq, float& s12,
+                                            int lz, const PairK& pk, float& sq, float& s12,
                                             float& s6) {
   const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
-  const float r2c = fmaxf(r2, kMinR2);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
+  const float r2c = fmaxf(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = ir * ir;
   const float u3 = u * (u * u);
   if (MODE == MMO_SHIFTED) {
     // w u^3 serves both LJ terms: 9 multiplies and FMAs after u^3 instead of 10
-    const float t = shift_t(r2c);
+    const float t = shift_t(r2c, pk);
     const float w = t * t;
     const float v = w * u3;
     sq = fmaf(p.q, w * ir, sq);
@@ -37,12 +37,12 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, in
 // F = ke qi qj / r + Aij u^6 - 2 Bij u^3, dF/dr2 = -u (ke qi qj / (2 r) + 6 Aij u^6 - 6 Bij u^3),
 // w = t^2 with t = 1 - r2/144, dw/dr2 = -t/72.
 __device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                                  int lz, float inv_s2, float c1, float aj,
+                                                  int lz, const PairK& pk, float c1, float aj,
                                                   float bj, float& e, float& gx, float& gy,
                                                   float& gz) {
   const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
-  const float r2c = fmaxf(r2, kMinR2);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
+  const float r2c = fmaxf(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = ir * ir;
   const float u3 = u * u * u;
@@ -50,10 +50,10 @@ __device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& 
   const float eq = p.q * ir, e12 = ab.a * u6, e6 = ab.b * u3;
   const float F = fmaf(c1, eq, fmaf(aj, e12, -2.0f * bj * e6));
   const float G = fmaf(0.5f * c1, eq, 6.0f * fmaf(aj, e12, -bj * e6));
-  const float t = shift_t(r2c);
+  const float t = shift_t(r2c, pk);
   const float w = t * t;
   float coef = fmaf(t * F, -1.0f / 72.0f, -(w * u) * G);
-  coef = (r2 >= kMinR2) ? coef : 0.0f;   // clamped distance: flat
+  coef = (r2 >= pk.min_r2) ? coef : 0.0f;   // clamped distance: flat
   e = fmaf(w, F, e);
   gx = fmaf(coef, dx, gx);
   gy = fmaf(coef, dy, gy);
@@ -93,6 +93,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
                                            float inv_s2, float qx, float qy, float qz, float R,
                                            const int (&lx)[K], const int (&ly)[K],
                                            const int (&lz)[K], Fold&& fold) {
+  const PairK pk = pair_k(inv_s2);
   const float R2 = R * R;
   const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
   const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
@@ -142,7 +143,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
         const RecAB ab = s_ab[i];
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          pair_energy<MMO_SHIFTED>(p, ab, lx[k], ly[k], lz[k], inv_s2, sq[k], s12[k], s6[k]);
+          pair_energy<MMO_SHIFTED>(p, ab, lx[k], ly[k], lz[k], pk, sq[k], s12[k], s6[k]);
       }
     }
     fold(sq, s12, s6);
@@ -289,6 +290,7 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
                                            const RecAB* __restrict__ s_ab, const Frame& fr,
                                            int n_atoms, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
+  const PairK pk = pair_k(fr.inv_s2);
   const int lane = threadIdx.x & (kWave - 1);
   const int aslot = lane & (kSub - 1), rslot = lane >> 3;
   for (int j0 = 0; j0 < n_atoms; j0 += kSub) {
@@ -340,9 +342,9 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
           if (GRAD)
-            pair_grad_shifted(p, ab, lx[0], ly[0], lz[0], fr.inv_s2, c1, fa, fb, fe, fx, fy, fz);
+            pair_grad_shifted(p, ab, lx[0], ly[0], lz[0], pk, c1, fa, fb, fe, fx, fy, fz);
           else
-            pair_energy<MMO_SHIFTED>(p, ab, lx[0], ly[0], lz[0], fr.inv_s2, sq, s12, s6);
+            pair_energy<MMO_SHIFTED>(p, ab, lx[0], ly[0], lz[0], pk, sq, s12, s6);
         }
       }
       if (GRAD) {
@@ -373,12 +375,13 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // cell rows that meet the 12 A sphere around atom j itself (no bounding-box growth):
 // lane k fetches the atom range of row k, the non-empty rows are compacted into LDS
 // with a wavefront prefix sum that lays their ranges end to end, and the concatenation
-// is walked 64 positions at a time.  Each lane finds the row of its position from a
-// ballot of the row ends falling in the current 64 positions (one LDS scatter + one
-// ballot per 64 pairs), so every lane works on every step.  wl: kAtomSweepInts ints of
-// this wavefront's LDS.  atom(j, lx, ly, lz, cq, ca, cb) gives atom j (wave-uniform);
-// with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from lane 0.  Sums: FP32
-// per atom and per 64 rows, FP64 across atoms.
+// (`total` pairs) is cut into 64 contiguous segments of ceil(total / 64) positions, one
+// per lane.  A lane finds the row of its first position by a 6-step search of the
+// compacted row starts and then walks its segment, stepping to the next row where one
+// ends, so the bookkeeping is one search per atom and a compare per pair.
+// wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, lx, ly, lz, cq, ca, cb)
+// gives atom j (wave-uniform); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1,
+// FP64) from lane 0.  Sums: FP32 per lane and atom, FP64 across atoms.
 constexpr int kAtomSweepInts = 3 * kWave;
 
 // LDS writes of this wavefront made visible to its own lanes
@@ -388,27 +391,33 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+__device__ __forceinline__ int in_vgpr_i(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 template <bool GRAD, class AtomFn, class GradOut>
 __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab, const Frame& fr,
                                            int n_atoms, int* wl, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
+  const PairK pk = pair_k(fr.inv_s2);
   const int lane = threadIdx.x & (kWave - 1);
   int* s_r0 = wl;                 // compacted rows: first receptor atom
-  int* s_beg = wl + kWave;        //                 first position in the concatenation
-  int* s_flag = wl + 2 * kWave;   // row ends inside the current 64 positions
+  int* s_beg = wl + kWave;        //                 first position; s_beg[nc] = total
   const uint64_t below = lane ? ((1ull << lane) - 1ull) : 0ull;
-  const uint64_t upto = (lane == kWave - 1) ? ~0ull : ((2ull << lane) - 1ull);
   constexpr float R = 12.05f;     // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
-  s_flag[lane] = 0;
   for (int j = 0; j < n_atoms; ++j) {
     int lx, ly, lz;
     double cq, ca, cb;
     atom(j, lx, ly, lz, cq, ca, cb);
     const float qx = uniformf((float)lx * fr.inv_s), qy = uniformf((float)ly * fr.inv_s),
                 qz = uniformf((float)lz * fr.inv_s);
+    // the pair loop reads the (wave-uniform) ligand atom from VGPRs: an SGPR operand
+    // would cost issue rate on every pair
+    const int vx = in_vgpr_i(lx), vy = in_vgpr_i(ly), vz = in_vgpr_i(lz);
     const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
     const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
     const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
@@ -446,37 +455,39 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
       const int total = uniformi(__shfl(incl, kWave - 1, kWave));
       if (total == 0) continue;
       const uint64_t nonempty = __ballot(len > 0);
+      const int nc = __popcll(nonempty);
       wave_lds_sync();            // the previous rows' readers are done
       if (len > 0) {
         const int c = __popcll(nonempty & below);
         s_r0[c] = r0;
         s_beg[c] = incl - len;
       }
-      int done = 0;               // compacted rows that end before the current positions
-      for (int t0 = 0; t0 < total; t0 += kWave) {
-        // rows ending inside [t0, t0 + 63] (ends of non-empty rows are distinct)
-        if (len > 0) {
-          const int b = incl - t0;
-          if (b >= 0 && b < kWave) s_flag[b] = 1;
-        }
-        wave_lds_sync();
-        const uint64_t ends = __ballot(s_flag[lane] != 0);
-        const int t = t0 + lane;
-        const int k = done + __popcll(ends & upto);   // row holding position t
-        const bool valid = t < total;
-        const int rk = s_r0[valid ? k : 0], bk = s_beg[valid ? k : 0];
-        wave_lds_sync();
-        s_flag[lane] = 0;
-        wave_lds_sync();
-        done += __popcll(ends);
-        if (valid) {
-          const int i = rk + (t - bk);
+      if (lane == 0) s_beg[nc] = total;
+      wave_lds_sync();
+      // this lane's segment [t, t_end) of the concatenation and the row holding t:
+      // the last compacted row k with s_beg[k] <= t (s_beg[0] = 0, rows non-empty)
+      const int m = (total + kWave - 1) / kWave;
+      int t = lane * m;
+      const int t_end = min(total, t + m);
+      int k = 0;
+      for (int step = kWave / 2; step > 0; step >>= 1)
+        if (k + step < nc && s_beg[k + step] <= t) k += step;
+      int i = s_r0[k] + (t - s_beg[k]);
+      int row_end = s_beg[k + 1];
+      for (int it = 0; it < m; ++it, ++t) {
+        if (t < t_end) {
+          if (t == row_end) {       // the next compacted row starts here
+            ++k;
+            i = s_r0[k];
+            row_end = s_beg[k + 1];
+          }
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
           if (GRAD)
-            pair_grad_shifted(p, ab, lx, ly, lz, fr.inv_s2, c1, fa, fb, fe, fx, fy, fz);
+            pair_grad_shifted(p, ab, vx, vy, vz, pk, c1, fa, fb, fe, fx, fy, fz);
           else
-            pair_energy<MMO_SHIFTED>(p, ab, lx, ly, lz, fr.inv_s2, sq, s12, s6);
+            pair_energy<MMO_SHIFTED>(p, ab, vx, vy, vz, pk, sq, s12, s6);
+          ++i;
         }
       }
     }

@@ -98,11 +98,11 @@ struct InterArgs {
 // fixed-point units (caller scales by -2 * 2^-k).
 template <int MODE>
 __device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                          int lz, float inv_s2, float c1, float aj, float bj,
+                                          int lz, const PairK& pk, float c1, float aj, float bj,
                                           float& e, float& gx, float& gy, float& gz) {
   const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * inv_s2;
-  const float r2c = fmaxf(r2, kMinR2);
+  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
+  const float r2c = fmaxf(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = ir * ir;
   const float u3 = u * u * u;
@@ -113,14 +113,14 @@ __device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, int 
   const float G = fmaf(0.5f * c1, eq, 6.0f * fmaf(aj, e12, -bj * e6));
   float w, coef;
   if (MODE == MMO_SHIFTED) {
-    const float t = shift_t(r2c);
+    const float t = shift_t(r2c, pk);
     w = t * t;
     coef = fmaf(t * F, -1.0f / 72.0f, -(w * u) * G);   // dw/dr2 = -t/72
   } else {
     w = 1.0f;
     coef = -u * G;
   }
-  coef = (r2 >= kMinR2) ? coef : 0.0f;   // clamped distance: flat
+  coef = (r2 >= pk.min_r2) ? coef : 0.0f;   // clamped distance: flat
   e = fmaf(w, F, e);
   gx = fmaf(coef, dx, gx);
   gy = fmaf(coef, dy, gy);
@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   const int per_lane = (li.n_atoms + split - 1) / split;
   const int nblk = (per_lane + K - 1) / K;
   const Frame fr = a.fr;
+  const PairK pk = pair_k(fr.inv_s2);
   double e_el = 0.0, e_vd = 0.0;
   int sat = 0;
   const int r_begin = blockIdx.y * a.rslice;
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
             const RecAB ab = s_ab[i];
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              pair_energy<MODE>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, sq[k], s12[k], s6[k]);
+              pair_energy<MODE>(p, ab, lx[k], ly[k], lz[k], pk, sq[k], s12[k], s6[k]);
           }
 #pragma unroll
           for (int k = 0; k < K; ++k) {
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
             const RecAB ab = s_ab[i];
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              pair_grad<MODE>(p, ab, lx[k], ly[k], lz[k], fr.inv_s2, c1[k], fa[k], fb[k], fe[k],
+              pair_grad<MODE>(p, ab, lx[k], ly[k], lz[k], pk, c1[k], fa[k], fb[k], fe[k],
                               fx[k], fy[k], fz[k]);
           }
 #pragma unroll
