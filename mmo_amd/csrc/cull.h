@@ -80,40 +80,45 @@ __device__ __forceinline__ float uniformf(float v) {
 }
 __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Visits every receptor atom within R of (qx, qy, qz) (plus some beyond it) through
-// the LDS cell list, accumulating the receptor-side sums for K ligand atoms per lane;
-// fold(sq, s12, s6) is called once per z slab of cells (FP32 partial sums).  A row of
-// cells holds its atoms sorted by x, so a row longer than cg.clip_min atoms is clipped
-// to the chord [qx - rx, qx + rx] of the sphere: one LDS read per lane of the row's first
-// (and, past 64 atoms, last) 64 atoms and a ballot give the atoms below and above it.
+// Visits every receptor atom within 12 A of the box [q - h, q + h] (plus some beyond it)
+// through the cell list, accumulating the receptor-side sums for K ligand atoms per
+// lane; fold(sq, s12, s6) is called once per z slab of cells (FP32 partial sums).  The
+// slabs, rows and row x ranges are those of the box grown by 12 A (the box's Minkowski
+// sum with the cutoff ball, ~5% smaller than the ball around the box centre of radius
+// 12 A + half-diagonal).  A row of cells holds its atoms sorted by x, so a row longer
+// than cg.clip_min atoms is clipped to that x range: one read per lane of the row's
+// first (and, past 64 atoms, last) 64 atoms and a ballot give the atoms below and above.
 template <int K, class Fold>
 __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab, float inv_s,
-                                           float inv_s2, float qx, float qy, float qz, float R,
+                                           float inv_s2, float qx, float qy, float qz,
+                                           float hx, float hy, float hz,
                                            const int (&lx)[K], const int (&ly)[K],
                                            const int (&lz)[K], Fold&& fold) {
   const PairK pk = pair_k(inv_s2);
-  const float R2 = R * R;
-  const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
-  const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
+  constexpr float R = 12.05f;   // cutoff + FP32 rounding margin
+  constexpr float R2 = R * R;
+  const int cz0 = max(0, (int)floorf((qz - hz - R - cg.z0) * cg.inv_c));
+  const int cz1 = min(cg.nz - 1, (int)floorf((qz + hz + R - cg.z0) * cg.inv_c));
   for (int cz = cz0; cz <= cz1; ++cz) {
+    // distances from the box [q - h, q + h] to the slab, then to the row
     const float zlo = cg.z0 + cz * cg.c;
-    const float dz = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
+    const float dz = fmaxf(0.0f, fmaxf(zlo - (qz + hz), (qz - hz) - (zlo + cg.c)));
     const float rz2 = R2 - dz * dz;
     if (rz2 < 0.0f) continue;
     const float ry = sqrtf(rz2);
-    const int cy0 = max(0, (int)floorf((qy - ry - cg.y0) * cg.inv_c));
-    const int cy1 = min(cg.ny - 1, (int)floorf((qy + ry - cg.y0) * cg.inv_c));
+    const int cy0 = max(0, (int)floorf((qy - hy - ry - cg.y0) * cg.inv_c));
+    const int cy1 = min(cg.ny - 1, (int)floorf((qy + hy + ry - cg.y0) * cg.inv_c));
     float sq[K], s12[K], s6[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
     for (int cy = cy0; cy <= cy1; ++cy) {
       const float ylo = cg.y0 + cy * cg.c;
-      const float dy = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
+      const float dy = fmaxf(0.0f, fmaxf(ylo - (qy + hy), (qy - hy) - (ylo + cg.c)));
       const float rx2 = rz2 - dy * dy;
       if (rx2 < 0.0f) continue;
-      const float rx = sqrtf(rx2);
+      const float rx = hx + sqrtf(rx2);
       const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
       const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
@@ -245,13 +250,13 @@ __device__ __forceinline__ void load_receptor(const RecPos* rpos, const RecAB* r
   s_cell = c;
 }
 
-// bounding box of the wavefront's valid fixed-point positions -> query centre and
-// radius 12 A + half-diagonal (+0.05 A for FP32 rounding), uniform across the wave
+// bounding box of the wavefront's valid fixed-point positions -> centre q and half
+// extents h (A), uniform across the wave; false when no lane holds a valid atom
 template <int K>
-__device__ __forceinline__ bool wave_query(const int (&lx)[K], const int (&ly)[K],
-                                           const int (&lz)[K], const bool (&valid)[K],
-                                           float inv_s, float& qx, float& qy, float& qz,
-                                           float& R) {
+__device__ __forceinline__ bool wave_box(const int (&lx)[K], const int (&ly)[K],
+                                         const int (&lz)[K], const bool (&valid)[K],
+                                         float inv_s, float& qx, float& qy, float& qz,
+                                         float& hx, float& hy, float& hz) {
   float mnx = 3.0e38f, mny = 3.0e38f, mnz = 3.0e38f;
   float mxx = -3.0e38f, mxy = -3.0e38f, mxz = -3.0e38f;
 #pragma unroll
@@ -266,7 +271,19 @@ __device__ __forceinline__ bool wave_query(const int (&lx)[K], const int (&ly)[K
   mxx = uniformf(wave_maxf(mxx)); mxy = uniformf(wave_maxf(mxy)); mxz = uniformf(wave_maxf(mxz));
   if (!(mnx <= mxx)) return false;
   qx = 0.5f * (mnx + mxx); qy = 0.5f * (mny + mxy); qz = 0.5f * (mnz + mxz);
-  const float hx = 0.5f * (mxx - mnx), hy = 0.5f * (mxy - mny), hz = 0.5f * (mxz - mnz);
+  hx = 0.5f * (mxx - mnx); hy = 0.5f * (mxy - mny); hz = 0.5f * (mxz - mnz);
+  return true;
+}
+
+// the same box as a sphere: centre q, radius 12 A + half-diagonal (+0.05 A for FP32
+// rounding), uniform across the wave
+template <int K>
+__device__ __forceinline__ bool wave_query(const int (&lx)[K], const int (&ly)[K],
+                                           const int (&lz)[K], const bool (&valid)[K],
+                                           float inv_s, float& qx, float& qy, float& qz,
+                                           float& R) {
+  float hx, hy, hz;
+  if (!wave_box<K>(lx, ly, lz, valid, inv_s, qx, qy, qz, hx, hy, hz)) return false;
   R = uniformf(12.0f + sqrtf(hx * hx + hy * hy + hz * hz) + 0.05f);
   return true;
 }

@@ -315,9 +315,9 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
         cq[k] = ca[k] = cb[k] = 0.0;
       }
     }
-    float qx, qy, qz, R;
-    if (!wave_query<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
-    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+    float qx, qy, qz, hx, hy, hz;
+    if (!wave_box<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) continue;
+    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, lx, ly, lz,
                   [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
@@ -460,9 +460,9 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
   ly[0] = valid[0] ? to_fixed(y, fr.oy, fr.scale, fr.lim, &sat) : 0;
   lz[0] = valid[0] ? to_fixed(z, fr.oz, fr.scale, fr.lim, &sat) : 0;
   double Sq = 0.0, S12 = 0.0, S6 = 0.0;
-  float qx, qy, qz, R;
-  if (wave_query<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) {
-    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, R, lx, ly, lz,
+  float qx, qy, qz, hx, hy, hz;
+  if (wave_box<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) {
+    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, lx, ly, lz,
                   [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
                     Sq += (double)sq[0];
                     S12 += (double)s12[0];

@@ -4,7 +4,10 @@ The library exposes its launch shapes as context settings (`Context.tune`):
 
   scan / culled batches   cull_k (ligand atoms per lane: 1, 2, 4), cull_waves
                           (wavefronts per workgroup, 1-16), cell_size (receptor cell
-                          edge in A; takes effect at the next set_receptor)
+                          edge in A; takes effect at the next set_receptor), scan_cull
+                          (1 receptor in LDS, 2 read through the scalar cache, 3 the
+                          same on the cells cut along x), cell_xsub (x cut of the pose
+                          and Monte Carlo sweeps' cells; next set_receptor)
   plain batches           inter_k (1, 2, 4, 8), inter_waves (1-4), inter_rtile (receptor
                           atoms per LDS tile)
   Monte Carlo             mc_waves_per_cu (resident chains per CU, 1-32)
@@ -72,10 +75,12 @@ def _time(fn, reps):
     return float(np.median(t))
 
 
-def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0, 6.0), ks=(1, 2, 4),
-              waves=(4, 8, 16), reps=2, store=True):
-    """Culled rigid scan: sweeps cell size x atoms per lane x wavefronts per workgroup on
-    a slice of the caller's scan (rots, trans) and leaves the best setting applied."""
+def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0), ks=(1,),
+              waves=(4, 8, 16), scan_culls=(1, 2, 3), reps=2, store=True):
+    """Culled rigid scan: sweeps cell size x receptor path (LDS or scalar cache) x atoms
+    per lane x wavefronts per workgroup on a slice of the caller's scan (rots, trans) and
+    leaves the best setting applied.  The pose sweeps' x cut (cell_xsub) stays as set:
+    it is the Monte Carlo's choice."""
     from . import ff
     import torch
     n = len(rots) * len(trans)
@@ -86,21 +91,22 @@ def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0, 6.0), ks=(1, 
     for cs in cell_sizes:
         ctx.tune(cell_size=cs)
         ctx.set_receptor(rec)
-        for k, w in itertools.product(ks, waves):
-            ctx.tune(cull_k=k, cull_waves=w)
+        for sc, k, w in itertools.product(scan_culls, ks, waves):
+            ctx.tune(scan_cull=sc, cull_k=k, cull_waves=w)
 
             def run():
                 ctx.scan(lig_id, ff.SHIFTED, rots, trans, out=out)
                 return ctx.last_kernel_ms("inter")
             ms = _time(run, reps)
             ok = bool(torch.allclose(out, ref, rtol=2e-6, atol=1e-6))
-            results.append({"cell_size": cs, "cull_k": k, "cull_waves": w, "ms": ms, "ok": ok})
+            results.append({"cell_size": cs, "scan_cull": sc, "cull_k": k, "cull_waves": w,
+                            "ms": ms, "ok": ok})
     good = [r for r in results if r["ok"]]
     best = min(good, key=lambda r: r["ms"])
     ctx.tune(cell_size=best["cell_size"])
     ctx.set_receptor(rec)
-    ctx.tune(cull_k=best["cull_k"], cull_waves=best["cull_waves"])
-    entry = {"best": {k: best[k] for k in ("cell_size", "cull_k", "cull_waves")},
+    ctx.tune(scan_cull=best["scan_cull"], cull_k=best["cull_k"], cull_waves=best["cull_waves"])
+    entry = {"best": {k: best[k] for k in ("cell_size", "scan_cull", "cull_k", "cull_waves")},
              "ms": best["ms"], "poses": n, "sweep": results}
     if store:
         _store(key_for("scan", lig_atoms=ctx.ligand_natoms(lig_id), rec_atoms=len(rec["xs"])), entry)
