@@ -451,6 +451,8 @@ def main():
     ctx = ff.Context(0)
     scan_key = tuner.key_for("scan", lig_atoms=len(lig["xs"]), rec_atoms=len(rec["xs"]))
     tuned = None if args.autotune else tuner.apply(ctx, scan_key)
+    if tuned is None and not args.autotune:
+        log("no recorded tuning for %s: library defaults" % scan_key)
     tune = dict(kv.split("=") for kv in args.tune.split(",") if kv)
     if "cell_size" in tune:
         ctx.tune(cell_size=float(tune.pop("cell_size")))
@@ -502,7 +504,8 @@ def main():
                      "traffic": measured_traffic(kernel + " scan"),
                      "kernel": kernel + " (scan)", "kernel_ms": avg_ms,
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
-        "tuning": {"applied": tuned, "cell_size": ctx.tuning("cell_size"),
+        "tuning": {"key": scan_key, "applied": tuned, "cell_size": ctx.tuning("cell_size"),
+                   "scan_cull": ctx.tuning("scan_cull"),
                    "cull_k": ctx.tuning("cull_k"), "cull_waves": ctx.tuning("cull_waves")},
     }
     want = set(x for x in args.extras.split(",") if x) if not args.only_scan else set()

@@ -298,8 +298,10 @@ double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
 /* Fixed-point scale exponent k (coordinates are stored as round(x * 2^k)). */
 int mmo_fixed_point_bits(const mmo_ctx* ctx);
 /* Kernel shape parameters, set by the autotuner (mmo_amd/tune.py): "inter_k",
- * "inter_waves", "inter_rtile", "cull_k", "cull_waves", "scan_cull", "cell_size"
- * (applies to the next mmo_set_receptor), "mc_waves_per_cu", "pose_cull" (1: batches in
+ * "inter_waves", "inter_rtile", "cull_k", "cull_waves" (rigid scan), "scan_cull" (rigid
+ * scan receptor path: 1 LDS, 2 scalar cache, 3 scalar cache on x-cut cells, 4 positions
+ * in LDS), "cell_size" (applies to the next mmo_set_receptor), "mc_waves_per_cu",
+ * "pose_waves" (batches: wavefronts per workgroup), "pose_cull" (1: batches in
  * the shifted form run one wavefront per pose with a culled receptor sweep; 0: tiles of
  * poses over the whole receptor), "pose_sweep" (that sweep: 1 lanes over the receptor
  * atoms near each ligand atom, 0 per 8-atom sub-block), "mc_sweep" (the same choice for

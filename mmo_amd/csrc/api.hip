@@ -83,7 +83,7 @@ struct mmo_ctx {
   // receptor cell list (culled shifted kernel)
   float cell_size = 5.0f;
   int row_clip = 4;           // scan/grid sweeps: clip rows longer than this to the x chord
-  int cell_xsub = 2;          // pose/MC sweeps: cells cut n-fold along x (rows clipped finer)
+  int cell_xsub = 4;          // pose/MC sweeps: cells cut n-fold along x (rows clipped finer)
   int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
   int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
   int mc_sweep = 1;           // the same choice for the Monte Carlo direct sum
@@ -91,7 +91,8 @@ struct mmo_ctx {
   int scan_cull = 1;
   int cull_k = 1;
   int mc_waves_per_cu = 8;
-  int cull_waves = 16;
+  int cull_waves = 16;         // rigid scan: wavefronts per workgroup
+  int pose_waves = 16;         // batches (one wavefront per pose): wavefronts per workgroup
   CellGrid cg{};              // scan and grid kernels: cubic cells
   DevBuf<int> d_cell;
   CellGrid cgp{};             // pose and Monte Carlo sweeps: cells cut along x, same atom order
@@ -953,7 +954,7 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     a.xyz = xyz; a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
     a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
     const bool comp = (flags & MMO_F_COMPONENTS) != 0;
-    const int waves = std::max(1, std::min(c->cull_waves, (b->npose + 255) / 256));
+    const int waves = std::max(1, std::min(c->pose_waves, (b->npose + 255) / 256));
     timer_begin(c, "inter", s);
     if (!want_grad || comp) {
       a.e_comp = comp ? e_out : nullptr;
@@ -1153,8 +1154,17 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
   } else {
     InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
     // scan_cull 1: receptor staged in LDS when it fits; 2: read through the scalar cache;
-    // 3: read through the scalar cache, rows cut along x by the pose sweeps' cell table
-    L.cull = (mode == MMO_SHIFTED && c->scan_cull) ? ((c->cull_ok && c->scan_cull == 1) ? 1 : 2) : 0;
+    // 3: read through the scalar cache, rows cut along x by the pose sweeps' cell table;
+    // 4: positions in LDS, LJ factors through the scalar cache
+    L.cull = 0;
+    if (mode == MMO_SHIFTED && c->scan_cull) {
+      L.cull = 2;
+      if (c->scan_cull == 1 && c->cull_ok) L.cull = 1;
+      if (c->scan_cull == 4 &&
+          (size_t)c->nrec * sizeof(RecPos) + (size_t)(c->cg.nx * c->cg.ny * c->cg.nz + 1) * sizeof(int) <=
+              150 * 1024)
+        L.cull = 3;
+    }
     if (L.cull && c->scan_cull == 3) {
       L.cg = c->cgp;
       L.cg.clip_min = 1 << 30;
@@ -1826,7 +1836,8 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "inter_rtile" && iv >= 64 && iv <= 6144) c->inter_rtile = iv;
   else if (k == "cull_k" && pow2_in(iv, 1, 4)) c->cull_k = iv;
   else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
-  else if (k == "scan_cull" && iv >= 0 && iv <= 3) c->scan_cull = iv;
+  else if (k == "pose_waves" && iv >= 1 && iv <= 16) c->pose_waves = iv;
+  else if (k == "scan_cull" && iv >= 0 && iv <= 4) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
@@ -1846,6 +1857,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "inter_rtile") return c->inter_rtile;
   if (k == "cull_k") return c->cull_k;
   if (k == "cull_waves") return c->cull_waves;
+  if (k == "pose_waves") return c->pose_waves;
   if (k == "scan_cull") return c->scan_cull;
   if (k == "cell_size") return c->cell_size;
   if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;

@@ -250,6 +250,30 @@ __device__ __forceinline__ void load_receptor(const RecPos* rpos, const RecAB* r
   s_cell = c;
 }
 
+// positions and charges (RecPos) + cell table into LDS, the LJ factors (RecAB) read in
+// place: a wavefront reads the same receptor atom on every lane, so RecAB comes through
+// the scalar cache while RecPos (the operands of the coordinate differences) comes from
+// LDS into VGPRs, keeping the per-pair SGPR operands to two
+__host__ __device__ __forceinline__ size_t receptor_pos_lds_bytes(int nrec, const CellGrid& cg) {
+  const size_t b = (size_t)nrec * sizeof(RecPos) + (size_t)(cg.nx * cg.ny * cg.nz + 1) * sizeof(int);
+  return (b + 15) & ~(size_t)15;
+}
+__device__ __forceinline__ void load_receptor_pos(const RecPos* __restrict__ rpos,
+                                                  const RecAB* __restrict__ rab, int nrec,
+                                                  const CellGrid& cg, const RecPos*& s_pos,
+                                                  const RecAB*& s_ab, const int*& s_cell,
+                                                  char* smem) {
+  const int ncell = cg.nx * cg.ny * cg.nz;
+  RecPos* p = reinterpret_cast<RecPos*>(smem);
+  int* c = reinterpret_cast<int*>(smem + (size_t)nrec * sizeof(RecPos));
+  for (int i = threadIdx.x; i < nrec; i += blockDim.x) p[i] = rpos[i];
+  for (int i = threadIdx.x; i <= ncell; i += blockDim.x) c[i] = cg.cell_start[i];
+  __syncthreads();
+  s_pos = p;
+  s_ab = rab;
+  s_cell = c;
+}
+
 // bounding box of the wavefront's valid fixed-point positions -> centre q and half
 // extents h (A), uniform across the wave; false when no lane holds a valid atom
 template <int K>

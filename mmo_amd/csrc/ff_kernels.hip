@@ -271,13 +271,18 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   }
 }
 
-template <int K, bool LDS, class Src>
+// RMODE: 0 receptor read in place (scalar cache), 1 receptor + cells in LDS, 2 positions
+// and cells in LDS with the LJ factors read in place
+template <int K, int RMODE, class Src>
 __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid cg) {
   extern __shared__ __align__(16) char smem[];
   const RecPos* s_pos;
   const RecAB* s_ab;
   const int* s_cell;
-  load_receptor<LDS>(a.rpos, a.rab, a.nrec, cg, s_pos, s_ab, s_cell, smem);
+  if constexpr (RMODE == 2)
+    load_receptor_pos(a.rpos, a.rab, a.nrec, cg, s_pos, s_ab, s_cell, smem);
+  else
+    load_receptor<RMODE == 1>(a.rpos, a.rab, a.nrec, cg, s_pos, s_ab, s_cell, smem);
 
   const int lane = threadIdx.x & (kWave - 1);
   const int tidx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
@@ -704,24 +709,27 @@ static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hip
   const int waves = L.cull_waves;
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
-  const bool in_lds = L.cull == 1;   // 2: receptor read from global memory
-  const size_t lds = in_lds ? (size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
-                                  (size_t)(L.cg.nx * L.cg.ny * L.cg.nz + 1) * sizeof(int)
-                            : 0;
+  // L.cull: 1 receptor in LDS, 2 read in place, 3 positions in LDS + LJ factors in place
+  const int rmode = L.cull == 1 ? 1 : (L.cull == 3 ? 2 : 0);
+  const size_t lds = rmode == 1 ? receptor_lds_bytes<true>(a.nrec, L.cg)
+                   : rmode == 2 ? receptor_pos_lds_bytes(a.nrec, L.cg) : 0;
   // dynamic LDS above 64 KiB must be allowed per kernel (gfx950 has 160 KiB per CU)
-#define MMO_CULL(KK)                                                                           \
-  case KK: {                                                                                   \
-    if (in_lds) {                                                                              \
+#define MMO_CULL_MODE(KK, RM)                                                                  \
+  {                                                                                            \
+    if (lds > 0) {                                                                             \
       hipError_t e = hipFuncSetAttribute(                                                      \
-          reinterpret_cast<const void*>(&k_inter_cull<KK, true, Src>),                         \
+          reinterpret_cast<const void*>(&k_inter_cull<KK, RM, Src>),                           \
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                               \
       if (e != hipSuccess) return e;                                                           \
-      hipLaunchKernelGGL((k_inter_cull<KK, true, Src>), dim3(blocks), dim3(waves * kWave),     \
-                         lds, s, a, L.cg);                                                     \
-    } else {                                                                                   \
-      hipLaunchKernelGGL((k_inter_cull<KK, false, Src>), dim3(blocks), dim3(waves * kWave),    \
-                         0, s, a, L.cg);                                                       \
     }                                                                                          \
+    hipLaunchKernelGGL((k_inter_cull<KK, RM, Src>), dim3(blocks), dim3(waves * kWave), lds, s, \
+                       a, L.cg);                                                               \
+  }
+#define MMO_CULL(KK)                                                                           \
+  case KK: {                                                                                   \
+    if (rmode == 1) MMO_CULL_MODE(KK, 1)                                                       \
+    else if (rmode == 2) MMO_CULL_MODE(KK, 2)                                                  \
+    else MMO_CULL_MODE(KK, 0)                                                                  \
     break;                                                                                     \
   }
   switch (L.cull_k) {
@@ -731,6 +739,7 @@ static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hip
     default: return hipErrorInvalidValue;
   }
 #undef MMO_CULL
+#undef MMO_CULL_MODE
   return hipGetLastError();
 }
 

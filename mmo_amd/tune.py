@@ -10,6 +10,7 @@ The library exposes its launch shapes as context settings (`Context.tune`):
                           and Monte Carlo sweeps' cells; next set_receptor)
   plain batches           inter_k (1, 2, 4, 8), inter_waves (1-4), inter_rtile (receptor
                           atoms per LDS tile)
+  batches (pose culled)   pose_waves (wavefronts per workgroup, 1-16)
   Monte Carlo             mc_waves_per_cu (resident chains per CU, 1-32)
 
 Each sweep times the real kernel on the caller's workload (HIP-event kernel times read
@@ -29,8 +30,13 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 
 def _device_name():
+    """Architecture and CU count (the marketing name varies between driver setups)."""
     import torch
-    return torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    if not torch.cuda.is_available():
+        return "cpu"
+    p = torch.cuda.get_device_properties(0)
+    arch = (getattr(p, "gcnArchName", "") or p.name).split(":")[0]
+    return "%s-%dcu" % (arch, p.multi_processor_count)
 
 
 def _load():
