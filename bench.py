@@ -408,6 +408,7 @@ def bench_mc(ctx, ligs, ids, chains_per_lig, frames, per_lig_sample, autotune):
             "s_batched": tb, "s_per_ligand": tp,
             "per_ligand_sample": "%d ligands x %d chains" % (len(sub), chains_per_lig),
             "tuned": tuned, "mc_waves_per_cu": ctx.tuning("mc_waves_per_cu"),
+            "mc_wg_waves": ctx.tuning("mc_wg_waves"),
             "launch": "one persistent launch per hipGraph replay, %d replays of %d frames"
                       % (nrep, frames // nrep)}
 

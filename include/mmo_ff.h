@@ -307,8 +307,9 @@ int mmo_fixed_point_bits(const mmo_ctx* ctx);
  * atoms near each ligand atom, 0 per 8-atom sub-block), "mc_sweep" (the same choice for
  * the Monte Carlo direct sum), "row_clip" (scan and grid sweeps clip cell rows longer
  * than this many atoms to the cutoff sphere's x chord), "cell_xsub" (the pose and Monte
- * Carlo sweeps use cells this many times narrower along x; applies to the next
- * mmo_set_receptor).  No counterpart in the reference. */
+ * Carlo sweeps use cells up to this many times narrower along x, fewer when the receptor
+ * and that cell table would pass 96 KiB; applies to the next mmo_set_receptor),
+ * "mc_wg_waves" (Monte Carlo chains per workgroup).  No counterpart in the reference. */
 int mmo_tune_set(mmo_ctx* ctx, const char* key, double value);
 double mmo_tune_get(mmo_ctx* ctx, const char* key);
 

@@ -90,7 +90,8 @@ struct mmo_ctx {
   bool cull_ok = false;
   int scan_cull = 1;
   int cull_k = 1;
-  int mc_waves_per_cu = 8;
+  int mc_waves_per_cu = 16;
+  int mc_wg_waves = 16;       // Monte Carlo chains (wavefronts) per workgroup
   int cull_waves = 16;         // rigid scan: wavefronts per workgroup
   int pose_waves = 16;         // batches (one wavefront per pose): wavefronts per workgroup
   CellGrid cg{};              // scan and grid kernels: cubic cells
@@ -528,8 +529,15 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   for (size_t k = 0; k < ncell; ++k) cstart[k + 1] += cstart[k];
   // the pose sweeps' grid: the same rows, cells cell_xsub times narrower along x; the
   // atom order (by row, then x) is sorted for both grids
-  const double csx = cs / c->cell_xsub;
-  const int gnxp = (int)std::floor((hi[0] - fr.ox - gx0) / csx) + 1;
+  // the cut is reduced until the receptor and this table fit in 96 KiB of LDS, leaving
+  // room for the Monte Carlo workgroup's conformers and chain states
+  int xsub = c->cell_xsub;
+  auto ncell_x = [&](int n) { return (int)std::floor((hi[0] - fr.ox - gx0) / (cs / n)) + 1; };
+  while (xsub > 1 && (size_t)n * (sizeof(RecPos) + sizeof(RecAB)) +
+                             ((size_t)ncell_x(xsub) * gny * gnz + 1) * sizeof(int) > 96 * 1024)
+    --xsub;
+  const double csx = cs / xsub;
+  const int gnxp = ncell_x(xsub);
   const size_t ncellp = (size_t)gnxp * gny * gnz;
   std::vector<int> cstartp(ncellp + 1, 0);
   size_t prev = 0;
@@ -1374,14 +1382,16 @@ static hipError_t mc_launch(mmo_mc* m, const McArgs& a, int nwork, hipStream_t s
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t cap = 160 * 1024;
   bool rec_lds = m->inter_mode != MMO_INTERP && m->ctx->cullp_ok;
-  int waves = 8;
+  const int wmax = std::min(mc_max_waves(a), m->ctx->mc_wg_waves);
+  int waves = wmax;
   while (waves > 1 && mc_lds_bytes(a, waves, rec_lds) > cap) --waves;
-  if (rec_lds && (waves < 4 || mc_lds_bytes(a, waves, rec_lds) > cap)) {
+  if (rec_lds && (waves < std::min(4, wmax) || mc_lds_bytes(a, waves, rec_lds) > cap)) {
     rec_lds = false;
-    waves = 8;
+    waves = wmax;
     while (waves > 1 && mc_lds_bytes(a, waves, false) > cap) --waves;
   }
-  const int resident = (ncu * m->ctx->mc_waves_per_cu + waves - 1) / waves;
+  // at least one workgroup per CU; more when the residency setting allows
+  const int resident = ncu * std::max(1, m->ctx->mc_waves_per_cu / waves);
   const int blocks = std::max(1, std::min((nwork + waves - 1) / waves, resident));
   return launch_mc(a, blocks, waves, rec_lds, s);
 }
@@ -1840,6 +1850,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "scan_cull" && iv >= 0 && iv <= 4) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
+  else if (k == "mc_wg_waves" && iv >= 1 && iv <= 16) c->mc_wg_waves = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
   else if (k == "pose_sweep" && (iv == 0 || iv == 1)) c->pose_sweep = iv;
   else if (k == "mc_sweep" && (iv == 0 || iv == 1)) c->mc_sweep = iv;
@@ -1862,6 +1873,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "cell_size") return c->cell_size;
   if (k == "cull_ok") return c->cull_ok ? 1.0 : 0.0;
   if (k == "mc_waves_per_cu") return c->mc_waves_per_cu;
+  if (k == "mc_wg_waves") return c->mc_wg_waves;
   if (k == "pose_cull") return c->pose_cull;
   if (k == "pose_sweep") return c->pose_sweep;
   if (k == "mc_sweep") return c->mc_sweep;

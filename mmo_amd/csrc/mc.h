@@ -81,6 +81,8 @@ struct McArgs {
 
 // waves: chains per workgroup (one wavefront each); rec_lds: the workgroup first copies
 // the receptor and its cell table to LDS (they must fit next to the conformers)
+// wavefronts per workgroup the Monte Carlo kernel is compiled for (the launch bound)
+int mc_max_waves(const McArgs& a);
 hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s);
 size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds);
 hipError_t mc_prepare();

@@ -233,8 +233,14 @@ __device__ __forceinline__ void wave_sync() {
 // A workgroup of `waves` wavefronts, each running its own chains; with REC_LDS the
 // workgroup first stages the receptor and its cell table in LDS, shared by all chains.
 // one instantiation per energy form, so that each carries only its own registers
+// Wavefronts per workgroup each form is compiled for.  16 caps the direct forms at 128
+// VGPRs (4 wavefronts per SIMD; the chain's control code then spills ~276 B per lane to
+// scratch), measured 15.4 M against 14.2 M steps/s for 8 (202 VGPRs, 2 per SIMD); the
+// grid form keeps 8.
+__host__ __device__ constexpr int mc_form_waves(int inter) { return inter == kMcInterGrid ? 8 : 16; }
+
 template <bool REC_LDS, int INTER>
-__global__ void __launch_bounds__(512) k_mc(McArgs a, int rec_bytes) {
+__global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int rec_bytes) {
   extern __shared__ __align__(16) char mc_lds[];
   RecView rv;
   load_receptor<REC_LDS>(a.rpos, a.rab, a.nrec, a.cg, rv.pos, rv.ab, rv.cell, mc_lds);
@@ -518,7 +524,10 @@ int mc_inter_kind(const McArgs& a) {
   return a.sweep == 1 ? kMcInterAtom : kMcInterSub;
 }
 
+int mc_max_waves(const McArgs& a) { return mc_form_waves(mc_inter_kind(a)); }
+
 hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s) {
+  if (waves < 1 || waves > mc_max_waves(a)) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_mc_reset_queue, dim3(1), dim3(1), 0, s, a.queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -11,7 +11,8 @@ The library exposes its launch shapes as context settings (`Context.tune`):
   plain batches           inter_k (1, 2, 4, 8), inter_waves (1-4), inter_rtile (receptor
                           atoms per LDS tile)
   batches (pose culled)   pose_waves (wavefronts per workgroup, 1-16)
-  Monte Carlo             mc_waves_per_cu (resident chains per CU, 1-32)
+  Monte Carlo             mc_wg_waves (chains per workgroup, 1-16), mc_waves_per_cu
+                          (resident chains per CU, 1-32)
 
 Each sweep times the real kernel on the caller's workload (HIP-event kernel times read
 back through `Context.last_kernel_ms`, or wall time around a synchronised launch for
@@ -145,15 +146,17 @@ def tune_batch(ctx, batch, xyz, flags, ks=(1, 2, 4), waves=(1, 2, 4), rtiles=(10
     return entry
 
 
-def tune_mc(ctx, make_mc, frames=50, waves=(2, 4, 8, 16), store=True, key_sig=None):
-    """Monte Carlo: resident chains per CU.  make_mc() builds a fresh MonteCarlo of the
-    caller's workload; each candidate advances it by `frames` frames (same results for
-    every setting: chains depend only on their own random streams)."""
+def tune_mc(ctx, make_mc, frames=50, shapes=((8, 8), (8, 16), (12, 12), (16, 16), (16, 32)),
+            store=True, key_sig=None):
+    """Monte Carlo: chains (wavefronts) per workgroup x resident chains per CU.  make_mc()
+    builds a fresh MonteCarlo of the caller's workload; each candidate advances it by
+    `frames` frames (same results for every setting: chains depend only on their own
+    random streams)."""
     import torch
     results = []
     ref = None
-    for w in waves:
-        ctx.tune(mc_waves_per_cu=w)
+    for wg, w in shapes:
+        ctx.tune(mc_wg_waves=wg, mc_waves_per_cu=w)
         m = make_mc()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -164,13 +167,13 @@ def tune_mc(ctx, make_mc, frames=50, waves=(2, 4, 8, 16), store=True, key_sig=No
         if ref is None:
             ref = st["e"].copy()
         ok = bool(np.array_equal(st["e"], ref))
-        results.append({"mc_waves_per_cu": w, "s": dt, "ok": ok,
+        results.append({"mc_wg_waves": wg, "mc_waves_per_cu": w, "s": dt, "ok": ok,
                         "steps_per_s": float(st["frame"].sum()) / dt})
         del m
     best = min((r for r in results if r["ok"]), key=lambda r: r["s"])
-    ctx.tune(mc_waves_per_cu=best["mc_waves_per_cu"])
-    entry = {"best": {"mc_waves_per_cu": best["mc_waves_per_cu"]}, "s": best["s"],
-             "sweep": results}
+    ctx.tune(mc_wg_waves=best["mc_wg_waves"], mc_waves_per_cu=best["mc_waves_per_cu"])
+    entry = {"best": {"mc_wg_waves": best["mc_wg_waves"], "mc_waves_per_cu": best["mc_waves_per_cu"]},
+             "s": best["s"], "sweep": results}
     if store:
         _store(key_for("mc", **(key_sig or {})), entry)
     return entry
