@@ -26,6 +26,7 @@
  */
 #ifndef MMO_FF_H
 #define MMO_FF_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -295,6 +296,38 @@ void mmo_min_destroy(mmo_min* mz);
 /* Kernel time of the last launch of `kernel` ("inter", "interp", "intra", "grid") measured
  * with HIP events on the launch stream, in milliseconds (-1 if none). */
 double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
+/* ------------------------------------------------------------- ligand input
+ * Host-only (no GPU needed).  The reference prepares ligands with mol2pqrs and reads
+ * them with Pqrs/Mol (src/mol2.ml:99-323, src/mol_graph.ml:53-192, src/mol2pqrs.ml:9-43,
+ * src/pqrs.ml:17-90, src/mol.ml:420-440): a .mol2 file is parsed (lone pairs removed),
+ * its topology derived (BFS topological distances; rotatable bonds: single, both ends of
+ * degree > 1, not in a ring; rotation group: the smaller side), written as .pqrs text
+ * and read back, so .mol2 and .pqrs inputs give the same molecules.  A molset holds the
+ * molecules; the accessors copy them into caller buffers sized by mmo_molset_dims.
+ * Malformed input -> MMO_ERR_BAD_ARG with the reason in mmo_last_error(NULL). */
+typedef struct mmo_molset mmo_molset;
+int mmo_molset_from_mol2(const char* text, size_t len, mmo_molset** out);
+/* ligands != 0: ligand records (header n:nrb:name, rot-bond lines, distance matrix);
+ * 0: one receptor record (n:name, atom lines only; Mol.receptor_of_pqrs_file) */
+int mmo_molset_from_pqrs(const char* text, size_t len, int ligands, mmo_molset** out);
+int mmo_molset_count(const mmo_molset* s);
+const char* mmo_molset_name(const mmo_molset* s, int i);
+/* atoms, rotatable bonds (-1 for a receptor record), atoms of all rotation groups */
+int mmo_molset_dims(const mmo_molset* s, int i, int* n_atoms, int* n_rbonds, int* n_rg_atoms);
+/* coordinates, partial charges, vdW radii (NaN where the element has none), elements;
+ * any pointer may be NULL */
+int mmo_molset_atoms(const mmo_molset* s, int i, double* xs, double* ys, double* zs,
+                     double* q, double* r, int32_t* anum);
+/* Mol.t topology: dists[i + j n] (n x n), rotatable bonds as (left, right) with right on
+ * the moving side, rotation groups in CSR (rg_off: n_rbonds + 1; rg_atoms: the moving
+ * atoms without `right`); any pointer may be NULL; receptor records have none */
+int mmo_molset_topology(const mmo_molset* s, int i, int32_t* dists, int32_t* rb_left,
+                        int32_t* rb_right, int32_t* rg_off, int32_t* rg_atoms);
+/* the .pqrs text of the set (the converted text for .mol2 input, the input itself for
+ * .pqrs); *len receives its size; buf may be NULL to query the size */
+int mmo_molset_pqrs(const mmo_molset* s, char* buf, size_t cap, size_t* len);
+void mmo_molset_destroy(mmo_molset* s);
+
 /* Fixed-point scale exponent k (coordinates are stored as round(x * 2^k)). */
 int mmo_fixed_point_bits(const mmo_ctx* ctx);
 /* Kernel shape parameters, set by the autotuner (mmo_amd/tune.py): "inter_k",

@@ -70,6 +70,15 @@ SIGNATURES = {
     "mmo_fixed_point_bits": (_i, [_p]),
     "mmo_tune_set": (_i, [_p, ct.c_char_p, _d]),
     "mmo_tune_get": (_d, [_p, ct.c_char_p]),
+    "mmo_molset_from_mol2": (_i, [ct.c_char_p, ct.c_size_t, _pp]),
+    "mmo_molset_from_pqrs": (_i, [ct.c_char_p, ct.c_size_t, _i, _pp]),
+    "mmo_molset_count": (_i, [_p]),
+    "mmo_molset_name": (ct.c_char_p, [_p, _i]),
+    "mmo_molset_dims": (_i, [_p, _i, _p, _p, _p]),
+    "mmo_molset_atoms": (_i, [_p, _i, _p, _p, _p, _p, _p, _p]),
+    "mmo_molset_topology": (_i, [_p, _i, _p, _p, _p, _p, _p]),
+    "mmo_molset_pqrs": (_i, [_p, ct.c_char_p, ct.c_size_t, ct.POINTER(ct.c_size_t)]),
+    "mmo_molset_destroy": (None, [_p]),
 }
 
 

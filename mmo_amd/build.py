@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["api.hip", "ff_kernels.hip", "pose_kernels.hip", "mc_kernels.hip", "min_kernels.hip",
-           "select_kernels.hip"]
+           "select_kernels.hip", "molio.cpp"]
 LIB = os.path.join(HERE, "libmmo_ff.so")
 ARCH = os.environ.get("MMO_OFFLOAD_ARCH", "gfx950")
 
@@ -35,13 +35,13 @@ def hipcc():
 
 
 def build(verbose=False, jobs=None):
-    """Compiles every HIP source to an object and links libmmo_ff.so."""
+    """Compiles every source (HIP kernels, host C++) to an object and links libmmo_ff.so."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
         cmd = [hipcc()] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:

@@ -394,6 +394,14 @@ int plain_setup(mmo_ctx* c, const double* xs, const double* ys, const double* zs
 
 }  // namespace
 
+namespace mmo {
+// errors of the host-only entry points (ligand input), read back by mmo_last_error(NULL)
+int host_fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+}  // namespace mmo
+
 // =========================================================================== C ABI
 extern "C" {
 

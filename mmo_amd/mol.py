@@ -127,52 +127,56 @@ class Mol:
 
 
 # ------------------------------------------------------------------------- .pqrs I/O
-def _parse_header(line):
-    """src/pqrs.ml:180-192: 'N:R:name' (ligand) or 'N:name' (receptor)."""
-    parts = line.strip().split(":")
-    if len(parts) >= 3:
-        try:
-            return int(parts[0]), int(parts[1]), ":".join(parts[2:])
-        except ValueError:
-            pass
-    return int(parts[0]), -1, ":".join(parts[1:])
+# Parsing runs in the library (mmo_molset_*, C++): the .pqrs reader of src/pqrs.ml:17-90,
+# :204-250 and Mol.ligands_of_pqrs_file / receptor_of_pqrs_file (src/mol.ml:413-440).
+def molecules_of_text(text, kind, ligand=True):
+    """Mol records of .pqrs ('pqrs') or .mol2 ('mol2') text, parsed by the library.
+    Malformed input raises ValueError with the library's message."""
+    import ctypes as ct
+    from ._lib import lib
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ct.c_void_p()
+    if kind == "mol2":
+        rc = lib.mmo_molset_from_mol2(data, len(data), ct.byref(h))
+    else:
+        rc = lib.mmo_molset_from_pqrs(data, len(data), 1 if ligand else 0, ct.byref(h))
+    if rc:
+        raise ValueError((lib.mmo_last_error(None) or b"").decode())
+    try:
+        return [_molecule(lib, h, i) for i in range(lib.mmo_molset_count(h))], _pqrs_of(lib, h)
+    finally:
+        lib.mmo_molset_destroy(h)
 
 
-def _read_one(lines, pos, ligand):
-    n, nrb, name = _parse_header(lines[pos])
-    pos += 1
-    xs, ys, zs, q, r, an = [], [], [], [], [], []
-    for _ in range(n):
-        f = lines[pos].split()
-        pos += 1
-        xs.append(float(f[0])); ys.append(float(f[1])); zs.append(float(f[2]))
-        q.append(float(f[3])); r.append(float(f[4]))
-        if f[5] not in SYM2ANUM:
-            raise ValueError("unsupported element: %s" % f[5])   # src/ptable.ml:103-107
-        an.append(SYM2ANUM[f[5]])
-    if not ligand:
-        return Mol(name, xs, ys, zs, q, an, r_a=r), pos
-    rbonds, groups = [], []
-    for _ in range(max(nrb, 0)):
-        head, bits = lines[pos].split("=", 1)
-        pos += 1
-        beg, end = (int(v) for v in head.split("-"))
-        b = [int(v) for v in bits.split()]
-        if len(b) != n or b[beg] == b[end]:
-            raise ValueError("bad rot-bond line: %s" % lines[pos - 1])
-        # axis from the fixed atom to the moving one (src/pqrs.ml:214-218)
-        left, right = (beg, end) if b[end] else (end, beg)
-        rbonds.append((left, right))
-        groups.append([i for i in range(n) if b[i] and i != right])   # src/pqrs.ml:243-250
-    dm = np.zeros((n, n), dtype=np.int32)
-    for i in range(n):
-        row = [int(v) for v in lines[pos].split()]
-        pos += 1
-        if len(row) != n or row[i] != 0:
-            raise ValueError("bad distance matrix row %d of %s" % (i, name))
-        dm[i] = row
-    return Mol(name, xs, ys, zs, q, an, r_a=r, dists_a=dm.T.reshape(-1), rbonds_a=rbonds,
-               rgroups_a=groups), pos
+def _pqrs_of(lib, h):
+    import ctypes as ct
+    n = ct.c_size_t()
+    lib.mmo_molset_pqrs(h, None, 0, ct.byref(n))
+    buf = ct.create_string_buffer(max(1, n.value))
+    lib.mmo_molset_pqrs(h, buf, n.value, ct.byref(n))
+    return buf.raw[:n.value].decode()
+
+
+def _molecule(lib, h, i):
+    import ctypes as ct
+    na, nrb, nrg = ct.c_int(), ct.c_int(), ct.c_int()
+    lib.mmo_molset_dims(h, i, ct.byref(na), ct.byref(nrb), ct.byref(nrg))
+    n = na.value
+    xs, ys, zs, q, r = (np.zeros(n) for _ in range(5))
+    an = np.zeros(n, dtype=np.int32)
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)
+    lib.mmo_molset_atoms(h, i, p(xs), p(ys), p(zs), p(q), p(r), p(an))
+    name = lib.mmo_molset_name(h, i).decode()
+    if nrb.value < 0:                       # receptor record: no topology
+        return Mol(name, xs, ys, zs, q, an, r_a=r)
+    dists = np.zeros(n * n, dtype=np.int32)
+    rl, rr = np.zeros(nrb.value, dtype=np.int32), np.zeros(nrb.value, dtype=np.int32)
+    off = np.zeros(nrb.value + 1, dtype=np.int32)
+    ga = np.zeros(nrg.value, dtype=np.int32)
+    lib.mmo_molset_topology(h, i, p(dists), p(rl), p(rr), p(off), p(ga))
+    groups = [ga[off[k]:off[k + 1]] for k in range(nrb.value)]
+    return Mol(name, xs, ys, zs, q, an, r_a=r, dists_a=dists,
+               rbonds_a=list(zip(rl.tolist(), rr.tolist())), rgroups_a=groups)
 
 
 def ligands_of_pqrs_file(fn):
@@ -182,21 +186,16 @@ def ligands_of_pqrs_file(fn):
 
 
 def ligands_of_pqrs_text(text, fn="<text>"):
-    lines = [l for l in text.splitlines() if l.strip()]
-    out, pos, seen = [], 0, set()
-    while pos < len(lines):
-        m, pos = _read_one(lines, pos, True)
-        if m.name in seen:
-            raise ValueError("duplicate molecule name %s in %s" % (m.name, fn))
-        seen.add(m.name)
-        out.append(m)
-    return out
+    try:
+        return molecules_of_text(text, "pqrs", ligand=True)[0]
+    except ValueError as e:
+        raise ValueError("%s: %s" % (fn, e)) from None
 
 
 def receptor_of_pqrs_file(fn):
     """Mol.receptor_of_pqrs_file, src/mol.ml:413-417."""
-    lines = [l for l in open(fn).read().splitlines() if l.strip()]
-    return _read_one(lines, 0, False)[0]
+    with open(fn) as f:
+        return molecules_of_text(f.read(), "pqrs", ligand=False)[0][0]
 
 
 def write_pqrs(fn, mols, ligand=True):
