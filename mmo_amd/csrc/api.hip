@@ -80,6 +80,7 @@ struct mmo_ctx {
   int fp_bits = 24;
   DevBuf<RecPos> rpos;
   DevBuf<RecAB> rab;
+  DevBuf<RecRes> rres;
   // receptor cell list (culled shifted kernel)
   float cell_size = 5.0f;
   int row_clip = 4;           // scan/grid sweeps: clip rows longer than this to the x chord
@@ -313,6 +314,7 @@ InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAto
   L.fr = c->fr;
   L.rpos = c->rpos.p;
   L.rab = c->rab.p;
+  L.rres = c->rres.p;
   L.nrec = c->nrec;
   L.ligs = ligs;
   L.latoms = latoms;
@@ -446,7 +448,7 @@ void mmo_ctx_destroy(mmo_ctx* c) {
   if (getpid() == c->pid) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    c->rpos.release(); c->rab.release(); c->grid.release();
+    c->rpos.release(); c->rab.release(); c->rres.release(); c->grid.release();
     c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
     c->d_pair_ij.release(); c->d_pair_par.release();
     c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
@@ -560,6 +562,7 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   for (size_t k = 0; k < ncellp; ++k) cstartp[k + 1] += cstartp[k];
   std::vector<RecPos> pos(n);
   std::vector<RecAB> ab(n);
+  std::vector<RecRes> res(n);
   for (int k = 0; k < n; ++k) {
     const int i = order[k];
     double A, B;
@@ -569,11 +572,16 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
     pos[k].y = (int)std::llrint((ys[i] - fr.oy) * fr.scale);
     pos[k].z = (int)std::llrint((zs[i] - fr.oz) * fr.scale);
     pos[k].q = (float)q[i];
+    // quantisation residuals (A): the pair kernels add them for atoms closer than 0.5 A
+    res[k].x = (float)((xs[i] - fr.ox) - pos[k].x / fr.scale);
+    res[k].y = (float)((ys[i] - fr.oy) - pos[k].y / fr.scale);
+    res[k].z = (float)((zs[i] - fr.oz) - pos[k].z / fr.scale);
     ab[k].a = (float)A;
     ab[k].b = (float)B;
   }
   if ((r = upload(c, c->rpos, pos.data(), pos.size(), c->stream))) return r;
   if ((r = upload(c, c->rab, ab.data(), ab.size(), c->stream))) return r;
+  if ((r = upload(c, c->rres, res.data(), res.size(), c->stream))) return r;
   if ((r = upload(c, c->d_cell, cstart.data(), cstart.size(), c->stream))) return r;
   if ((r = upload(c, c->d_cellp, cstartp.data(), cstartp.size(), c->stream))) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -968,6 +976,7 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     L.cull = (c->cullp_ok && b->npose >= 1024) ? 1 : 2;
     PoseCullArgs a{};
     a.xyz = xyz; a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
+    a.rres = c->rres.p;
     a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
     const bool comp = (flags & MMO_F_COMPONENTS) != 0;
     const int waves = std::max(1, std::min(c->pose_waves, (b->npose + 255) / 256));
@@ -1217,7 +1226,7 @@ int mmo_grid_compute(mmo_ctx* c, const int32_t* probe_anum, const double* probe_
   if ((r = upload(c, c->s_ppar, probe.data(), probe.size(), s))) return r;
   HIPCHK(c, c->grid.ensure(nvox * ntypes));
   GridArgs a{};
-  a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p; a.nrec = c->nrec;
+  a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p; a.rres = c->rres.p; a.nrec = c->nrec;
   a.nx = nx; a.ny = ny; a.nz = nz; a.nvox = nvox;
   a.ox = origin[0]; a.oy = origin[1]; a.oz = origin[2]; a.step = step;
   if (roi_center) {
@@ -1377,6 +1386,7 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.max_rot0 = 15.0 * deg; a.max_trans0 = 0.15;
   a.roi_x = m->roi[0]; a.roi_y = m->roi[1]; a.roi_z = m->roi[2]; a.roi_r = m->roi_r;
   a.inter_mode = m->inter_mode; a.fr = c->fr; a.rpos = c->rpos.p; a.rab = c->rab.p;
+  a.rres = c->rres.p;
   a.nrec = c->nrec; a.sweep = c->mc_sweep;
   a.cg = c->cgp; a.g = c->g;
   return a;

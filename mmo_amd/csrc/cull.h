@@ -7,15 +7,17 @@
 namespace mmo {
 
 // --------------------------------------------------------------- pair arithmetic
-template <int MODE>
-__device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                            int lz, const PairK& pk, float& sq, float& s12,
+// NEAR = false: the caller knows the pair is at least 0.5 A apart (the residual check is
+// skipped, which keeps the loop branch-free)
+template <int MODE, bool NEAR = true>
+__device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, const LigPt& l,
+                                            int i, const PairK& pk, float& sq, float& s12,
                                             float& s6) {
-  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
-  const float r2c = fmaxf(r2, pk.min_r2);
+  const float dx = (float)(p.x - l.x), dy = (float)(p.y - l.y), dz = (float)(p.z - l.z);
+  const float r2 = NEAR ? pair_r2(dx, dy, dz, l, i, pk) : (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
+  const float r2c = max_r2(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
-  const float u = ir * ir;
+  const float u = inv_r2(ir, r2c);
   const float u3 = u * (u * u);
   if (MODE == MMO_SHIFTED) {
     // w u^3 serves both LJ terms: 9 multiplies and FMAs after u^3 instead of 10
@@ -36,15 +38,15 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, in
 // coef * (x_rec - x_lig) in fixed-point units (the caller scales by -2 * 2^-k).
 // F = ke qi qj / r + Aij u^6 - 2 Bij u^3, dF/dr2 = -u (ke qi qj / (2 r) + 6 Aij u^6 - 6 Bij u^3),
 // w = t^2 with t = 1 - r2/144, dw/dr2 = -t/72.
-__device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                                  int lz, const PairK& pk, float c1, float aj,
+__device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& ab, const LigPt& l,
+                                                  int i, const PairK& pk, float c1, float aj,
                                                   float bj, float& e, float& gx, float& gy,
                                                   float& gz) {
-  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
-  const float r2c = fmaxf(r2, pk.min_r2);
+  const float dx = (float)(p.x - l.x), dy = (float)(p.y - l.y), dz = (float)(p.z - l.z);
+  const float r2 = pair_r2(dx, dy, dz, l, i, pk);
+  const float r2c = max_r2(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
-  const float u = ir * ir;
+  const float u = inv_r2(ir, r2c);
   const float u3 = u * u * u;
   const float u6 = u3 * u3;
   const float eq = p.q * ir, e12 = ab.a * u6, e6 = ab.b * u3;
@@ -91,12 +93,12 @@ __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfir
 template <int K, class Fold>
 __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
-                                           const RecAB* __restrict__ s_ab, float inv_s,
+                                           const RecAB* __restrict__ s_ab,
+                                           const RecRes* __restrict__ rres, float inv_s,
                                            float inv_s2, float qx, float qy, float qz,
                                            float hx, float hy, float hz,
-                                           const int (&lx)[K], const int (&ly)[K],
-                                           const int (&lz)[K], Fold&& fold) {
-  const PairK pk = pair_k(inv_s2);
+                                           const LigPt (&l)[K], Fold&& fold) {
+  const PairK pk = pair_k(inv_s2, inv_s, rres);
   constexpr float R = 12.05f;   // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
   const int cz0 = max(0, (int)floorf((qz - hz - R - cg.z0) * cg.inv_c));
@@ -122,6 +124,9 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
       const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
       const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
+      // a row whose cells are 0.5 A or more from the box in y or z holds no atom closer
+      // than 0.5 A to any lane's atom: its pairs skip the near-pair residual check
+      const bool near_row = dz < 0.55f && dy < 0.55f;
       const int rowb = (cz * cg.ny + cy) * cg.nx;
       int a0 = uniformi(s_cell[rowb + cx0]);
       int a1 = uniformi(s_cell[rowb + cx1 + 1]);
@@ -142,13 +147,23 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
         a0 = uniformi(a0 + below);
         a1 = uniformi(upto);
       }
-#pragma unroll 4
-      for (int i = a0; i < a1; ++i) {
-        const RecPos p = s_pos[i];
-        const RecAB ab = s_ab[i];
+      if (near_row) {
+        for (int i = a0; i < a1; ++i) {
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          pair_energy<MMO_SHIFTED>(p, ab, lx[k], ly[k], lz[k], pk, sq[k], s12[k], s6[k]);
+          for (int k = 0; k < K; ++k)
+            pair_energy<MMO_SHIFTED, true>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
+        }
+      } else {
+#pragma unroll 4
+        for (int i = a0; i < a1; ++i) {
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            pair_energy<MMO_SHIFTED, false>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
+        }
       }
     }
     fold(sq, s12, s6);
@@ -299,6 +314,16 @@ __device__ __forceinline__ bool wave_box(const int (&lx)[K], const int (&ly)[K],
   return true;
 }
 
+template <int K>
+__device__ __forceinline__ bool wave_box(const LigPt (&l)[K], const bool (&valid)[K], float inv_s,
+                                         float& qx, float& qy, float& qz, float& hx, float& hy,
+                                         float& hz) {
+  int lx[K], ly[K], lz[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) { lx[k] = l[k].x; ly[k] = l[k].y; lz[k] = l[k].z; }
+  return wave_box<K>(lx, ly, lz, valid, inv_s, qx, qy, qz, hx, hy, hz);
+}
+
 // the same box as a sphere: centre q, radius 12 A + half-diagonal (+0.05 A for FP32
 // rounding), uniform across the wave
 template <int K>
@@ -320,28 +345,30 @@ __device__ __forceinline__ bool wave_query(const int (&lx)[K], const int (&ly)[K
 // inside the cutoff even though a whole pose spans up to 24 A.  The (z, y) cell rows of
 // the sphere's box are enumerated across the lanes, which fetch their atom ranges in
 // parallel (one memory latency per 64 rows, not one per row); the rows are then swept
-// in order, each lane reading the ranges back with readlane.  atom(j, lx, ly, lz, cq,
-// ca, cb) loads atom j; with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1 units,
+// in order, each lane reading the ranges back with readlane.  atom(j, l, cq, ca, cb)
+// loads atom j (LigPt); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1 units,
 // FP64) once per atom.  Sums: FP32 per 64 rows, FP64 across them.
 constexpr int kSub = 8;
 
 template <bool GRAD, class AtomFn, class GradOut>
 __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
-                                           const RecAB* __restrict__ s_ab, const Frame& fr,
+                                           const RecAB* __restrict__ s_ab,
+                                           const RecRes* __restrict__ rres, const Frame& fr,
                                            int n_atoms, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
-  const PairK pk = pair_k(fr.inv_s2);
+  const PairK pk = pair_k(fr.inv_s2, fr.inv_s, rres);
   const int lane = threadIdx.x & (kWave - 1);
   const int aslot = lane & (kSub - 1), rslot = lane >> 3;
   for (int j0 = 0; j0 < n_atoms; j0 += kSub) {
     const int j = j0 + aslot;
-    int lx[1], ly[1], lz[1];
+    LigPt l[1];
     bool valid[1];
     double cq = 0.0, ca = 0.0, cb = 0.0;
     valid[0] = j < n_atoms;
-    lx[0] = ly[0] = lz[0] = 0;
-    if (valid[0]) atom(j, lx[0], ly[0], lz[0], cq, ca, cb);
+    l[0] = lig_pt_zero();
+    if (valid[0]) atom(j, l[0], cq, ca, cb);
+    int lx[1] = {l[0].x}, ly[1] = {l[0].y}, lz[1] = {l[0].z};
     float qx, qy, qz, R;
     if (!wave_query<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, R)) continue;
     const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
@@ -383,9 +410,9 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
           if (GRAD)
-            pair_grad_shifted(p, ab, lx[0], ly[0], lz[0], pk, c1, fa, fb, fe, fx, fy, fz);
+            pair_grad_shifted(p, ab, l[0], i, pk, c1, fa, fb, fe, fx, fy, fz);
           else
-            pair_energy<MMO_SHIFTED>(p, ab, lx[0], ly[0], lz[0], pk, sq, s12, s6);
+            pair_energy<MMO_SHIFTED>(p, ab, l[0], i, pk, sq, s12, s6);
         }
       }
       if (GRAD) {
@@ -420,8 +447,9 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // per lane.  A lane finds the row of its first position by a 6-step search of the
 // compacted row starts and then walks its segment, stepping to the next row where one
 // ends, so the bookkeeping is one search per atom and a compare per pair.
-// wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, lx, ly, lz, cq, ca, cb)
-// gives atom j (wave-uniform); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1,
+// wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, l, cq, ca, cb) gives
+// atom j (LigPt, wave-uniform); NEAR = false skips the residual correction of pairs
+// closer than 0.5 A (energy form only); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1,
 // FP64) from lane 0.  Sums: FP32 per lane and atom, FP64 across atoms.
 constexpr int kAtomSweepInts = 3 * kWave;
 
@@ -437,13 +465,14 @@ __device__ __forceinline__ int in_vgpr_i(int x) {
   return x;
 }
 
-template <bool GRAD, class AtomFn, class GradOut>
+template <bool GRAD, class AtomFn, class GradOut, bool NEAR = true>
 __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
-                                           const RecAB* __restrict__ s_ab, const Frame& fr,
+                                           const RecAB* __restrict__ s_ab,
+                                           const RecRes* __restrict__ rres, const Frame& fr,
                                            int n_atoms, int* wl, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
-  const PairK pk = pair_k(fr.inv_s2);
+  const PairK pk = pair_k(fr.inv_s2, fr.inv_s, rres);
   const int lane = threadIdx.x & (kWave - 1);
   int* s_r0 = wl;                 // compacted rows: first receptor atom
   int* s_beg = wl + kWave;        //                 first position; s_beg[nc] = total
@@ -451,14 +480,14 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
   constexpr float R = 12.05f;     // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
   for (int j = 0; j < n_atoms; ++j) {
-    int lx, ly, lz;
+    LigPt lu;
     double cq, ca, cb;
-    atom(j, lx, ly, lz, cq, ca, cb);
-    const float qx = uniformf((float)lx * fr.inv_s), qy = uniformf((float)ly * fr.inv_s),
-                qz = uniformf((float)lz * fr.inv_s);
+    atom(j, lu, cq, ca, cb);
+    const float qx = uniformf((float)lu.x * fr.inv_s), qy = uniformf((float)lu.y * fr.inv_s),
+                qz = uniformf((float)lu.z * fr.inv_s);
     // the pair loop reads the (wave-uniform) ligand atom from VGPRs: an SGPR operand
     // would cost issue rate on every pair
-    const int vx = in_vgpr_i(lx), vy = in_vgpr_i(ly), vz = in_vgpr_i(lz);
+    const LigPt lv{in_vgpr_i(lu.x), in_vgpr_i(lu.y), in_vgpr_i(lu.z), lu.rx, lu.ry, lu.rz};
     const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
     const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
     const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
@@ -525,9 +554,9 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
           if (GRAD)
-            pair_grad_shifted(p, ab, vx, vy, vz, pk, c1, fa, fb, fe, fx, fy, fz);
+            pair_grad_shifted(p, ab, lv, i, pk, c1, fa, fb, fe, fx, fy, fz);
           else
-            pair_energy<MMO_SHIFTED>(p, ab, vx, vy, vz, pk, sq, s12, s6);
+            pair_energy<MMO_SHIFTED, NEAR>(p, ab, lv, i, pk, sq, s12, s6);
           ++i;
         }
       }

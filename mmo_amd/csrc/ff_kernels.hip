@@ -80,6 +80,7 @@ struct InterArgs {
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
+  const RecRes* rres;
   int nrec, rtile;
   const LigInfo* ligs;
   const LigAtom* latoms;
@@ -97,14 +98,14 @@ struct InterArgs {
 // energy + dE/dr2 contribution; gradient accumulated as coef * (x_rec - x_lig) in
 // fixed-point units (caller scales by -2 * 2^-k).
 template <int MODE>
-__device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, int lx, int ly,
-                                          int lz, const PairK& pk, float c1, float aj, float bj,
+__device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, const LigPt& l,
+                                          int i, const PairK& pk, float c1, float aj, float bj,
                                           float& e, float& gx, float& gy, float& gz) {
-  const float dx = (float)(p.x - lx), dy = (float)(p.y - ly), dz = (float)(p.z - lz);
-  const float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
-  const float r2c = fmaxf(r2, pk.min_r2);
+  const float dx = (float)(p.x - l.x), dy = (float)(p.y - l.y), dz = (float)(p.z - l.z);
+  const float r2 = pair_r2(dx, dy, dz, l, i, pk);
+  const float r2c = max_r2(r2, pk.min_r2);
   const float ir = __builtin_amdgcn_rsqf(r2c);
-  const float u = ir * ir;
+  const float u = inv_r2(ir, r2c);
   const float u3 = u * u * u;
   const float u6 = u3 * u3;
   const float eq = p.q * ir, e12 = ab.a * u6, e6 = ab.b * u3;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   const int per_lane = (li.n_atoms + split - 1) / split;
   const int nblk = (per_lane + K - 1) / K;
   const Frame fr = a.fr;
-  const PairK pk = pair_k(fr.inv_s2);
+  const PairK pk = pair_k(fr.inv_s2, fr.inv_s, a.rres);
   double e_el = 0.0, e_vd = 0.0;
   int sat = 0;
   const int r_begin = blockIdx.y * a.rslice;
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
     __syncthreads();
     if (!tile_ok) continue;
     for (int blk = 0; blk < nblk; ++blk) {
-      int lx[K], ly[K], lz[K];
+      LigPt l[K];
       double cq[K], ca[K], cb[K];
       float c1[K], fa[K], fb[K];
       int row[K];
@@ -175,14 +176,12 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
         if (pose_ok && j < li.n_atoms) {
           double x, y, z;
           a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
-          lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
-          ly[k] = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
-          lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+          l[k] = lig_pt(x, y, z, fr, &sat);
           const LigAtom la = a.latoms[li.atom_base + j];
           cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
           row[k] = GRAD ? a.src.row(pose, j) : 0;
         } else {
-          lx[k] = ly[k] = lz[k] = 0;
+          l[k] = lig_pt_zero();
           cq[k] = ca[k] = cb[k] = 0.0;
           row[k] = -1;
         }
@@ -200,7 +199,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
             const RecAB ab = s_ab[i];
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              pair_energy<MODE>(p, ab, lx[k], ly[k], lz[k], pk, sq[k], s12[k], s6[k]);
+              pair_energy<MODE>(p, ab, l[k], rb + i, pk, sq[k], s12[k], s6[k]);
           }
 #pragma unroll
           for (int k = 0; k < K; ++k) {
@@ -223,7 +222,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
             const RecAB ab = s_ab[i];
 #pragma unroll
             for (int k = 0; k < K; ++k)
-              pair_grad<MODE>(p, ab, lx[k], ly[k], lz[k], pk, c1[k], fa[k], fb[k], fe[k],
+              pair_grad<MODE>(p, ab, l[k], rb + i, pk, c1[k], fa[k], fb[k], fe[k],
                               fx[k], fy[k], fz[k]);
           }
 #pragma unroll
@@ -274,7 +273,8 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
 // RMODE: 0 receptor read in place (scalar cache), 1 receptor + cells in LDS, 2 positions
 // and cells in LDS with the LJ factors read in place
 template <int K, int RMODE, class Src>
-__global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid cg) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(K == 1 ? 8 : 1)))
+k_inter_cull(InterArgs<Src> a, CellGrid cg) {
   extern __shared__ __align__(16) char smem[];
   const RecPos* s_pos;
   const RecAB* s_ab;
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
   int sat = 0;
 
   for (int blk = 0; blk < nblk; ++blk) {
-    int lx[K], ly[K], lz[K];
+    LigPt l[K];
     bool valid[K];
     double cq[K], ca[K], cb[K];
 #pragma unroll
@@ -310,19 +310,17 @@ __global__ void __launch_bounds__(1024) k_inter_cull(InterArgs<Src> a, CellGrid 
       if (valid[k]) {
         double x, y, z;
         a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
-        lx[k] = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
-        ly[k] = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
-        lz[k] = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+        l[k] = lig_pt(x, y, z, fr, &sat);
         const LigAtom la = a.latoms[li.atom_base + j];
         cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
       } else {   // inert: zero parameters
-        lx[k] = ly[k] = lz[k] = 0;
+        l[k] = lig_pt_zero();
         cq[k] = ca[k] = cb[k] = 0.0;
       }
     }
     float qx, qy, qz, hx, hy, hz;
-    if (!wave_box<K>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) continue;
-    cull_sweep<K>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, lx, ly, lz,
+    if (!wave_box<K>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) continue;
+    cull_sweep<K>(cg, s_cell, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, l,
                   [&](const float (&sq)[K], const float (&s12)[K], const float (&s6)[K]) {
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
@@ -368,10 +366,8 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
   const double* x = a.xyz + 3 * row0;
   int sat = 0;
   double e_el = 0.0, e_vd = 0.0;
-  auto atom = [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
-    lx = to_fixed(x[3 * j], fr.ox, fr.scale, fr.lim, &sat);
-    ly = to_fixed(x[3 * j + 1], fr.oy, fr.scale, fr.lim, &sat);
-    lz = to_fixed(x[3 * j + 2], fr.oz, fr.scale, fr.lim, &sat);
+  auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
+    l = lig_pt(x[3 * j], x[3 * j + 1], x[3 * j + 2], fr, &sat);
     const LigAtom la = a.latoms[li.atom_base + j];
     cq = la.keq; ca = la.a; cb = la.b;
   };
@@ -380,9 +376,9 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
     g[0] += gx; g[1] += gy; g[2] += gz;
   };
   if (ATOMWISE)
-    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
+    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
   else
-    pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, gout, e_el, e_vd);
+    pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, gout, e_el, e_vd);
   (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
   // every lane of an atom slot holds part of the sums: the wavefront sum counts each
   // pair once (lanes of one slot share the atom, split the receptor atoms)
@@ -460,14 +456,12 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
   }
   const Frame fr = a.fr;
   int sat = 0;
-  int lx[1], ly[1], lz[1];
-  lx[0] = valid[0] ? to_fixed(x, fr.ox, fr.scale, fr.lim, &sat) : 0;
-  ly[0] = valid[0] ? to_fixed(y, fr.oy, fr.scale, fr.lim, &sat) : 0;
-  lz[0] = valid[0] ? to_fixed(z, fr.oz, fr.scale, fr.lim, &sat) : 0;
+  LigPt l[1];
+  l[0] = valid[0] ? lig_pt(x, y, z, fr, &sat) : lig_pt_zero();
   double Sq = 0.0, S12 = 0.0, S6 = 0.0;
   float qx, qy, qz, hx, hy, hz;
-  if (wave_box<1>(lx, ly, lz, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) {
-    cull_sweep<1>(cg, s_cell, s_pos, s_ab, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, lx, ly, lz,
+  if (wave_box<1>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) {
+    cull_sweep<1>(cg, s_cell, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, l,
                   [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
                     Sq += (double)sq[0];
                     S12 += (double)s12[0];
@@ -683,7 +677,7 @@ hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const in
                                double* part, hipStream_t s) {
   InterArgs<CoordSrc> a;
   a.src.xyz = xyz; a.src.atom_off = atom_off; a.src.tiles = tiles; a.src.perm = perm;
-  a.ntiles = ntiles; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab; a.nrec = L.nrec;
+  a.ntiles = ntiles; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab; a.rres = L.rres; a.nrec = L.nrec;
   a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms; a.lig_xyz = L.lig_xyz;
   a.e_stride = e_stride; a.grad = grad; a.write_e = write_e;
   a.rslice = L.rslice > 0 ? L.rslice : L.nrec;
@@ -749,7 +743,7 @@ hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, 
   InterArgs<ScanSrc> a;
   a.src.rots = rots; a.src.trans = trans; a.src.tperm = tperm; a.src.ntrans = ntrans;
   a.src.tiles_per_rot = (ntrans + kWave - 1) / kWave; a.src.lig = lig;
-  a.ntiles = nrot * a.src.tiles_per_rot; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab;
+  a.ntiles = nrot * a.src.tiles_per_rot; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab; a.rres = L.rres;
   a.nrec = L.nrec; a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms;
   a.lig_xyz = L.lig_xyz; a.e_out = e_out; a.e_stride = 1; a.e_comp = nullptr; a.grad = nullptr;
   a.write_e = 1; a.rslice = L.nrec; a.npose_total = nrot * ntrans;

@@ -19,6 +19,7 @@ struct InterLaunch {
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
+  const RecRes* rres;
   int nrec;
   const LigInfo* ligs;
   const LigAtom* latoms;
@@ -59,6 +60,7 @@ struct GridArgs {
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
+  const RecRes* rres;
   int nrec;
   int nx, ny, nz;
   size_t nvox;
@@ -120,6 +122,7 @@ struct PoseCullArgs {
   double* e_comp;           // nullable: [pose*e_stride + 2] elec, [+3] vdW
   double* grad;             // GRAD: dE/dxyz rows, added to what the buffer holds
   int write_e;
+  const RecRes* rres;       // receptor quantisation residuals
 };
 // atomwise: lanes over the receptor atoms near each ligand atom (atom_sweep) instead of
 // 8-atom sub-blocks (pose_sweep)

@@ -73,6 +73,7 @@ struct McArgs {
   Frame fr;
   const RecPos* rpos;
   const RecAB* rab;
+  const RecRes* rres;       // receptor quantisation residuals
   int nrec;
   int sweep;                 // 1: atom_sweep, 0: pose_sweep (direct sum, FP32)
   CellGrid cg;

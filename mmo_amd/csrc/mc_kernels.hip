@@ -140,20 +140,21 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
   int sat = 0;
   if (INTER != kMcInterF64) {
     double e_el = 0.0, e_vd = 0.0;
-    auto atom = [&](int j, int& lx, int& ly, int& lz, double& cq, double& ca, double& cb) {
+    auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
       double x, y, z;
       v.at(j, x, y, z);
-      lx = to_fixed(x, fr.ox, fr.scale, fr.lim, &sat);
-      ly = to_fixed(y, fr.oy, fr.scale, fr.lim, &sat);
-      lz = to_fixed(z, fr.oz, fr.scale, fr.lim, &sat);
+      l = lig_pt(x, y, z, fr, &sat);
       const LigAtom la = a.latoms[li.atom_base + j];
       cq = la.keq; ca = la.a; cb = la.b;
     };
     auto none = [](int, double, double, double) {};
     if (INTER == kMcInterAtom)
-      atom_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, wl, atom, none, e_el, e_vd);
+      // the chains skip the residual correction of pairs closer than 0.5 A: it changes
+      // only energies above 1e7 kcal/mol (by up to ~2e-5 relative)
+      atom_sweep<false, decltype(atom)&, decltype(none)&, false>(a.cg, s_cell, s_pos, s_ab, a.rres, fr,
+                                                               li.n_atoms, wl, atom, none, e_el, e_vd);
     else
-      pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, fr, li.n_atoms, atom, none, e_el, e_vd);
+      pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, none, e_el, e_vd);
     return wave_sum(e_el + e_vd);
   }
   constexpr int K = 2;

@@ -233,6 +233,43 @@ def test_rigid_scan_subset_matches_oracle(sys0):
     torch.cuda.synchronize()
 
 
+def test_rigid_scan_full_size_sample_and_invariants(sys0):
+    """The scan at its full size (4,608 rotations x 20^3 translations = 36,864,000 poses):
+    a random sample of 1,024 poses against the oracle, under the library's tuned shape
+    and every receptor path (LDS, scalar cache, x-cut cells, positions in LDS), which
+    must agree to the last bit with one another where they visit the same pairs."""
+    import torch
+    ctx, lig, rec = sys0
+    lid = ctx.add_ligand(lig)
+    rots = synth.so3_rotations(4608)
+    trans = synth.lattice((0, 0, 0), 20, 0.5)
+    n = len(rots) * len(trans)
+    out = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    print("full scan", n, flush=True)
+    ctx.scan(lid, ff.SHIFTED, rots, trans, out=out)
+    e = out.cpu().numpy()
+    assert np.isfinite(e).all()
+    rng = np.random.default_rng(5)
+    sample = rng.choice(n, 1024, replace=False)
+    pr, pt = (sample // len(trans)).astype(np.int32), (sample % len(trans)).astype(np.int32)
+    ref = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=8)
+    sums = [inter_abs_sum(rec, _place(lig, rots[a], trans[b])[1], lig, True) for a, b in zip(pr, pt)]
+    assert_energy(e[sample], ref, sums, what="full scan sample")
+    keep = ctx.tuning("scan_cull")
+    try:
+        for sc in (1, 2, 4):
+            ctx.tune(scan_cull=sc)
+            print("full scan, receptor path", sc, flush=True)
+            ctx.scan(lid, ff.SHIFTED, rots, trans, out=out)
+            e2 = out.cpu().numpy()
+            assert_energy(e2[sample], ref, sums, what="full scan path %d" % sc)
+            rel = np.abs(e2 - e) / np.maximum(np.abs(e), 1e-3)
+            assert rel.max() < 1e-6, ("path", sc, rel.max())
+    finally:
+        ctx.tune(scan_cull=keep)
+    torch.cuda.synchronize()
+
+
 def test_rigid_scan_interp_matches_oracle():
     import torch
     lig, rec = synth.config0()
@@ -621,6 +658,35 @@ def test_mc_fp32_matches_cpu_restatement():
             assert st["resets"][k] == ref["resets"]
             assert st["best_e"][k] == pytest.approx(ref["best_e"], rel=1e-5, abs=1e-3)
     ctx.tune(mc_sweep=1)
+
+
+def test_mc_config0_full_length_fp64():
+    """The CPU-runnable workload at its full length: the 75-atom ligand in the 3000-atom
+    receptor, one chain, 10,000 MC steps with FP64 pair arithmetic, against the CPU
+    restatement (decisions, step sizes, energies, final conformer)."""
+    from tests.mc_ref import simulate
+    from mmo_amd import mc as mcmod
+    lig, rec = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    lid = ctx.add_ligand(lig)
+    rot0 = synth.random_rotations(np.random.default_rng(7), 1)
+    roi = (np.zeros(3), 6.0)
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI | mcmod.FP64
+    m = mcmod.MonteCarlo(ctx, [lid], rot0, np.zeros((1, 3)), ff.SHIFTED, total_frames=10000,
+                         flags=flags, roi_center=roi[0], roi_r=roi[1], seed=11)
+    print("config0 mc 10000 frames", flush=True)
+    m.run(10000)
+    st = m.state()
+    conf, _ = m.coords()
+    ref = simulate(lig, rec, rot0[0], np.zeros(3), 10000, seed=11, chain=0, hard_roi=roi)
+    assert st["frame"][0] == 10000
+    assert st["accepted"][0] == ref["accepted"] and st["resets"][0] == ref["resets"]
+    assert st["ooroi"][0] == ref["ooroi"]
+    for key in ("best_e", "e"):
+        assert st[key][0] == pytest.approx(ref[key], rel=1e-6, abs=1e-4), key
+    assert np.allclose(st["max_rot"][0], ref["max_rot"]) and np.allclose(st["max_trans"][0], ref["max_trans"])
+    assert np.allclose(conf[0], ref["conf"], atol=1e-8)
 
 
 def test_mc_launch_forms_agree():
