@@ -92,6 +92,38 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
+// Wavefront scans and sums through DPP (row_shr within rows of 16 lanes, then row_bcast
+// across rows): six VALU operations instead of six LDS permutes.  DPP controls (gfx9):
+// row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 = 0x143; lanes without a
+// source read 0.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+// inclusive prefix sum over the 64 lanes (all lanes active)
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += dpp_i<0x111>(v);
+  v += dpp_i<0x112>(v);
+  v += dpp_i<0x114>(v);
+  v += dpp_i<0x118>(v);
+  v += dpp_i<0x142, 0xa>(v);
+  v += dpp_i<0x143, 0xc>(v);
+  return v;
+}
+// sum over the 64 lanes (all lanes active), uniform result
+__device__ __forceinline__ float wave_sum_f(float v) {
+  v += dpp_f<0x111>(v);
+  v += dpp_f<0x112>(v);
+  v += dpp_f<0x114>(v);
+  v += dpp_f<0x118>(v);
+  v += dpp_f<0x142, 0xa>(v);
+  v += dpp_f<0x143, 0xc>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
 __device__ __forceinline__ double wave_max(double v) {
   for (int off = 1; off < kWave; off <<= 1) v = fmax(v, __shfl_xor(v, off, kWave));
   return v;

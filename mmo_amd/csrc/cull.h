@@ -493,14 +493,18 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
     const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
     const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
     const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
-    const int nyc = cy1 - cy0 + 1;
-    const int nrows = (cz1 >= cz0 && nyc > 0) ? (cz1 - cz0 + 1) * nyc : 0;
+    const int nyc = cy1 - cy0 + 1, nzc = cz1 - cz0 + 1;
+    const int nrows = (nzc > 0 && nyc > 0) ? nzc * nyc : 0;
+    // up to 8 x 8 rows (cells of 3.5 A or more): lane = 8 z + y, no division
+    const bool grid8 = nyc <= 8 && nzc <= 8;
     float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
     for (int base_row = 0; base_row < nrows; base_row += kWave) {
       int r0 = 0, len = 0;
       const int idx = base_row + lane;
-      if (idx < nrows) {
-        const int cz = cz0 + idx / nyc, cy = cy0 + idx % nyc;
+      const bool in = grid8 ? (base_row == 0 && (lane >> 3) < nzc && (lane & 7) < nyc) : idx < nrows;
+      if (in) {
+        const int cz = grid8 ? cz0 + (lane >> 3) : cz0 + idx / nyc;
+        const int cy = grid8 ? cy0 + (lane & 7) : cy0 + idx % nyc;
         const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
         const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
         const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
@@ -517,13 +521,12 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
         }
       }
       // inclusive prefix sum of the row lengths: row k covers [incl - len, incl)
-      int incl = len;
-      for (int off = 1; off < kWave; off <<= 1) {
-        const int v = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += v;
+      const int incl = wave_incl_scan(len);
+      const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+      if (total == 0) {
+        if (grid8) break;
+        continue;
       }
-      const int total = uniformi(__shfl(incl, kWave - 1, kWave));
-      if (total == 0) continue;
       const uint64_t nonempty = __ballot(len > 0);
       const int nc = __popcll(nonempty);
       wave_lds_sync();            // the previous rows' readers are done
@@ -564,7 +567,8 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
     if (GRAD) {
       e_vd += (double)fe;
       const double gs = -2.0 * (double)fr.inv_s;
-      const double gx = wave_sum((double)fx), gy = wave_sum((double)fy), gz = wave_sum((double)fz);
+      const double gx = (double)wave_sum_f(fx), gy = (double)wave_sum_f(fy),
+                   gz = (double)wave_sum_f(fz);
       if (lane == 0) gout(j, gs * gx, gs * gy, gs * gz);
     } else {
       e_el = fma((double)sq, cq, e_el);
