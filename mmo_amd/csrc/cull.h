@@ -69,13 +69,30 @@ __device__ __forceinline__ void pair_grad_shifted(const RecPos& p, const RecAB& 
 // 12 A + half-diagonal around the box centre; every receptor atom within 12 A of any
 // of the wavefront's atoms is in a visited cell, and pairs beyond 12 A that are
 // visited contribute exactly 0 (the shift weight is clamped to 0).
+// wavefront min / max through DPP (lanes without a source keep their own value);
+// the result is uniform (read from lane 63, which has seen every lane)
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_keep_f(float v) {
+  const int b = __float_as_int(v);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, ROW_MASK, 0xf, false));
+}
 __device__ __forceinline__ float wave_minf(float v) {
-  for (int off = 1; off < kWave; off <<= 1) v = fminf(v, __shfl_xor(v, off, kWave));
-  return v;
+  v = fminf(v, dpp_keep_f<0x111>(v));
+  v = fminf(v, dpp_keep_f<0x112>(v));
+  v = fminf(v, dpp_keep_f<0x114>(v));
+  v = fminf(v, dpp_keep_f<0x118>(v));
+  v = fminf(v, dpp_keep_f<0x142, 0xa>(v));
+  v = fminf(v, dpp_keep_f<0x143, 0xc>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ float wave_maxf(float v) {
-  for (int off = 1; off < kWave; off <<= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
-  return v;
+  v = fmaxf(v, dpp_keep_f<0x111>(v));
+  v = fmaxf(v, dpp_keep_f<0x112>(v));
+  v = fmaxf(v, dpp_keep_f<0x114>(v));
+  v = fmaxf(v, dpp_keep_f<0x118>(v));
+  v = fmaxf(v, dpp_keep_f<0x142, 0xa>(v));
+  v = fmaxf(v, dpp_keep_f<0x143, 0xc>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 __device__ __forceinline__ float uniformf(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
