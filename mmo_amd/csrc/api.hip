@@ -162,10 +162,14 @@ struct mmo_mc {
 
 struct mmo_batch {
   mmo_ctx* ctx = nullptr;
-  int npose = 0, natoms = 0, ndof = 0, ntiles = 0, max_atoms = 0;
+  int npose = 0, natoms = 0, ndof = 0, ntiles = 0, max_atoms = 0, max_rb = 0, max_rg = 0;
   std::vector<int> pose_lig, atom_off, dof_off, perm;
   std::vector<Tile> tiles;
-  DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm;
+  // poses by ligand size, largest first, and the pose kernel's slot order dealt from it
+  // for workgroups of order_waves wavefronts (0: not dealt yet)
+  std::vector<int> by_size, order;
+  int order_waves = 0;
+  DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm, d_order;
   DevBuf<Tile> d_tiles;
   DevBuf<double> d_xyz, d_e, d_g, d_part;
 };
@@ -220,11 +224,16 @@ int upload(mmo_ctx* c, DevBuf<T>& b, const T* src, size_t n, hipStream_t s) {
 }
 
 std::vector<uint32_t> intra_pairs(const int32_t* dists, int n) {
-  // src/mol.ml:203-208 (interacting = dists[i + j*n] >= 3), pairs i < j (src/mol.ml:885-890)
+  // src/mol.ml:203-208 (interacting = dists[i + j*n] >= 3), pairs i < j (src/mol.ml:885-890).
+  // Listed by diagonal (d = j - i, then i) rather than by row: consecutive entries then
+  // hold distinct i and distinct j, so the lanes of one wavefront add their gradient
+  // terms to distinct LDS words instead of all to atom i's.
   std::vector<uint32_t> out;
-  for (int i = 0; i < n - 1; ++i)
-    for (int j = i + 1; j < n; ++j)
+  for (int d = 1; d < n; ++d)
+    for (int i = 0; i + d < n; ++i) {
+      const int j = i + d;
       if (dists[i + (size_t)j * n] >= 3) out.push_back((uint32_t)i | ((uint32_t)j << 16));
+    }
   return out;
 }
 
@@ -241,6 +250,27 @@ int pair_params(mmo_ctx* c, const std::vector<uint32_t>& pij, const double* q,
     out[p] = make_double4(kElecWeight * q[i] * q[j], Ai * Aj, 2.0 * Bi * Bj, 0.0);
   }
   return MMO_OK;
+}
+
+// Slot order of the pose kernel (slot = workgroup * waves + wavefront) for workgroups of
+// `waves` wavefronts: wavefront w of a workgroup runs on SIMD w % 4 of its CU, so the
+// wavefronts of one workgroup with equal w % 4 form a bin that shares one SIMD.  The
+// poses, largest ligand first, are dealt over the bins in snake order (forward, then
+// backward, ...), which evens out the ligand atoms per SIMD.
+std::vector<int> deal_poses(const std::vector<int>& by_size, int waves) {
+  const int npose = (int)by_size.size();
+  const int simds = std::min(waves, 4), rounds = (waves + simds - 1) / simds;
+  const int nbin = ((npose + waves - 1) / waves) * simds;
+  std::vector<int> order(npose, -1);
+  size_t next = 0;
+  for (int t = 0; t < rounds; ++t)
+    for (int k = 0; k < nbin; ++k) {
+      const int bin = (t & 1) ? nbin - 1 - k : k;
+      const int w = bin % simds + simds * t;
+      const long long slot = (long long)(bin / simds) * waves + w;
+      if (w < waves && slot < npose) order[slot] = by_size[next++];
+    }
+  return order;   // every slot < npose is visited once: next == npose
 }
 
 int sync_ligands(mmo_ctx* c) {
@@ -871,6 +901,25 @@ int mmo_ligand_natoms(const mmo_ctx* c, int id) {
 }
 
 // ------------------------------------------------------------------ batches
+// wavefronts (poses) per workgroup of the pose kernel
+static int pose_cull_waves(const mmo_ctx* c, int npose) {
+  return std::max(1, std::min(c->pose_waves, (npose + 255) / 256));
+}
+
+// Deals the batch's slot order for workgroups of `waves` wavefronts (at creation, and
+// again only if the workgroup shape changes).  The host copy stays in the batch while
+// the upload may be in flight; a re-deal first waits for the previous upload.
+static int deal_batch(mmo_batch* b, int waves, hipStream_t s) {
+  if (b->order_waves == waves) return MMO_OK;
+  mmo_ctx* c = b->ctx;
+  if (b->order_waves != 0) HIPCHK(c, hipStreamSynchronize(s));
+  b->order = deal_poses(b->by_size, waves);
+  int r = upload(c, b->d_order, b->order.data(), b->order.size(), s);
+  if (r) return r;
+  b->order_waves = waves;
+  return MMO_OK;
+}
+
 int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch** out) {
   int r = check_ctx(c);
   if (r) return r;
@@ -898,6 +947,8 @@ int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch**
       return fail(c, MMO_ERR_BAD_ARG, "batch too large: more than 2^31 - 1 atom rows");
     }
     b->max_atoms = std::max(b->max_atoms, c->ligs[l].n);
+    b->max_rb = std::max(b->max_rb, (int)c->ligs[l].rb_left.size());
+    b->max_rg = std::max(b->max_rg, c->ligs[l].rg_off.back());
   }
   b->atom_off[npose] = (int)na;
   b->dof_off[npose] = (int)nd;
@@ -905,8 +956,14 @@ int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch**
   b->ndof = (int)nd;
   plan_tiles(b->pose_lig, &c->ligs, nullptr, c->inter_k, b->perm, b->tiles);
   b->ntiles = (int)b->tiles.size();
+  b->by_size.resize(npose);
+  for (int p = 0; p < npose; ++p) b->by_size[p] = p;
+  std::stable_sort(b->by_size.begin(), b->by_size.end(), [&](int u, int v) {
+    return c->ligs[pose_lig[u]].n > c->ligs[pose_lig[v]].n;
+  });
   hipStream_t s = c->stream;
-  if ((r = upload(c, b->d_pose_lig, b->pose_lig.data(), npose, s)) ||
+  if ((r = deal_batch(b, pose_cull_waves(c, npose), s)) ||
+      (r = upload(c, b->d_pose_lig, b->pose_lig.data(), npose, s)) ||
       (r = upload(c, b->d_atom_off, b->atom_off.data(), npose + 1, s)) ||
       (r = upload(c, b->d_dof_off, b->dof_off.data(), npose + 1, s)) ||
       (r = upload(c, b->d_perm, b->perm.data(), npose, s)) ||
@@ -928,6 +985,7 @@ void mmo_batch_destroy(mmo_batch* b) {
   if (getpid() == b->ctx->pid) {
     (void)hipSetDevice(b->ctx->device);
     b->d_pose_lig.release(); b->d_atom_off.release(); b->d_dof_off.release(); b->d_perm.release();
+    b->d_order.release();
     b->d_tiles.release(); b->d_xyz.release(); b->d_e.release(); b->d_g.release(); b->d_part.release();
   }
   delete b;
@@ -979,7 +1037,9 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     a.rres = c->rres.p;
     a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
     const bool comp = (flags & MMO_F_COMPONENTS) != 0;
-    const int waves = std::max(1, std::min(c->pose_waves, (b->npose + 255) / 256));
+    const int waves = pose_cull_waves(c, b->npose);
+    if ((r = deal_batch(b, waves, s))) return r;
+    a.order = b->d_order.p;
     timer_begin(c, "inter", s);
     if (!want_grad || comp) {
       a.e_comp = comp ? e_out : nullptr;
@@ -1093,8 +1153,13 @@ int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double
   a.ligs = c->d_ligs.p; a.lig_xyz = c->d_lig_xyz.p; a.rb_left = c->d_rb_left.p;
   a.rb_right = c->d_rb_right.p; a.rg_off = c->d_rg_off.p; a.rg_atoms = c->d_rg_atoms.p;
   a.pose_lig = b->d_pose_lig.p; a.atom_off = b->d_atom_off.p; a.dof_off = b->d_dof_off.p;
-  a.npose = b->npose; a.max_atoms = b->max_atoms; a.dofs = dofs; a.xyz_out = xyz;
+  a.npose = b->npose; a.max_atoms = b->max_atoms; a.max_rb = b->max_rb; a.max_rg = b->max_rg;
+  a.dofs = dofs; a.xyz_out = xyz;
   a.radius_out = radius_out;
+  if (dof_lds_bytes(a, g) > kMaxLds)
+    return fail(c, MMO_ERR_BAD_ARG,
+                "ligands too large for the DOF kernels: %zu bytes of LDS per pose (limit %zu)",
+                dof_lds_bytes(a, g), kMaxLds);
   HIPCHK(c, launch_apply_config(a, s));
   const int score_flags = flags & ~MMO_F_COMPONENTS;
   if (score_flags & ~MMO_F_GRAD) {

@@ -88,10 +88,6 @@ __device__ __forceinline__ int group_or(int v, int split) {
   for (int off = 1; off < split; off <<= 1) v |= __shfl_xor(v, off, kWave);
   return v;
 }
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 1; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
-  return v;
-}
 // Wavefront scans and sums through DPP (row_shr within rows of 16 lanes, then row_bcast
 // across rows): six VALU operations instead of six LDS permutes.  DPP controls (gfx9):
 // row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 = 0x143; lanes without a
@@ -124,9 +120,41 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   v += dpp_f<0x143, 0xc>(v);
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
+// FP64 through DPP: both 32-bit halves move under the same control; lanes without a
+// source (or outside ROW_MASK) receive `old`
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_d(double v, double old) {
+  const long long b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi =
+      __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum / max over the 64 lanes (all lanes active), uniform result: the row scan of
+// wave_sum_f in FP64, read from lane 63
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_d<0x111>(v, 0.0);
+  v += dpp_d<0x112>(v, 0.0);
+  v += dpp_d<0x114>(v, 0.0);
+  v += dpp_d<0x118>(v, 0.0);
+  v += dpp_d<0x142, 0xa>(v, 0.0);
+  v += dpp_d<0x143, 0xc>(v, 0.0);
+  return readlane_d(v, 63);
+}
 __device__ __forceinline__ double wave_max(double v) {
-  for (int off = 1; off < kWave; off <<= 1) v = fmax(v, __shfl_xor(v, off, kWave));
-  return v;
+  v = fmax(v, dpp_d<0x111>(v, v));
+  v = fmax(v, dpp_d<0x112>(v, v));
+  v = fmax(v, dpp_d<0x114>(v, v));
+  v = fmax(v, dpp_d<0x118>(v, v));
+  v = fmax(v, dpp_d<0x142, 0xa>(v, v));
+  v = fmax(v, dpp_d<0x143, 0xc>(v, v));
+  return readlane_d(v, 63);
 }
 
 // t = clamp(1 - r2/144, 0, 1): the cutoff test r2 < 144 and the shift

@@ -7,7 +7,7 @@
 //             (K per lane) as 32-bit fixed point, so x_rec - x_lig is exact in integer
 //             arithmetic.  Pair math is FP32, chunks of kChunk receptor atoms are
 //             folded into FP64 per-pose sums.
-//   k_intra   intra-ligand nonbonded energy, src/mol.ml:881-903, FP64, one wavefront
+//   k_intra   intra-ligand nonbonded energy, src/mol.ml:881-903, FP64, one workgroup
 //             per pose over the precomputed list of pairs with topo distance >= 3.
 //   k_interp  trilinear interpolation in per-type FP32 grids, src/G3D.ml:97-157 and
 //             src/mol.ml:1012-1020, FP64 weights.
@@ -345,8 +345,11 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
 
 // ----------------------------------------------------- inter kernel, pose-culled
 // Shifted form for arbitrary poses (batches, gradients, minimisation): one wavefront per
-// pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep), so the
-// cost follows the pairs inside the cutoff wherever the poses are.
+// pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep) or per atom
+// (atom_sweep), so the cost follows the pairs inside the cutoff wherever the poses are.
+// With an order, slot s of the launch (workgroup s / waves, wavefront s % waves) scores
+// pose order[s]: the host deals the poses, largest ligands first, so that the SIMDs
+// receive equal shares of ligand atoms instead of whatever the batch order gives them.
 template <bool GRAD, bool LDS, bool ATOMWISE>
 __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, const RecPos* rpos,
                                                     const RecAB* rab, int nrec, CellGrid cg) {
@@ -356,42 +359,44 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
   const int* s_cell;
   load_receptor<LDS>(rpos, rab, nrec, cg, s_pos, s_ab, s_cell, smem);
   const int lane = threadIdx.x & (kWave - 1);
-  const int p = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
-  if (p >= a.npose) return;
   // this wavefront's scratch for atom_sweep, after the receptor copy
   int* wl = reinterpret_cast<int*>(smem + receptor_lds_bytes<LDS>(nrec, cg)) +
             (threadIdx.x / kWave) * kAtomSweepInts;
-  const LigInfo li = a.ligs[a.pose_lig[p]];
-  const size_t row0 = (size_t)a.atom_off[p];
-  const double* x = a.xyz + 3 * row0;
-  int sat = 0;
-  double e_el = 0.0, e_vd = 0.0;
-  auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
-    l = lig_pt(x[3 * j], x[3 * j + 1], x[3 * j + 2], fr, &sat);
-    const LigAtom la = a.latoms[li.atom_base + j];
-    cq = la.keq; ca = la.a; cb = la.b;
-  };
-  auto gout = [&](int j, double gx, double gy, double gz) {
-    double* g = a.grad + 3 * (row0 + j);
-    g[0] += gx; g[1] += gy; g[2] += gz;
-  };
-  if (ATOMWISE)
-    atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
-  else
-    pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, gout, e_el, e_vd);
-  (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
-  // every lane of an atom slot holds part of the sums: the wavefront sum counts each
-  // pair once (lanes of one slot share the atom, split the receptor atoms)
-  e_el = wave_sum(e_el);
-  e_vd = wave_sum(e_vd);
-  if (lane == 0) {
-    const size_t o = (size_t)p * a.e_stride;
-    if (!GRAD || a.write_e) a.e_out[o] = e_el + e_vd;
-    if (!GRAD && a.e_comp) {
-      a.e_comp[o + 2] = e_el;
-      a.e_comp[o + 3] = e_vd;
+  auto score = [&](int p) {
+    const LigInfo li = a.ligs[a.pose_lig[p]];
+    const size_t row0 = (size_t)a.atom_off[p];
+    const double* x = a.xyz + 3 * row0;
+    int sat = 0;
+    double e_el = 0.0, e_vd = 0.0;
+    auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
+      l = lig_pt(x[3 * j], x[3 * j + 1], x[3 * j + 2], fr, &sat);
+      const LigAtom la = a.latoms[li.atom_base + j];
+      cq = la.keq; ca = la.a; cb = la.b;
+    };
+    auto gout = [&](int j, double gx, double gy, double gz) {
+      double* g = a.grad + 3 * (row0 + j);
+      g[0] += gx; g[1] += gy; g[2] += gz;
+    };
+    if (ATOMWISE)
+      atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
+    else
+      pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, gout, e_el, e_vd);
+    (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
+    // every lane of an atom slot holds part of the sums: the wavefront sum counts each
+    // pair once (lanes of one slot share the atom, split the receptor atoms)
+    e_el = wave_sum(e_el);
+    e_vd = wave_sum(e_vd);
+    if (lane == 0) {
+      const size_t o = (size_t)p * a.e_stride;
+      if (!GRAD || a.write_e) a.e_out[o] = e_el + e_vd;
+      if (!GRAD && a.e_comp) {
+        a.e_comp[o + 2] = e_el;
+        a.e_comp[o + 3] = e_vd;
+      }
     }
-  }
+  };
+  const int slot = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  if (slot < a.npose) score(a.order ? a.order[slot] : slot);
 }
 
 hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
@@ -482,33 +487,52 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
 }
 
 // ------------------------------------------------------------------ intra kernel
-// one wavefront per pose; pairs of the ligand's list are spread over lanes
+// One workgroup of kIntraWaves wavefronts per pose; the pairs of the ligand's list are
+// spread over all its lanes, and each lane loads its next pair while it evaluates the
+// current one, so a large ligand's list is not walked by one wavefront one dependent
+// global load at a time.  Each wavefront adds its gradient terms into its own LDS copy
+// (the lanes of one LDS atomic are applied in lane order) and the copies are summed in
+// wavefront order, so the result does not depend on how the wavefronts interleave.
+constexpr int kIntraWaves = 4;
+
 template <bool GRAD>
-__global__ void __launch_bounds__(64) k_intra(IntraArgs a) {
+__global__ void __launch_bounds__(kIntraWaves * kWave) k_intra(IntraArgs a) {
   extern __shared__ __align__(16) char smem[];
+  __shared__ double s_part[kIntraWaves];
   double* sx = reinterpret_cast<double*>(smem);
   const int pose = blockIdx.x;
   if (pose >= a.npose) return;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   const LigInfo li = a.ligs[a.pose_lig[pose]];
   const int L = li.n_atoms;
   double* sy = sx + L;
   double* sz = sy + L;
-  double* gx = sz + L;
+  double* g0 = sz + L;                        // kIntraWaves copies of (gx, gy, gz)
+  double* gx = g0 + 3 * L * (tid / kWave);    // this wavefront's copy
   double* gy = gx + L;
   double* gz = gy + L;
   const int base = a.atom_off[pose];
-  for (int j = lane; j < L; j += kWave) {
+  for (int j = tid; j < L; j += nt) {
     const double* r = a.xyz + 3 * (size_t)(base + j);
     sx[j] = r[0]; sy[j] = r[1]; sz[j] = r[2];
-    if (GRAD) { gx[j] = 0.0; gy[j] = 0.0; gz[j] = 0.0; }
   }
+  if (GRAD)
+    for (int j = tid; j < 3 * L * (nt / kWave); j += nt) g0[j] = 0.0;
+  const uint32_t* pij = a.pair_ij + li.pair_base;
+  const double4* ppar = a.pair_par + li.pair_base;
+  const int n = li.n_pairs;
+  uint32_t ij_next = tid < n ? pij[tid] : 0u;
+  double4 pp_next = tid < n ? ppar[tid] : make_double4(0.0, 0.0, 0.0, 0.0);
   __syncthreads();
   double e = 0.0;
-  for (int p = lane; p < li.n_pairs; p += kWave) {
-    const uint32_t ij = a.pair_ij[li.pair_base + p];
+  for (int p = tid; p < n; p += nt) {
+    const uint32_t ij = ij_next;
+    const double4 pp = pp_next;   // ke qi qj, Ai Aj, 2 Bi Bj
+    if (p + nt < n) {
+      ij_next = pij[p + nt];
+      pp_next = ppar[p + nt];
+    }
     const int i = ij & 0xffff, j = ij >> 16;
-    const double4 pp = a.pair_par[li.pair_base + p];   // ke qi qj, Ai Aj, 2 Bi Bj
     const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
     const double rr = sqrt(dx * dx + dy * dy + dz * dz);
     const double r = rr < 0.01 ? 0.01 : rr;   // Math.non_zero_dist
@@ -525,13 +549,21 @@ __global__ void __launch_bounds__(64) k_intra(IntraArgs a) {
     }
   }
   e = wave_sum(e);
-  if (lane == 0) a.e_out[(size_t)pose * a.e_stride] = e;
+  if ((tid & (kWave - 1)) == 0) s_part[tid / kWave] = e;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < nt / kWave; ++w) t += s_part[w];
+    a.e_out[(size_t)pose * a.e_stride] = t;
+  }
   if (GRAD) {
-    __syncthreads();
-    for (int j = lane; j < L; j += kWave) {
+    for (int j = tid; j < L; j += nt) {
+      double sum[3] = {0.0, 0.0, 0.0};
+      for (int w = 0; w < nt / kWave; ++w)
+        for (int c = 0; c < 3; ++c) sum[c] += g0[3 * L * w + c * L + j];
       double* g = a.grad + 3 * (size_t)(base + j);
-      if (a.grad_accumulate) { g[0] += gx[j]; g[1] += gy[j]; g[2] += gz[j]; }
-      else { g[0] = gx[j]; g[1] = gy[j]; g[2] = gz[j]; }
+      if (a.grad_accumulate) { g[0] += sum[0]; g[1] += sum[1]; g[2] += sum[2]; }
+      else { g[0] = sum[0]; g[1] = sum[1]; g[2] = sum[2]; }
     }
   }
 }
@@ -753,9 +785,18 @@ hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, 
 
 hipError_t launch_intra(const IntraArgs& a, bool grad, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
-  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
-  if (grad) hipLaunchKernelGGL(k_intra<true>, dim3(a.npose), dim3(kWave), lds, s, a);
-  else hipLaunchKernelGGL(k_intra<false>, dim3(a.npose), dim3(kWave), lds, s, a);
+  // coordinates, then one gradient copy per wavefront
+  const size_t lds = (size_t)a.max_atoms * (grad ? 3 + 3 * kIntraWaves : 3) * sizeof(double);
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(grad ? reinterpret_cast<const void*>(&k_intra<true>)
+                                            : reinterpret_cast<const void*>(&k_intra<false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  const dim3 block(kIntraWaves * kWave);
+  if (grad) hipLaunchKernelGGL(k_intra<true>, dim3(a.npose), block, lds, s, a);
+  else hipLaunchKernelGGL(k_intra<false>, dim3(a.npose), block, lds, s, a);
   return hipGetLastError();
 }
 

@@ -84,6 +84,7 @@ struct DofArgs {
   const int* atom_off;
   const int* dof_off;
   int npose, max_atoms;
+  int max_rb, max_rg;           // most rbonds and rotation-group atoms of a ligand in the batch
   const double* dofs;
   double* xyz_out;
   double* radius_out;           // nullable
@@ -123,12 +124,19 @@ struct PoseCullArgs {
   double* grad;             // GRAD: dE/dxyz rows, added to what the buffer holds
   int write_e;
   const RecRes* rres;       // receptor quantisation residuals
+  // nullable: the pose of each launch slot (slot = workgroup * waves + wavefront), a
+  // permutation of 0..npose-1
+  const int* order;
 };
 // atomwise: lanes over the receptor atoms near each ligand atom (atom_sweep) instead of
 // 8-atom sub-blocks (pose_sweep)
 hipError_t launch_pose_cull(const InterLaunch& L, const PoseCullArgs& a, bool grad, int waves,
                             bool atomwise, hipStream_t s);
 
+// LDS a workgroup may hold on gfx950
+constexpr size_t kMaxLds = 160 * 1024;
+// LDS bytes of one pose of the DOF kernels (coordinates plus the staged torsion tree)
+size_t dof_lds_bytes(const DofArgs& a, bool grad);
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s);
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s);
 

@@ -24,10 +24,17 @@ namespace mmo {
 
 struct Rod { double m[9]; };
 
-// Rot.of_axis_angle, src/rot.ml:121-146
-__device__ __forceinline__ Rod rodrigues(double x, double y, double z, double theta) {
-  double s, c;
-  sincos(theta, &s, &c);
+// LDS of the DOF kernels: `arrays` FP64 coordinate arrays of max_atoms entries, then the
+// staged torsion tree (TorsionLds)
+__host__ __device__ constexpr size_t dof_coord_bytes(int max_atoms, int arrays) {
+  return (size_t)arrays * max_atoms * sizeof(double);
+}
+constexpr size_t torsion_lds_bytes(int max_rb, int max_rg) {
+  return 2 * (size_t)(max_rb + 3) * sizeof(double) + (size_t)(3 * max_rb + 1 + max_rg) * sizeof(int);
+}
+
+// Rot.of_axis_angle, src/rot.ml:121-146, from the sine and cosine of the angle
+__device__ __forceinline__ Rod rodrigues(double x, double y, double z, double s, double c) {
   const double o = 1.0 - c;
   Rod r;
   r.m[0] = c + x * x * o;     r.m[1] = x * y * o - z * s; r.m[2] = x * z * o + y * s;
@@ -36,13 +43,11 @@ __device__ __forceinline__ Rod rodrigues(double x, double y, double z, double th
   return r;
 }
 
-// Rot.r_xyz, src/rot.ml:52-66, and its partial derivatives
-__device__ __forceinline__ void rot_xyz(double al, double be, double ga, double* m, double* da,
+// Rot.r_xyz, src/rot.ml:52-66, and its partial derivatives; sn/cs hold the sines and
+// cosines of (alpha, beta, gamma)
+__device__ __forceinline__ void rot_xyz(const double* sn, const double* cs, double* m, double* da,
                                         double* db, double* dg) {
-  double as, ac, bs, bc, gs, gc;
-  sincos(al, &as, &ac);
-  sincos(be, &bs, &bc);
-  sincos(ga, &gs, &gc);
+  const double as = sn[0], ac = cs[0], bs = sn[1], bc = cs[1], gs = sn[2], gc = cs[2];
   m[0] = bc * gc;  m[1] = gc * as * bs - ac * gs;  m[2] = as * gs + ac * gc * bs;
   m[3] = bc * gs;  m[4] = ac * gc + as * bs * gs;  m[5] = ac * bs * gs - gc * as;
   m[6] = -bs;      m[7] = bc * as;                 m[8] = ac * bc;
@@ -59,21 +64,62 @@ __device__ __forceinline__ void rot_xyz(double al, double be, double ga, double*
   }
 }
 
+// The pose's torsion tree staged in LDS before the serial torsion loop, so that the
+// loop reads nothing from global memory: rbond ends, rotation groups (offsets relative
+// to the ligand's first group atom) and the sine and cosine of every torsion angle
+// (entries 0..nrb-1) and of the three rigid-body angles (entries nrb..nrb+2), evaluated
+// by all lanes at once.
+struct TorsionLds {
+  double* sn;
+  double* cs;
+  int* left;
+  int* right;
+  int* off;
+  int* atoms;
+};
+
+__device__ TorsionLds stage_torsions(const DofArgs& a, const LigInfo& li, const double* d,
+                                     char* base) {
+  TorsionLds t;
+  t.sn = reinterpret_cast<double*>(base);
+  t.cs = t.sn + a.max_rb + 3;
+  t.left = reinterpret_cast<int*>(t.cs + a.max_rb + 3);
+  t.right = t.left + a.max_rb;
+  t.off = t.right + a.max_rb;
+  t.atoms = t.off + a.max_rb + 1;
+  const int nrb = li.nrb;
+  for (int k = threadIdx.x; k < nrb + 3; k += kWave) {
+    double s, c;
+    sincos(k < nrb ? d[6 + k] : d[3 + (k - nrb)], &s, &c);
+    t.sn[k] = s;
+    t.cs[k] = c;
+    if (k < nrb) {
+      t.left[k] = a.rb_left[li.rb_base + k];
+      t.right[k] = a.rb_right[li.rb_base + k];
+    }
+  }
+  const int g0 = a.rg_off[li.rg_base];
+  for (int k = threadIdx.x; k <= nrb; k += kWave) t.off[k] = a.rg_off[li.rg_base + k] - g0;
+  const int ng = a.rg_off[li.rg_base + nrb] - g0;
+  for (int m = threadIdx.x; m < ng; m += kWave) t.atoms[m] = a.rg_atoms[g0 + m];
+  return t;
+}
+
 // torsions in rbond order on the conformer held in LDS (all 64 lanes participate)
-__device__ void forward_torsions(const DofArgs& a, const LigInfo& li, const double* d,
-                                 double* sx, double* sy, double* sz) {
+__device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double* sx, double* sy,
+                                 double* sz) {
   const int lane = threadIdx.x;
   for (int k = 0; k < li.nrb; ++k) {
     __syncthreads();
-    const int left = a.rb_left[li.rb_base + k], right = a.rb_right[li.rb_base + k];
+    const int left = t.left[k], right = t.right[k];
     const double px = sx[right], py = sy[right], pz = sz[right];
     double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
     const double mag = sqrt(vx * vx + vy * vy + vz * vz);   // V3.normalize
-    const Rod R = rodrigues(vx / mag, vy / mag, vz / mag, d[6 + k]);
-    const int g0 = a.rg_off[li.rg_base + k], g1 = a.rg_off[li.rg_base + k + 1];
+    const Rod R = rodrigues(vx / mag, vy / mag, vz / mag, t.sn[k], t.cs[k]);
+    const int g0 = t.off[k], g1 = t.off[k + 1];
     __syncthreads();
     for (int m = g0 + lane; m < g1; m += kWave) {
-      const int i = a.rg_atoms[m];
+      const int i = t.atoms[m];
       const double zx = sx[i] - px, zy = sy[i] - py, zz = sz[i] - pz;
       sx[i] = (R.m[0] * zx + R.m[1] * zy + R.m[2] * zz) + px;
       sy[i] = (R.m[3] * zx + R.m[4] * zy + R.m[5] * zz) + py;
@@ -101,8 +147,9 @@ __global__ void __launch_bounds__(64) k_apply_config(DofArgs a) {
   double* sy = sx + L;
   double* sz = sy + L;
   const double* d = a.dofs + a.dof_off[pose];
+  const TorsionLds t = stage_torsions(a, li, d, smem + dof_coord_bytes(a.max_atoms, 3));
   load_conformer(a, li, sx, sy, sz);
-  forward_torsions(a, li, d, sx, sy, sz);
+  forward_torsions(li, t, sx, sy, sz);   // its barriers publish the staged tree
   // centre: the registered conformer's (origin) until a torsion updates it to the mean
   double cx = 0.0, cy = 0.0, cz = 0.0;
   if (li.nrb > 0) {
@@ -118,7 +165,7 @@ __global__ void __launch_bounds__(64) k_apply_config(DofArgs a) {
   rmax = wave_max(rmax);
   if (lane == 0 && a.radius_out) a.radius_out[pose] = rmax;
   double M[9];
-  rot_xyz(d[3], d[4], d[5], M, nullptr, nullptr, nullptr);
+  rot_xyz(t.sn + li.nrb, t.cs + li.nrb, M, nullptr, nullptr, nullptr);
   const int base = a.atom_off[pose];
   for (int j = lane; j < L; j += kWave) {
     double* o = a.xyz_out + 3 * (size_t)(base + j);
@@ -142,10 +189,11 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
   double* hz = hy + L;
   const double* d = a.dofs + a.dof_off[pose];
   double* gd = a.gdof + a.dof_off[pose];
+  const TorsionLds t = stage_torsions(a, li, d, smem + dof_coord_bytes(a.max_atoms, 6));
   load_conformer(a, li, sx, sy, sz);
-  forward_torsions(a, li, d, sx, sy, sz);
+  forward_torsions(li, t, sx, sy, sz);
   double M[9], dA[9], dB[9], dG[9];
-  rot_xyz(d[3], d[4], d[5], M, dA, dB, dG);
+  rot_xyz(t.sn + li.nrb, t.cs + li.nrb, M, dA, dB, dG);
   const int base = a.atom_off[pose];
   double tx = 0.0, ty = 0.0, tz = 0.0, S[9];
   for (int q = 0; q < 9; ++q) S[q] = 0.0;
@@ -170,20 +218,18 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
   if (lane == 0) { gd[0] = tx; gd[1] = ty; gd[2] = tz; gd[3] = ga; gd[4] = gb; gd[5] = gg; }
   __syncthreads();
   for (int k = li.nrb - 1; k >= 0; --k) {
-    const int left = a.rb_left[li.rb_base + k], right = a.rb_right[li.rb_base + k];
+    const int left = t.left[k], right = t.right[k];
     const double px = sx[right], py = sy[right], pz = sz[right];
     const double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
     const double mag = sqrt(vx * vx + vy * vy + vz * vz);
     const double ax = vx / mag, ay = vy / mag, az = vz / mag;
-    const double tau = d[6 + k];
-    const Rod R = rodrigues(ax, ay, az, tau);
-    double sn, cs;
-    sincos(tau, &sn, &cs);
+    const double sn = t.sn[k], cs = t.cs[k];
+    const Rod R = rodrigues(ax, ay, az, sn, cs);
     const double omc = 1.0 - cs;
-    const int g0 = a.rg_off[li.rg_base + k], g1 = a.rg_off[li.rg_base + k + 1];
+    const int g0 = t.off[k], g1 = t.off[k + 1];
     double tb = 0.0, abx = 0.0, aby = 0.0, abz = 0.0, pbx = 0.0, pby = 0.0, pbz = 0.0;
     for (int m = g0 + lane; m < g1; m += kWave) {
-      const int i = a.rg_atoms[m];
+      const int i = t.atoms[m];
       const double yx = sx[i] - px, yy = sy[i] - py, yz = sz[i] - pz;   // rotated offset
       const double h0 = hx[i], h1 = hy[i], h2 = hz[i];
       // dE/dtau: h . (a x y)
@@ -219,16 +265,35 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
   }
 }
 
+size_t dof_lds_bytes(const DofArgs& a, bool grad) {
+  return dof_coord_bytes(a.max_atoms, grad ? 6 : 3) + torsion_lds_bytes(a.max_rb, a.max_rg);
+}
+
+// the launches above 64 KiB of LDS raise the kernel's dynamic-LDS limit (160 KiB per
+// workgroup on gfx950); callers check dof_lds_bytes against kMaxLds first
+template <class K>
+static hipError_t lds_limit(K* kernel, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 hipError_t launch_apply_config(const DofArgs& a, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
-  const size_t lds = (size_t)a.max_atoms * 3 * sizeof(double);
+  const size_t lds = dof_lds_bytes(a, false);
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  hipError_t e = lds_limit(&k_apply_config, lds);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_apply_config, dim3(a.npose), dim3(kWave), lds, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
-  const size_t lds = (size_t)a.max_atoms * 6 * sizeof(double);
+  const size_t lds = dof_lds_bytes(a, true);
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  hipError_t e = lds_limit(&k_dof_grad, lds);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_dof_grad, dim3(a.npose), dim3(kWave), lds, s, a);
   return hipGetLastError();
 }
