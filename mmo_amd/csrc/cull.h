@@ -482,7 +482,7 @@ __device__ __forceinline__ int in_vgpr_i(int x) {
   return x;
 }
 
-template <bool GRAD, class AtomFn, class GradOut, bool NEAR = true>
+template <bool GRAD, class AtomFn, class GradOut, bool NEAR = true, bool PRELOAD = true>
 __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab,
@@ -496,101 +496,133 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
   const uint64_t below = lane ? ((1ull << lane) - 1ull) : 0ull;
   constexpr float R = 12.05f;     // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
-  for (int j = 0; j < n_atoms; ++j) {
-    LigPt lu;
-    double cq, ca, cb;
-    atom(j, lu, cq, ca, cb);
-    const float qx = uniformf((float)lu.x * fr.inv_s), qy = uniformf((float)lu.y * fr.inv_s),
-                qz = uniformf((float)lu.z * fr.inv_s);
-    // the pair loop reads the (wave-uniform) ligand atom from VGPRs: an SGPR operand
-    // would cost issue rate on every pair
-    const LigPt lv{in_vgpr_i(lu.x), in_vgpr_i(lu.y), in_vgpr_i(lu.z), lu.rx, lu.ry, lu.rz};
-    const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
-    const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
-    const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
-    const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
-    const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
-    const int nyc = cy1 - cy0 + 1, nzc = cz1 - cz0 + 1;
-    const int nrows = (nzc > 0 && nyc > 0) ? nzc * nyc : 0;
-    // up to 8 x 8 rows (cells of 3.5 A or more): lane = 8 z + y, no division
-    const bool grid8 = nyc <= 8 && nzc <= 8;
-    float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int base_row = 0; base_row < nrows; base_row += kWave) {
-      int r0 = 0, len = 0;
-      const int idx = base_row + lane;
-      const bool in = grid8 ? (base_row == 0 && (lane >> 3) < nzc && (lane & 7) < nyc) : idx < nrows;
-      if (in) {
-        const int cz = grid8 ? cz0 + (lane >> 3) : cz0 + idx / nyc;
-        const int cy = grid8 ? cy0 + (lane & 7) : cy0 + idx % nyc;
-        const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
-        const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
-        const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
-        const float rx2 = R2 - dzb * dzb - dyb * dyb;
-        if (rx2 >= 0.0f) {
-          const float rx = sqrtf(rx2);
-          const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
-          const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
-          if (cx0 <= cx1) {
-            const int rowb = (cz * cg.ny + cy) * cg.nx;
-            r0 = s_cell[rowb + cx0];
-            len = s_cell[rowb + cx1 + 1] - r0;
+  // PRELOAD: the atoms of a 64-atom chunk are fetched together, one per lane, and
+  // broadcast one at a time; in GRAD the lane of an atom keeps its gradient until the
+  // chunk ends, so no atom waits on its own load or gradient write
+  for (int jb = 0; jb < n_atoms; jb += (PRELOAD ? kWave : 1)) {
+    LigPt mine = lig_pt_zero();
+    double mq = 0.0, ma = 0.0, mb = 0.0;
+    if (PRELOAD && jb + lane < n_atoms) atom(jb + lane, mine, mq, ma, mb);
+    float gxm = 0.0f, gym = 0.0f, gzm = 0.0f;
+    const int nj = PRELOAD ? min(kWave, n_atoms - jb) : 1;
+    for (int jj = 0; jj < nj; ++jj) {
+      const int j = jb + jj;
+      LigPt lu;
+      double cq, ca, cb;
+      if (PRELOAD) {
+        lu.x = __builtin_amdgcn_readlane(mine.x, jj);
+        lu.y = __builtin_amdgcn_readlane(mine.y, jj);
+        lu.z = __builtin_amdgcn_readlane(mine.z, jj);
+        lu.rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rx), jj));
+        lu.ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.ry), jj));
+        lu.rz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rz), jj));
+        cq = readlane_d(mq, jj);
+        ca = readlane_d(ma, jj);
+        cb = readlane_d(mb, jj);
+      } else {
+        atom(j, lu, cq, ca, cb);
+      }
+      const float qx = uniformf((float)lu.x * fr.inv_s), qy = uniformf((float)lu.y * fr.inv_s),
+                  qz = uniformf((float)lu.z * fr.inv_s);
+      // the pair loop reads the (wave-uniform) ligand atom from VGPRs: an SGPR operand
+      // would cost issue rate on every pair
+      const LigPt lv{in_vgpr_i(lu.x), in_vgpr_i(lu.y), in_vgpr_i(lu.z), lu.rx, lu.ry, lu.rz};
+      const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
+      const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
+      const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
+      const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
+      const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
+      const int nyc = cy1 - cy0 + 1, nzc = cz1 - cz0 + 1;
+      const int nrows = (nzc > 0 && nyc > 0) ? nzc * nyc : 0;
+      // up to 8 x 8 rows (cells of 3.5 A or more): lane = 8 z + y, no division
+      const bool grid8 = nyc <= 8 && nzc <= 8;
+      float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      for (int base_row = 0; base_row < nrows; base_row += kWave) {
+        int r0 = 0, len = 0;
+        const int idx = base_row + lane;
+        const bool in = grid8 ? (base_row == 0 && (lane >> 3) < nzc && (lane & 7) < nyc) : idx < nrows;
+        if (in) {
+          const int cz = grid8 ? cz0 + (lane >> 3) : cz0 + idx / nyc;
+          const int cy = grid8 ? cy0 + (lane & 7) : cy0 + idx % nyc;
+          const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
+          const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
+          const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
+          const float rx2 = R2 - dzb * dzb - dyb * dyb;
+          if (rx2 >= 0.0f) {
+            const float rx = sqrtf(rx2);
+            const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+            const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
+            if (cx0 <= cx1) {
+              const int rowb = (cz * cg.ny + cy) * cg.nx;
+              r0 = s_cell[rowb + cx0];
+              len = s_cell[rowb + cx1 + 1] - r0;
+            }
+          }
+        }
+        // inclusive prefix sum of the row lengths: row k covers [incl - len, incl)
+        const int incl = wave_incl_scan(len);
+        const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
+        if (total == 0) {
+          if (grid8) break;
+          continue;
+        }
+        const uint64_t nonempty = __ballot(len > 0);
+        const int nc = __popcll(nonempty);
+        wave_lds_sync();            // the previous rows' readers are done
+        if (len > 0) {
+          const int c = __popcll(nonempty & below);
+          s_r0[c] = r0;
+          s_beg[c] = incl - len;
+        }
+        if (lane == 0) s_beg[nc] = total;
+        wave_lds_sync();
+        // this lane's segment [t, t_end) of the concatenation and the row holding t:
+        // the last compacted row k with s_beg[k] <= t (s_beg[0] = 0, rows non-empty)
+        const int m = (total + kWave - 1) / kWave;
+        int t = lane * m;
+        const int t_end = min(total, t + m);
+        int k = 0;
+        for (int step = kWave / 2; step > 0; step >>= 1)
+          if (k + step < nc && s_beg[k + step] <= t) k += step;
+        int i = s_r0[k] + (t - s_beg[k]);
+        int row_end = s_beg[k + 1];
+        for (int it = 0; it < m; ++it, ++t) {
+          if (t < t_end) {
+            if (t == row_end) {       // the next compacted row starts here
+              ++k;
+              i = s_r0[k];
+              row_end = s_beg[k + 1];
+            }
+            const RecPos p = s_pos[i];
+            const RecAB ab = s_ab[i];
+            if (GRAD)
+              pair_grad_shifted(p, ab, lv, i, pk, c1, fa, fb, fe, fx, fy, fz);
+            else
+              pair_energy<MMO_SHIFTED, NEAR>(p, ab, lv, i, pk, sq, s12, s6);
+            ++i;
           }
         }
       }
-      // inclusive prefix sum of the row lengths: row k covers [incl - len, incl)
-      const int incl = wave_incl_scan(len);
-      const int total = __builtin_amdgcn_readlane(incl, kWave - 1);
-      if (total == 0) {
-        if (grid8) break;
-        continue;
-      }
-      const uint64_t nonempty = __ballot(len > 0);
-      const int nc = __popcll(nonempty);
-      wave_lds_sync();            // the previous rows' readers are done
-      if (len > 0) {
-        const int c = __popcll(nonempty & below);
-        s_r0[c] = r0;
-        s_beg[c] = incl - len;
-      }
-      if (lane == 0) s_beg[nc] = total;
-      wave_lds_sync();
-      // this lane's segment [t, t_end) of the concatenation and the row holding t:
-      // the last compacted row k with s_beg[k] <= t (s_beg[0] = 0, rows non-empty)
-      const int m = (total + kWave - 1) / kWave;
-      int t = lane * m;
-      const int t_end = min(total, t + m);
-      int k = 0;
-      for (int step = kWave / 2; step > 0; step >>= 1)
-        if (k + step < nc && s_beg[k + step] <= t) k += step;
-      int i = s_r0[k] + (t - s_beg[k]);
-      int row_end = s_beg[k + 1];
-      for (int it = 0; it < m; ++it, ++t) {
-        if (t < t_end) {
-          if (t == row_end) {       // the next compacted row starts here
-            ++k;
-            i = s_r0[k];
-            row_end = s_beg[k + 1];
-          }
-          const RecPos p = s_pos[i];
-          const RecAB ab = s_ab[i];
-          if (GRAD)
-            pair_grad_shifted(p, ab, lv, i, pk, c1, fa, fb, fe, fx, fy, fz);
-          else
-            pair_energy<MMO_SHIFTED, NEAR>(p, ab, lv, i, pk, sq, s12, s6);
-          ++i;
+      if (GRAD) {
+        e_vd += (double)fe;
+        const float gx = wave_sum_f(fx), gy = wave_sum_f(fy), gz = wave_sum_f(fz);
+        if (PRELOAD) {
+          gxm = lane == jj ? gx : gxm;
+          gym = lane == jj ? gy : gym;
+          gzm = lane == jj ? gz : gzm;
+        } else if (lane == 0) {
+          const double gs = -2.0 * (double)fr.inv_s;
+          gout(j, gs * (double)gx, gs * (double)gy, gs * (double)gz);
         }
+      } else {
+        e_el = fma((double)sq, cq, e_el);
+        e_vd = fma((double)s12, ca, e_vd);
+        e_vd = fma(-2.0 * (double)s6, cb, e_vd);
       }
     }
-    if (GRAD) {
-      e_vd += (double)fe;
+    if (GRAD && PRELOAD && jb + lane < n_atoms) {
       const double gs = -2.0 * (double)fr.inv_s;
-      const double gx = (double)wave_sum_f(fx), gy = (double)wave_sum_f(fy),
-                   gz = (double)wave_sum_f(fz);
-      if (lane == 0) gout(j, gs * gx, gs * gy, gs * gz);
-    } else {
-      e_el = fma((double)sq, cq, e_el);
-      e_vd = fma((double)s12, ca, e_vd);
-      e_vd = fma(-2.0 * (double)s6, cb, e_vd);
+      gout(jb + lane, gs * (double)gxm, gs * (double)gym, gs * (double)gzm);
     }
   }
 }
