@@ -461,9 +461,10 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // lane k fetches the atom range of row k, the non-empty rows are compacted into LDS
 // with a wavefront prefix sum that lays their ranges end to end, and the concatenation
 // (`total` pairs) is cut into 64 contiguous segments of ceil(total / 64) positions, one
-// per lane.  A lane finds the row of its first position by a 6-step search of the
-// compacted row starts and then walks its segment, stepping to the next row where one
-// ends, so the bookkeeping is one search per atom and a compare per pair.
+// per lane.  A lane finds the row of its first position from a count of rows per
+// first lane (LDS atomics) and a prefix sum across the lanes, and then walks its
+// segment, stepping to the next row where one ends, so the bookkeeping is a few LDS
+// round trips per atom and a compare per pair.
 // wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, l, cq, ca, cb) gives
 // atom j (LigPt, wave-uniform); NEAR = false skips the residual correction of pairs
 // closer than 0.5 A (energy form only); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1,
@@ -493,6 +494,7 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
   const int lane = threadIdx.x & (kWave - 1);
   int* s_r0 = wl;                 // compacted rows: first receptor atom
   int* s_beg = wl + kWave;        //                 first position; s_beg[nc] = total
+  int* s_cnt = wl + 2 * kWave;    // [1..63]: rows whose first segment lane is this one
   const uint64_t below = lane ? ((1ull << lane) - 1ull) : 0ull;
   constexpr float R = 12.05f;     // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
@@ -568,6 +570,14 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
         }
         const uint64_t nonempty = __ballot(len > 0);
         const int nc = __popcll(nonempty);
+        // this lane's segment [t, t_end) of the concatenation
+        const int m = (total + kWave - 1) / kWave;
+        int t = lane * m;
+        const int t_end = min(total, t + m);
+        // first lane whose segment starts at or after this row's start; only the first
+        // non-empty row starts at 0 (lane 0), and a row that no segment starts in (first
+        // lane 64) is reached by walking
+        const int first_lane = (incl - len + m - 1) / m;
         wave_lds_sync();            // the previous rows' readers are done
         if (len > 0) {
           const int c = __popcll(nonempty & below);
@@ -575,23 +585,22 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
           s_beg[c] = incl - len;
         }
         if (lane == 0) s_beg[nc] = total;
+        if (lane > 0) s_cnt[lane] = 0;
         wave_lds_sync();
-        // this lane's segment [t, t_end) of the concatenation and the row holding t:
-        // the last compacted row k with s_beg[k] <= t (s_beg[0] = 0, rows non-empty)
-        const int m = (total + kWave - 1) / kWave;
-        int t = lane * m;
-        const int t_end = min(total, t + m);
-        int k = 0;
-        for (int step = kWave / 2; step > 0; step >>= 1)
-          if (k + step < nc && s_beg[k + step] <= t) k += step;
+        if (len > 0 && first_lane > 0 && first_lane < kWave) atomicAdd(&s_cnt[first_lane], 1);
+        wave_lds_sync();
+        // the row holding t: k = (rows starting at or before t) - 1, a prefix sum over
+        // the lanes of how many rows first reach each lane (1 at lane 0: row 0)
+        const int k = wave_incl_scan(lane > 0 ? s_cnt[lane] : 1) - 1;
         int i = s_r0[k] + (t - s_beg[k]);
         int row_end = s_beg[k + 1];
+        int kw = k;
         for (int it = 0; it < m; ++it, ++t) {
           if (t < t_end) {
             if (t == row_end) {       // the next compacted row starts here
-              ++k;
-              i = s_r0[k];
-              row_end = s_beg[k + 1];
+              ++kw;
+              i = s_r0[kw];
+              row_end = s_beg[kw + 1];
             }
             const RecPos p = s_pos[i];
             const RecAB ab = s_ab[i];
