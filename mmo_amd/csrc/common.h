@@ -157,6 +157,11 @@ __device__ __forceinline__ double wave_max(double v) {
   return readlane_d(v, 63);
 }
 
+// Square root for the culling bounds: the hardware instruction (within 1 ulp) instead
+// of the correctly rounded expansion of sqrtf; every bound it feeds carries a margin of
+// 0.05 A beyond the 12 A cutoff, so no pair inside the cutoff is culled.
+__device__ __forceinline__ float bound_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // t = clamp(1 - r2/144, 0, 1): the cutoff test r2 < 144 and the shift
 // S(r) = (1 - (r/12)^2)^2 (src/FF.ml:17-20) in one clamped FMA.
 __device__ __forceinline__ float shift_t(float r2c) {

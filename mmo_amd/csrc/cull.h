@@ -126,7 +126,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
     const float dz = fmaxf(0.0f, fmaxf(zlo - (qz + hz), (qz - hz) - (zlo + cg.c)));
     const float rz2 = R2 - dz * dz;
     if (rz2 < 0.0f) continue;
-    const float ry = sqrtf(rz2);
+    const float ry = bound_sqrt(rz2);
     const int cy0 = max(0, (int)floorf((qy - hy - ry - cg.y0) * cg.inv_c));
     const int cy1 = min(cg.ny - 1, (int)floorf((qy + hy + ry - cg.y0) * cg.inv_c));
     float sq[K], s12[K], s6[K];
@@ -137,7 +137,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
       const float dy = fmaxf(0.0f, fmaxf(ylo - (qy + hy), (qy - hy) - (ylo + cg.c)));
       const float rx2 = rz2 - dy * dy;
       if (rx2 < 0.0f) continue;
-      const float rx = hx + sqrtf(rx2);
+      const float rx = hx + bound_sqrt(rx2);
       const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
       const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
@@ -207,7 +207,7 @@ __device__ __forceinline__ double cull_sweep_f64(const CellGrid& cg, const int* 
     const float dz = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
     const float rz2 = R2 - dz * dz;
     if (rz2 < 0.0f) continue;
-    const float ry = sqrtf(rz2);
+    const float ry = bound_sqrt(rz2);
     const int cy0 = max(0, (int)floorf((qy - ry - cg.y0) * cg.inv_c));
     const int cy1 = min(cg.ny - 1, (int)floorf((qy + ry - cg.y0) * cg.inv_c));
     for (int cy = cy0; cy <= cy1; ++cy) {
@@ -215,7 +215,7 @@ __device__ __forceinline__ double cull_sweep_f64(const CellGrid& cg, const int* 
       const float dy = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
       const float rx2 = rz2 - dy * dy;
       if (rx2 < 0.0f) continue;
-      const float rx = sqrtf(rx2);
+      const float rx = bound_sqrt(rx2);
       const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
       const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
       if (cx0 > cx1) continue;
@@ -408,7 +408,7 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
         const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
         const float rx2 = R2 - dzb * dzb - dyb * dyb;
         if (rx2 >= 0.0f) {
-          const float rx = sqrtf(rx2);
+          const float rx = bound_sqrt(rx2);
           const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
           const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
           if (cx0 <= cx1) {
@@ -476,6 +476,15 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ceil(a / m) for 0 <= a < 64 m, given inv_m within 1 ulp of 1/m: the float quotient
+// (below 64) is within 1 of floor(a / m) and is corrected in integers
+__device__ __forceinline__ int ceil_div_small(int a, int m, float inv_m) {
+  int q = (int)((float)a * inv_m);
+  q -= q * m > a ? 1 : 0;
+  q += (q + 1) * m <= a ? 1 : 0;
+  return q + (q * m < a ? 1 : 0);
 }
 
 __device__ __forceinline__ int in_vgpr_i(int x) {
@@ -551,7 +560,7 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
           const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
           const float rx2 = R2 - dzb * dzb - dyb * dyb;
           if (rx2 >= 0.0f) {
-            const float rx = sqrtf(rx2);
+            const float rx = bound_sqrt(rx2);
             const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
             const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
             if (cx0 <= cx1) {
@@ -577,7 +586,7 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
         // first lane whose segment starts at or after this row's start; only the first
         // non-empty row starts at 0 (lane 0), and a row that no segment starts in (first
         // lane 64) is reached by walking
-        const int first_lane = (incl - len + m - 1) / m;
+        const int first_lane = ceil_div_small(incl - len, m, __builtin_amdgcn_rcpf((float)m));
         wave_lds_sync();            // the previous rows' readers are done
         if (len > 0) {
           const int c = __popcll(nonempty & below);
