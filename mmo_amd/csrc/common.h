@@ -157,6 +157,14 @@ __device__ __forceinline__ double wave_max(double v) {
   return readlane_d(v, 63);
 }
 
+// orders this wavefront's LDS accesses before the barrier against those after it (LDS
+// that only the wavefront itself reads and writes needs no workgroup barrier)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Square root for the culling bounds: the hardware instruction (within 1 ulp) instead
 // of the correctly rounded expansion of sqrtf; every bound it feeds carries a margin of
 // 0.05 A beyond the 12 A cutoff, so no pair inside the cutoff is culled.

@@ -472,11 +472,6 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 constexpr int kAtomSweepInts = 3 * kWave;
 
 // LDS writes of this wavefront made visible to its own lanes
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 // ceil(a / m) for 0 <= a < 64 m, given inv_m within 1 ulp of 1/m: the float quotient
 // (below 64) is within 1 of floor(a / m) and is corrected in integers

@@ -90,6 +90,12 @@ struct DofArgs {
   double* radius_out;           // nullable
   const double* grad_xyz;       // dE/dxyz (for the adjoint)
   double* gdof;                 // dE/dDOF
+  // nullable: the adjoint also evaluates E_intra and its gradient (pair list below) and
+  // writes E_intra to e_intra[pose * e_stride]
+  double* e_intra;
+  int e_stride;
+  const uint32_t* pair_ij;
+  const float4* pair_par;       // ke qi qj, Ai Aj, 2 Bi Bj in FP32
 };
 
 // part: scratch of (nslices * npose * e_stride) doubles when the receptor is split

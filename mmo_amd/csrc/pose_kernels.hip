@@ -24,6 +24,9 @@ namespace mmo {
 
 struct Rod { double m[9]; };
 
+// wavefronts per pose of the adjoint when it also evaluates E_intra
+constexpr int kDofWaves = 4;
+
 // LDS of the DOF kernels: `arrays` FP64 coordinate arrays of max_atoms entries, then the
 // staged torsion tree (TorsionLds)
 __host__ __device__ constexpr size_t dof_coord_bytes(int max_atoms, int arrays) {
@@ -88,7 +91,7 @@ __device__ TorsionLds stage_torsions(const DofArgs& a, const LigInfo& li, const 
   t.off = t.right + a.max_rb;
   t.atoms = t.off + a.max_rb + 1;
   const int nrb = li.nrb;
-  for (int k = threadIdx.x; k < nrb + 3; k += kWave) {
+  for (int k = threadIdx.x; k < nrb + 3; k += blockDim.x) {
     double s, c;
     sincos(k < nrb ? d[6 + k] : d[3 + (k - nrb)], &s, &c);
     t.sn[k] = s;
@@ -99,16 +102,16 @@ __device__ TorsionLds stage_torsions(const DofArgs& a, const LigInfo& li, const 
     }
   }
   const int g0 = a.rg_off[li.rg_base];
-  for (int k = threadIdx.x; k <= nrb; k += kWave) t.off[k] = a.rg_off[li.rg_base + k] - g0;
+  for (int k = threadIdx.x; k <= nrb; k += blockDim.x) t.off[k] = a.rg_off[li.rg_base + k] - g0;
   const int ng = a.rg_off[li.rg_base + nrb] - g0;
-  for (int m = threadIdx.x; m < ng; m += kWave) t.atoms[m] = a.rg_atoms[g0 + m];
+  for (int m = threadIdx.x; m < ng; m += blockDim.x) t.atoms[m] = a.rg_atoms[g0 + m];
   return t;
 }
 
-// torsions in rbond order on the conformer held in LDS (all 64 lanes participate)
+// torsions in rbond order on the conformer held in LDS (all threads participate)
 __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double* sx, double* sy,
                                  double* sz) {
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
   for (int k = 0; k < li.nrb; ++k) {
     __syncthreads();
     const int left = t.left[k], right = t.right[k];
@@ -118,7 +121,7 @@ __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double*
     const Rod R = rodrigues(vx / mag, vy / mag, vz / mag, t.sn[k], t.cs[k]);
     const int g0 = t.off[k], g1 = t.off[k + 1];
     __syncthreads();
-    for (int m = g0 + lane; m < g1; m += kWave) {
+    for (int m = g0 + tid; m < g1; m += nt) {
       const int i = t.atoms[m];
       const double zx = sx[i] - px, zy = sy[i] - py, zz = sz[i] - pz;
       sx[i] = (R.m[0] * zx + R.m[1] * zy + R.m[2] * zz) + px;
@@ -129,9 +132,49 @@ __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double*
   __syncthreads();
 }
 
+// src/mol.ml:881-903 over the ligand's pair list (topo distance >= 3) for the conformer
+// in LDS, every thread of the workgroup taking pairs p = tid, tid + blockDim.x, ... with
+// the next pair loaded ahead.  Pair arithmetic in FP32 from the FP64 coordinate
+// differences, energy sums in FP64; gradient terms are added in FP64 to this wavefront's
+// LDS copy g*.  Returns the wavefront's energy sum (uniform).
+__device__ double intra_pairs(const uint32_t* pij, const float4* ppar, int n, const double* sx,
+                              const double* sy, const double* sz, double* gx, double* gy,
+                              double* gz) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  uint32_t ij_next = tid < n ? pij[tid] : 0u;
+  float4 pp_next = tid < n ? ppar[tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  double e = 0.0;
+  for (int p = tid; p < n; p += nt) {
+    const uint32_t ij = ij_next;
+    const float4 pp = pp_next;   // ke qi qj, Ai Aj, 2 Bi Bj
+    if (p + nt < n) {
+      ij_next = pij[p + nt];
+      pp_next = ppar[p + nt];
+    }
+    const int i = ij & 0xffff, j = ij >> 16;
+    const float dx = (float)(sx[i] - sx[j]), dy = (float)(sy[i] - sy[j]),
+                dz = (float)(sz[i] - sz[j]);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool apart = r2 >= 1.0e-4f;                       // r >= 0.01 A
+    const float ir = apart ? __builtin_amdgcn_rsqf(r2) : 100.0f;   // Math.non_zero_dist
+    const float u = ir * ir;
+    const float u3 = u * u * u;
+    const float u6 = u3 * u3;
+    e += (double)(pp.x * ir + pp.y * u6 - pp.z * u3);
+    if (apart) {
+      // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
+      const float c = (-pp.x * ir * u - 12.0f * pp.y * u6 * u + 6.0f * pp.z * u3 * u);
+      const double cx = (double)(c * dx), cy = (double)(c * dy), cz = (double)(c * dz);
+      atomicAdd(&gx[i], cx); atomicAdd(&gy[i], cy); atomicAdd(&gz[i], cz);
+      atomicAdd(&gx[j], -cx); atomicAdd(&gy[j], -cy); atomicAdd(&gz[j], -cz);
+    }
+  }
+  return wave_sum(e);
+}
+
 __device__ void load_conformer(const DofArgs& a, const LigInfo& li, double* sx, double* sy,
                                double* sz) {
-  for (int j = threadIdx.x; j < li.n_atoms; j += kWave) {
+  for (int j = threadIdx.x; j < li.n_atoms; j += blockDim.x) {
     const double* c = a.lig_xyz + 3 * (size_t)(li.atom_base + j);
     sx[j] = c[0]; sy[j] = c[1]; sz[j] = c[2];
   }
@@ -175,23 +218,49 @@ __global__ void __launch_bounds__(64) k_apply_config(DofArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
+// INTRA: E_intra and its gradient are evaluated here too, on the conformer in LDS
+// (E_intra does not depend on the rigid-body DOFs): all kDofWaves wavefronts share the
+// pair list as in k_intra, each into its own LDS gradient copy, summed in wavefront
+// order into the conformer-frame adjoint; E_intra goes to e_intra[pose * e_stride].
+// The rigid-body sums and the reverse torsion pass then run on wavefront 0 alone.
+template <bool INTRA>
+__global__ void __launch_bounds__(INTRA ? kDofWaves * kWave : kWave) k_dof_grad(DofArgs a) {
   extern __shared__ __align__(16) char smem[];
+  __shared__ double s_part[kDofWaves];
   const int pose = blockIdx.x;
   if (pose >= a.npose) return;
   const LigInfo li = a.ligs[a.pose_lig[pose]];
-  const int L = li.n_atoms, lane = threadIdx.x;
+  const int L = li.n_atoms, tid = threadIdx.x, lane = tid & (kWave - 1);
   double* sx = reinterpret_cast<double*>(smem);
   double* sy = sx + L;
   double* sz = sy + L;
   double* hx = sz + L;
   double* hy = hx + L;
   double* hz = hy + L;
+  double* gi = hz + L;   // INTRA: kDofWaves copies of (gx, gy, gz)
   const double* d = a.dofs + a.dof_off[pose];
   double* gd = a.gdof + a.dof_off[pose];
-  const TorsionLds t = stage_torsions(a, li, d, smem + dof_coord_bytes(a.max_atoms, 6));
+  const TorsionLds t =
+      stage_torsions(a, li, d, smem + dof_coord_bytes(a.max_atoms, INTRA ? 6 + 3 * kDofWaves : 6));
   load_conformer(a, li, sx, sy, sz);
-  forward_torsions(li, t, sx, sy, sz);
+  if (INTRA)
+    for (int j = tid; j < 3 * L * kDofWaves; j += blockDim.x) gi[j] = 0.0;
+  forward_torsions(li, t, sx, sy, sz);   // ends with a barrier
+  if (INTRA) {
+    double* gx = gi + 3 * L * (tid / kWave);
+    double* gy = gx + L;
+    double* gz = gy + L;
+    const double e = intra_pairs(a.pair_ij + li.pair_base, a.pair_par + li.pair_base, li.n_pairs,
+                                 sx, sy, sz, gx, gy, gz);
+    if (lane == 0) s_part[tid / kWave] = e;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < kDofWaves; ++w) sum += s_part[w];
+      a.e_intra[(size_t)pose * a.e_stride] = sum;
+    }
+    if (tid >= kWave) return;   // wavefront 0 goes on alone: no workgroup barrier below
+  }
   double M[9], dA[9], dB[9], dG[9];
   rot_xyz(t.sn + li.nrb, t.cs + li.nrb, M, dA, dB, dG);
   const int base = a.atom_off[pose];
@@ -204,10 +273,17 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
     const double c[3] = {sx[j], sy[j], sz[j]};
     for (int r = 0; r < 3; ++r)
       for (int cc = 0; cc < 3; ++cc) S[3 * r + cc] += (r == 0 ? g0 : r == 1 ? g1 : g2) * c[cc];
-    // h = M^T g
-    hx[j] = M[0] * g0 + M[3] * g1 + M[6] * g2;
-    hy[j] = M[1] * g0 + M[4] * g1 + M[7] * g2;
-    hz[j] = M[2] * g0 + M[5] * g1 + M[8] * g2;
+    // h = M^T g (+ the conformer-frame intra gradient)
+    double i0 = 0.0, i1 = 0.0, i2 = 0.0;
+    if (INTRA)
+      for (int w = 0; w < kDofWaves; ++w) {
+        i0 += gi[3 * L * w + j];
+        i1 += gi[3 * L * w + L + j];
+        i2 += gi[3 * L * w + 2 * L + j];
+      }
+    hx[j] = (M[0] * g0 + M[3] * g1 + M[6] * g2) + i0;
+    hy[j] = (M[1] * g0 + M[4] * g1 + M[7] * g2) + i1;
+    hz[j] = (M[2] * g0 + M[5] * g1 + M[8] * g2) + i2;
   }
   tx = wave_sum(tx); ty = wave_sum(ty); tz = wave_sum(tz);
   double ga = 0.0, gb = 0.0, gg = 0.0;
@@ -216,7 +292,7 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
     ga += dA[q] * sq; gb += dB[q] * sq; gg += dG[q] * sq;
   }
   if (lane == 0) { gd[0] = tx; gd[1] = ty; gd[2] = tz; gd[3] = ga; gd[4] = gb; gd[5] = gg; }
-  __syncthreads();
+  wave_lds_sync();
   for (int k = li.nrb - 1; k >= 0; --k) {
     const int left = t.left[k], right = t.right[k];
     const double px = sx[right], py = sy[right], pz = sz[right];
@@ -252,7 +328,7 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
     tb = wave_sum(tb);
     abx = wave_sum(abx); aby = wave_sum(aby); abz = wave_sum(abz);
     pbx = wave_sum(pbx); pby = wave_sum(pby); pbz = wave_sum(pbz);
-    __syncthreads();
+    wave_lds_sync();
     if (lane == 0) {
       gd[6 + k] = tb;
       const double adot = ax * abx + ay * aby + az * abz;
@@ -261,12 +337,13 @@ __global__ void __launch_bounds__(64) k_dof_grad(DofArgs a) {
       hx[right] += vbx + pbx; hy[right] += vby + pby; hz[right] += vbz + pbz;
       hx[left] -= vbx; hy[left] -= vby; hz[left] -= vbz;
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
 size_t dof_lds_bytes(const DofArgs& a, bool grad) {
-  return dof_coord_bytes(a.max_atoms, grad ? 6 : 3) + torsion_lds_bytes(a.max_rb, a.max_rg);
+  const int arrays = grad ? (a.e_intra ? 6 + 3 * kDofWaves : 6) : 3;
+  return dof_coord_bytes(a.max_atoms, arrays) + torsion_lds_bytes(a.max_rb, a.max_rg);
 }
 
 // the launches above 64 KiB of LDS raise the kernel's dynamic-LDS limit (160 KiB per
@@ -292,9 +369,15 @@ hipError_t launch_dof_grad(const DofArgs& a, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
   const size_t lds = dof_lds_bytes(a, true);
   if (lds > kMaxLds) return hipErrorInvalidConfiguration;
-  hipError_t e = lds_limit(&k_dof_grad, lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_dof_grad, dim3(a.npose), dim3(kWave), lds, s, a);
+  if (a.e_intra) {
+    hipError_t e = lds_limit(&k_dof_grad<true>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dof_grad<true>, dim3(a.npose), dim3(kDofWaves * kWave), lds, s, a);
+  } else {
+    hipError_t e = lds_limit(&k_dof_grad<false>, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dof_grad<false>, dim3(a.npose), dim3(kWave), lds, s, a);
+  }
   return hipGetLastError();
 }
 
