@@ -151,3 +151,54 @@ def test_size_limits(system):
     assert _code(lambda: Minimizer(b, history=0)) == ERR_BAD_ARG
     assert _code(lambda: Minimizer(b, history=17)) == ERR_BAD_ARG
     torch.cuda.synchronize()
+
+
+def _zigzag(n, nrb):
+    """Planar zig-zag chain of n carbons (1.5 A bonds, 120 degree angles) centred at the
+    origin, with rotatable bonds 1-2, 2-3, ... (nrb of them) turning the atoms beyond."""
+    k = np.arange(n)
+    xs = 1.5 * math.cos(math.pi / 6) * k
+    ys = np.where(k % 2 == 0, 0.0, 0.75)
+    c = np.stack([xs, ys, np.zeros(n)], 1)
+    c -= c.mean(0)
+    groups = [list(range(b + 2, n)) for b in range(1, nrb + 1)]
+    return {"xs": c[:, 0], "ys": c[:, 1], "zs": c[:, 2],
+            "q": np.where(k % 2 == 0, 0.1, -0.1), "anum": np.full(n, 6, dtype=np.int32),
+            "dists": np.abs(k[:, None] - k[None, :]).astype(np.int32),
+            "rb_left": list(range(1, nrb + 1)), "rb_right": list(range(2, nrb + 2)),
+            "rg_off": np.concatenate([[0], np.cumsum([len(g) for g in groups])]),
+            "rg_atoms": np.concatenate(groups).astype(np.int32)}
+
+
+def test_dof_gradient_intra_routes(system):
+    """With the gradient, E_intra comes from the DOF adjoint (FP32 pair terms) while it
+    fits the adjoint's LDS, else from the intra kernel: the first agrees with the
+    energy-only call (FP64 pair terms) to the energy tolerance, and a ligand past the
+    limit (1200 atoms) gives the intra kernel's value exactly."""
+    import torch
+    from tests.parity import assert_energy, intra_abs_sum
+    ctx, _, _ = system
+    rng = np.random.default_rng(11)
+    for n in (60, 1200):
+        print("edge: intra route, %d-atom chain" % n, flush=True)
+        lig = _zigzag(n, 3)
+        lid = ctx.add_ligand(lig)
+        b = ctx.batch([lid, lid])
+        d = torch.tensor(np.concatenate([np.concatenate([[40.0, 40.0, 40.0], rng.uniform(-1, 1, 3),
+                                                         rng.uniform(-0.5, 0.5, 3)])
+                                         for _ in range(2)]), device="cuda:0")
+        e_g = torch.zeros((2, 4), dtype=torch.float64, device="cuda:0")
+        e_0 = torch.zeros_like(e_g)
+        x_t = torch.zeros((b.natoms, 3), dtype=torch.float64, device="cuda:0")
+        g = torch.zeros_like(d)
+        b.score_dofs_dev(d, e_g, xyz_out=x_t, gdof=g, flags=ff.F_INTRA | ff.F_GRAD)
+        b.score_dofs_dev(d, e_0, flags=ff.F_INTRA)
+        torch.cuda.synchronize()
+        eg, e0 = e_g.cpu().numpy()[:, 1], e_0.cpu().numpy()[:, 1]
+        assert np.isfinite(g.cpu().numpy()).all()
+        if n < 1000:
+            x = x_t.cpu().numpy()
+            sums = [intra_abs_sum(lig, x[n * p:n * (p + 1)]) for p in range(2)]
+            assert_energy(eg, e0, sums, what="fused intra, %d atoms" % n)
+        else:
+            assert np.array_equal(eg, e0)
