@@ -114,7 +114,6 @@ struct mmo_ctx {
   DevBuf<double> d_lig_xyz;
   DevBuf<uint32_t> d_pair_ij;
   DevBuf<double4> d_pair_par;
-  DevBuf<float4> d_pair_parf;   // the same parameters in FP32 (the adjoint's intra pairs)
   DevBuf<int> d_rb_left, d_rb_right, d_rg_off, d_rg_atoms;
   // scratch for the plain (one ligand) forms
   DevBuf<double> s_xyz, s_e, s_g, s_part;
@@ -321,10 +320,6 @@ int sync_ligands(mmo_ctx* c) {
   if ((r = upload(c, c->d_lig_xyz, xyz.data(), xyz.size(), s))) return r;
   if ((r = upload(c, c->d_pair_ij, pij.data(), pij.size(), s))) return r;
   if ((r = upload(c, c->d_pair_par, ppar.data(), ppar.size(), s))) return r;
-  std::vector<float4> pparf(ppar.size());
-  for (size_t k = 0; k < ppar.size(); ++k)
-    pparf[k] = make_float4((float)ppar[k].x, (float)ppar[k].y, (float)ppar[k].z, 0.0f);
-  if ((r = upload(c, c->d_pair_parf, pparf.data(), pparf.size(), s))) return r;
   if ((r = upload(c, c->d_rb_left, rbl.data(), rbl.size(), s))) return r;
   if ((r = upload(c, c->d_rb_right, rbr.data(), rbr.size(), s))) return r;
   if ((r = upload(c, c->d_rg_off, rgo.data(), rgo.size(), s))) return r;
@@ -485,7 +480,7 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     c->rpos.release(); c->rab.release(); c->rres.release(); c->grid.release();
     c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
-    c->d_pair_ij.release(); c->d_pair_par.release(); c->d_pair_parf.release();
+    c->d_pair_ij.release(); c->d_pair_par.release();
     c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
     c->s_xyz.release(); c->s_e.release(); c->s_g.release(); c->s_part.release();
     c->s_lig.release(); c->s_latom.release(); c->s_pij.release(); c->s_ppar.release();
@@ -1181,7 +1176,7 @@ int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double
         ai.e_intra = e_out + 1;
         ai.e_stride = 4;
         ai.pair_ij = c->d_pair_ij.p;
-        ai.pair_par = c->d_pair_parf.p;
+        ai.pair_par = c->d_pair_par.p;
         if (dof_lds_bytes(ai, true) <= kMaxLds) {
           a = ai;
           bflags &= ~MMO_F_INTRA;

@@ -95,7 +95,7 @@ struct DofArgs {
   double* e_intra;
   int e_stride;
   const uint32_t* pair_ij;
-  const float4* pair_par;       // ke qi qj, Ai Aj, 2 Bi Bj in FP32
+  const double4* pair_par;      // ke qi qj, Ai Aj, 2 Bi Bj
 };
 
 // part: scratch of (nslices * npose * e_stride) doubles when the receptor is split

@@ -133,40 +133,37 @@ __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double*
 }
 
 // src/mol.ml:881-903 over the ligand's pair list (topo distance >= 3) for the conformer
-// in LDS, every thread of the workgroup taking pairs p = tid, tid + blockDim.x, ... with
-// the next pair loaded ahead.  Pair arithmetic in FP32 from the FP64 coordinate
-// differences, energy sums in FP64; gradient terms are added in FP64 to this wavefront's
-// LDS copy g*.  Returns the wavefront's energy sum (uniform).
-__device__ double intra_pairs(const uint32_t* pij, const float4* ppar, int n, const double* sx,
+// in LDS, FP64 as in k_intra, every thread of the workgroup taking pairs p = tid,
+// tid + blockDim.x, ... with the next pair loaded ahead; gradient terms go to this
+// wavefront's LDS copy g*.  Returns the wavefront's energy sum (uniform).
+__device__ double intra_pairs(const uint32_t* pij, const double4* ppar, int n, const double* sx,
                               const double* sy, const double* sz, double* gx, double* gy,
                               double* gz) {
   const int tid = threadIdx.x, nt = blockDim.x;
   uint32_t ij_next = tid < n ? pij[tid] : 0u;
-  float4 pp_next = tid < n ? ppar[tid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  double4 pp_next = tid < n ? ppar[tid] : make_double4(0.0, 0.0, 0.0, 0.0);
   double e = 0.0;
   for (int p = tid; p < n; p += nt) {
     const uint32_t ij = ij_next;
-    const float4 pp = pp_next;   // ke qi qj, Ai Aj, 2 Bi Bj
+    const double4 pp = pp_next;   // ke qi qj, Ai Aj, 2 Bi Bj
     if (p + nt < n) {
       ij_next = pij[p + nt];
       pp_next = ppar[p + nt];
     }
     const int i = ij & 0xffff, j = ij >> 16;
-    const float dx = (float)(sx[i] - sx[j]), dy = (float)(sy[i] - sy[j]),
-                dz = (float)(sz[i] - sz[j]);
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const bool apart = r2 >= 1.0e-4f;                       // r >= 0.01 A
-    const float ir = apart ? __builtin_amdgcn_rsqf(r2) : 100.0f;   // Math.non_zero_dist
-    const float u = ir * ir;
-    const float u3 = u * u * u;
-    const float u6 = u3 * u3;
-    e += (double)(pp.x * ir + pp.y * u6 - pp.z * u3);
-    if (apart) {
+    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = rr < 0.01 ? 0.01 : rr;   // Math.non_zero_dist
+    const double ir = 1.0 / r;
+    const double u = ir * ir;
+    const double u3 = u * u * u;
+    const double u6 = u3 * u3;
+    e += pp.x * ir + pp.y * u6 - pp.z * u3;
+    if (rr >= 0.01) {
       // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
-      const float c = (-pp.x * ir * u - 12.0f * pp.y * u6 * u + 6.0f * pp.z * u3 * u);
-      const double cx = (double)(c * dx), cy = (double)(c * dy), cz = (double)(c * dz);
-      atomicAdd(&gx[i], cx); atomicAdd(&gy[i], cy); atomicAdd(&gz[i], cz);
-      atomicAdd(&gx[j], -cx); atomicAdd(&gy[j], -cy); atomicAdd(&gz[j], -cz);
+      const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
+      atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
+      atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
     }
   }
   return wave_sum(e);

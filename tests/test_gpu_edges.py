@@ -171,10 +171,10 @@ def _zigzag(n, nrb):
 
 
 def test_dof_gradient_intra_routes(system):
-    """With the gradient, E_intra comes from the DOF adjoint (FP32 pair terms) while it
-    fits the adjoint's LDS, else from the intra kernel: the first agrees with the
-    energy-only call (FP64 pair terms) to the energy tolerance, and a ligand past the
-    limit (1200 atoms) gives the intra kernel's value exactly."""
+    """With the gradient, E_intra comes from the DOF adjoint (on the conformer) while it
+    fits the adjoint's LDS, else from the intra kernel (on the posed coordinates): the
+    first agrees with the energy-only call to the energy tolerance, and a ligand past
+    the limit (1200 atoms) gives the intra kernel's value exactly."""
     import torch
     from tests.parity import assert_energy, intra_abs_sum
     ctx, _, _ = system
