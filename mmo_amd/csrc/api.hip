@@ -1027,7 +1027,7 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
   if (want_grad) HIPCHK(c, launch_fill(grad, 3 * (size_t)b->natoms, 0.0, s));
   HIPCHK(c, launch_fill(e_out, 4 * (size_t)b->npose, 0.0, s));
   if ((inter & MMO_F_INTER_SHIFTED) && c->pose_cull) {
-    // one wavefront per pose, receptor culled per 8-atom sub-block
+    // one wavefront per pose over the culled receptor (per-atom or 8-atom sub-block sweep)
     InterLaunch L = inter_launch(c, MMO_SHIFTED, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
     L.cg = c->cgp;
     // the receptor goes to LDS when enough poses per workgroup amortise the copy

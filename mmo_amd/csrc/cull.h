@@ -466,12 +466,12 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // segment, stepping to the next row where one ends, so the bookkeeping is a few LDS
 // round trips per atom and a compare per pair.
 // wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, l, cq, ca, cb) gives
-// atom j (LigPt, wave-uniform); NEAR = false skips the residual correction of pairs
-// closer than 0.5 A (energy form only); with GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1,
-// FP64) from lane 0.  Sums: FP32 per lane and atom, FP64 across atoms.
+// atom j (LigPt); with PRELOAD it is called by lane j % 64 for its chunk's atoms (so it
+// may depend on the lane's j), else by all lanes for one j at a time.  NEAR = false
+// skips the residual correction of pairs closer than 0.5 A (energy form only); with
+// GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from the lane holding atom j
+// (PRELOAD) or from lane 0.  Sums: FP32 per lane and atom, FP64 across atoms.
 constexpr int kAtomSweepInts = 3 * kWave;
-
-// LDS writes of this wavefront made visible to its own lanes
 
 // ceil(a / m) for 0 <= a < 64 m, given inv_m within 1 ulp of 1/m: the float quotient
 // (below 64) is within 1 of floor(a / m) and is corrected in integers

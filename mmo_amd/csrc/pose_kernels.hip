@@ -1,5 +1,6 @@
 // pose_kernels.hip -- DOFs -> coordinates (Optim.apply_config, src/optim.ml:64-80) and
-// its adjoint (dE/dxyz -> dE/dDOF), one wavefront per pose, FP64 throughout.
+// its adjoint (dE/dxyz -> dE/dDOF), one workgroup per pose (one wavefront; four when the
+// adjoint also evaluates E_intra), FP64 throughout.
 //
 // Forward: the registered centred conformer c0; for k = 0..R-1 the atoms of rotation
 // group k turn by tau_k about the axis right_k - left_k through right_k
