@@ -429,7 +429,11 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
         sw_reset(st.sw);
         ++st.resets;
       } else if (metropolis) {
-        const bool accept = (curr_e <= st.prev_e) || (u5 < exp(-(curr_e - st.prev_e) * a.beta));
+        // u5 is a multiple of 2^-53, so once positive it exceeds exp(-x) for every x >= 40
+        // (exp(-40) < 2^-53): those proposals are rejected without evaluating exp
+        const double x = (curr_e - st.prev_e) * a.beta;
+        const bool accept =
+            (curr_e <= st.prev_e) || (!(x >= 40.0 && u5 > 0.0) && u5 < exp(-x));
         if (rigid) sw_process(st.sw, accept);
         else if (just_rotated > -1) sw_process(rsw[just_rotated], accept);
         if (accept) {
