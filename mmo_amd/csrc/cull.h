@@ -487,13 +487,15 @@ __device__ __forceinline__ int in_vgpr_i(int x) {
   return x;
 }
 
-template <bool GRAD, class AtomFn, class GradOut, bool NEAR = true, bool PRELOAD = true>
+template <bool GRAD, class AtomFn, class GradOut, bool NEAR = true, bool PRELOAD = true,
+          int NA = 1>
 __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab,
                                            const RecRes* __restrict__ rres, const Frame& fr,
                                            int n_atoms, int* wl, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
+  static_assert(NA == 1 || (NA == 2 && PRELOAD), "two atoms per pass need the preload");
   const PairK pk = pair_k(fr.inv_s2, fr.inv_s, rres);
   const int lane = threadIdx.x & (kWave - 1);
   int* s_r0 = wl;                 // compacted rows: first receptor atom
@@ -504,45 +506,65 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
   constexpr float R2 = R * R;
   // PRELOAD: the atoms of a 64-atom chunk are fetched together, one per lane, and
   // broadcast one at a time; in GRAD the lane of an atom keeps its gradient until the
-  // chunk ends, so no atom waits on its own load or gradient write
+  // chunk ends, so no atom waits on its own load or gradient write.  NA = 2: two atoms
+  // per pass share one row listing (the rows of the union of their cutoff spheres, each
+  // row's x range the hull of the two) and every receptor atom read is paired with both;
+  // an odd last atom is paired with itself and its copy is dropped.
   for (int jb = 0; jb < n_atoms; jb += (PRELOAD ? kWave : 1)) {
     LigPt mine = lig_pt_zero();
     double mq = 0.0, ma = 0.0, mb = 0.0;
     if (PRELOAD && jb + lane < n_atoms) atom(jb + lane, mine, mq, ma, mb);
     float gxm = 0.0f, gym = 0.0f, gzm = 0.0f;
     const int nj = PRELOAD ? min(kWave, n_atoms - jb) : 1;
-    for (int jj = 0; jj < nj; ++jj) {
-      const int j = jb + jj;
-      LigPt lu;
-      double cq, ca, cb;
-      if (PRELOAD) {
-        lu.x = __builtin_amdgcn_readlane(mine.x, jj);
-        lu.y = __builtin_amdgcn_readlane(mine.y, jj);
-        lu.z = __builtin_amdgcn_readlane(mine.z, jj);
-        lu.rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rx), jj));
-        lu.ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.ry), jj));
-        lu.rz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rz), jj));
-        cq = readlane_d(mq, jj);
-        ca = readlane_d(ma, jj);
-        cb = readlane_d(mb, jj);
-      } else {
-        atom(j, lu, cq, ca, cb);
+    for (int jj = 0; jj < nj; jj += NA) {
+      LigPt lu[NA];
+      double cq[NA], ca[NA], cb[NA];
+      bool live[NA];
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        live[u] = jj + u < nj;
+        const int ju = live[u] ? jj + u : jj;
+        if (PRELOAD) {
+          lu[u].x = __builtin_amdgcn_readlane(mine.x, ju);
+          lu[u].y = __builtin_amdgcn_readlane(mine.y, ju);
+          lu[u].z = __builtin_amdgcn_readlane(mine.z, ju);
+          lu[u].rx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rx), ju));
+          lu[u].ry = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.ry), ju));
+          lu[u].rz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine.rz), ju));
+          cq[u] = readlane_d(mq, ju);
+          ca[u] = readlane_d(ma, ju);
+          cb[u] = readlane_d(mb, ju);
+        } else {
+          atom(jb + jj, lu[u], cq[u], ca[u], cb[u]);
+        }
       }
-      const float qx = uniformf((float)lu.x * fr.inv_s), qy = uniformf((float)lu.y * fr.inv_s),
-                  qz = uniformf((float)lu.z * fr.inv_s);
-      // the pair loop reads the (wave-uniform) ligand atom from VGPRs: an SGPR operand
-      // would cost issue rate on every pair
-      const LigPt lv{in_vgpr_i(lu.x), in_vgpr_i(lu.y), in_vgpr_i(lu.z), lu.rx, lu.ry, lu.rz};
-      const float c1 = (float)cq, fa = (float)ca, fb = (float)cb;
-      const int cz0 = max(0, (int)floorf((qz - R - cg.z0) * cg.inv_c));
-      const int cz1 = min(cg.nz - 1, (int)floorf((qz + R - cg.z0) * cg.inv_c));
-      const int cy0 = max(0, (int)floorf((qy - R - cg.y0) * cg.inv_c));
-      const int cy1 = min(cg.ny - 1, (int)floorf((qy + R - cg.y0) * cg.inv_c));
+      float qx[NA], qy[NA], qz[NA], c1[NA], fa[NA], fb[NA];
+      LigPt lv[NA];
+      float zlo_q = 3.0e38f, zhi_q = -3.0e38f, ylo_q = 3.0e38f, yhi_q = -3.0e38f;
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        qx[u] = uniformf((float)lu[u].x * fr.inv_s);
+        qy[u] = uniformf((float)lu[u].y * fr.inv_s);
+        qz[u] = uniformf((float)lu[u].z * fr.inv_s);
+        // the pair loop reads the (wave-uniform) ligand atoms from VGPRs: an SGPR operand
+        // would cost issue rate on every pair
+        lv[u] = LigPt{in_vgpr_i(lu[u].x), in_vgpr_i(lu[u].y), in_vgpr_i(lu[u].z), lu[u].rx,
+                      lu[u].ry, lu[u].rz};
+        c1[u] = (float)cq[u]; fa[u] = (float)ca[u]; fb[u] = (float)cb[u];
+        zlo_q = fminf(zlo_q, qz[u]); zhi_q = fmaxf(zhi_q, qz[u]);
+        ylo_q = fminf(ylo_q, qy[u]); yhi_q = fmaxf(yhi_q, qy[u]);
+      }
+      const int cz0 = max(0, (int)floorf((zlo_q - R - cg.z0) * cg.inv_c));
+      const int cz1 = min(cg.nz - 1, (int)floorf((zhi_q + R - cg.z0) * cg.inv_c));
+      const int cy0 = max(0, (int)floorf((ylo_q - R - cg.y0) * cg.inv_c));
+      const int cy1 = min(cg.ny - 1, (int)floorf((yhi_q + R - cg.y0) * cg.inv_c));
       const int nyc = cy1 - cy0 + 1, nzc = cz1 - cz0 + 1;
       const int nrows = (nzc > 0 && nyc > 0) ? nzc * nyc : 0;
       // up to 8 x 8 rows (cells of 3.5 A or more): lane = 8 z + y, no division
       const bool grid8 = nyc <= 8 && nzc <= 8;
-      float sq = 0.0f, s12 = 0.0f, s6 = 0.0f, fe = 0.0f, fx = 0.0f, fy = 0.0f, fz = 0.0f;
+      float sq[NA], s12[NA], s6[NA], fe[NA], fx[NA], fy[NA], fz[NA];
+#pragma unroll
+      for (int u = 0; u < NA; ++u) sq[u] = s12[u] = s6[u] = fe[u] = fx[u] = fy[u] = fz[u] = 0.0f;
       for (int base_row = 0; base_row < nrows; base_row += kWave) {
         int r0 = 0, len = 0;
         const int idx = base_row + lane;
@@ -551,13 +573,21 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
           const int cz = grid8 ? cz0 + (lane >> 3) : cz0 + idx / nyc;
           const int cy = grid8 ? cy0 + (lane & 7) : cy0 + idx % nyc;
           const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
-          const float dzb = fmaxf(0.0f, fmaxf(zlo - qz, qz - (zlo + cg.c)));
-          const float dyb = fmaxf(0.0f, fmaxf(ylo - qy, qy - (ylo + cg.c)));
-          const float rx2 = R2 - dzb * dzb - dyb * dyb;
-          if (rx2 >= 0.0f) {
-            const float rx = bound_sqrt(rx2);
-            const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
-            const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
+          float xlo = 3.0e38f, xhi = -3.0e38f;
+#pragma unroll
+          for (int u = 0; u < NA; ++u) {
+            const float dzb = fmaxf(0.0f, fmaxf(zlo - qz[u], qz[u] - (zlo + cg.c)));
+            const float dyb = fmaxf(0.0f, fmaxf(ylo - qy[u], qy[u] - (ylo + cg.c)));
+            const float rx2 = R2 - dzb * dzb - dyb * dyb;
+            if (rx2 >= 0.0f) {
+              const float rx = bound_sqrt(rx2);
+              xlo = fminf(xlo, qx[u] - rx);
+              xhi = fmaxf(xhi, qx[u] + rx);
+            }
+          }
+          if (xlo <= xhi) {
+            const int cx0 = max(0, (int)floorf((xlo - cg.x0) * cg.inv_cx));
+            const int cx1 = min(cg.nx - 1, (int)floorf((xhi - cg.x0) * cg.inv_cx));
             if (cx0 <= cx1) {
               const int rowb = (cz * cg.ny + cy) * cg.nx;
               r0 = s_cell[rowb + cx0];
@@ -608,29 +638,37 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
             }
             const RecPos p = s_pos[i];
             const RecAB ab = s_ab[i];
-            if (GRAD)
-              pair_grad_shifted(p, ab, lv, i, pk, c1, fa, fb, fe, fx, fy, fz);
-            else
-              pair_energy<MMO_SHIFTED, NEAR>(p, ab, lv, i, pk, sq, s12, s6);
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+              if (GRAD)
+                pair_grad_shifted(p, ab, lv[u], i, pk, c1[u], fa[u], fb[u], fe[u], fx[u], fy[u],
+                                  fz[u]);
+              else
+                pair_energy<MMO_SHIFTED, NEAR>(p, ab, lv[u], i, pk, sq[u], s12[u], s6[u]);
+            }
             ++i;
           }
         }
       }
-      if (GRAD) {
-        e_vd += (double)fe;
-        const float gx = wave_sum_f(fx), gy = wave_sum_f(fy), gz = wave_sum_f(fz);
-        if (PRELOAD) {
-          gxm = lane == jj ? gx : gxm;
-          gym = lane == jj ? gy : gym;
-          gzm = lane == jj ? gz : gzm;
-        } else if (lane == 0) {
-          const double gs = -2.0 * (double)fr.inv_s;
-          gout(j, gs * (double)gx, gs * (double)gy, gs * (double)gz);
+#pragma unroll
+      for (int u = 0; u < NA; ++u) {
+        if (!live[u]) continue;
+        if (GRAD) {
+          e_vd += (double)fe[u];
+          const float gx = wave_sum_f(fx[u]), gy = wave_sum_f(fy[u]), gz = wave_sum_f(fz[u]);
+          if (PRELOAD) {
+            gxm = lane == jj + u ? gx : gxm;
+            gym = lane == jj + u ? gy : gym;
+            gzm = lane == jj + u ? gz : gzm;
+          } else if (lane == 0) {
+            const double gs = -2.0 * (double)fr.inv_s;
+            gout(jb + jj, gs * (double)gx, gs * (double)gy, gs * (double)gz);
+          }
+        } else {
+          e_el = fma((double)sq[u], cq[u], e_el);
+          e_vd = fma((double)s12[u], ca[u], e_vd);
+          e_vd = fma(-2.0 * (double)s6[u], cb[u], e_vd);
         }
-      } else {
-        e_el = fma((double)sq, cq, e_el);
-        e_vd = fma((double)s12, ca, e_vd);
-        e_vd = fma(-2.0 * (double)s6, cb, e_vd);
       }
     }
     if (GRAD && PRELOAD && jb + lane < n_atoms) {

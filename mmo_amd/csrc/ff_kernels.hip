@@ -344,6 +344,9 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
 }
 
 // ----------------------------------------------------- inter kernel, pose-culled
+// ligand atoms per pass of the per-atom sweep
+constexpr int kPoseSweepAtoms = 2;
+
 // Shifted form for arbitrary poses (batches, gradients, minimisation): one wavefront per
 // pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep) or per atom
 // (atom_sweep), so the cost follows the pairs inside the cutoff wherever the poses are.
@@ -378,7 +381,8 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
       g[0] += gx; g[1] += gy; g[2] += gz;
     };
     if (ATOMWISE)
-      atom_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
+      atom_sweep<GRAD, decltype(atom)&, decltype(gout)&, true, true, kPoseSweepAtoms>(
+          cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, gout, e_el, e_vd);
     else
       pose_sweep<GRAD>(cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, gout, e_el, e_vd);
     (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom

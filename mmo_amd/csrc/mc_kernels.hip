@@ -33,6 +33,8 @@ namespace mmo {
 // the chains' atom sweep fetches its ligand atoms 64 at a time (one per lane) and
 // broadcasts them, rather than one dependent fetch per atom
 constexpr bool kMcPreload = true;
+// ligand atoms per pass of the chains' per-atom sweep
+constexpr int kMcSweepAtoms = 2;
 
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC 2011).
@@ -155,7 +157,7 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
     if (INTER == kMcInterAtom)
       // the chains skip the residual correction of pairs closer than 0.5 A: it changes
       // only energies above 1e7 kcal/mol (by up to ~2e-5 relative)
-      atom_sweep<false, decltype(atom)&, decltype(none)&, false, kMcPreload>(a.cg, s_cell, s_pos, s_ab, a.rres, fr,
+      atom_sweep<false, decltype(atom)&, decltype(none)&, false, kMcPreload, kMcSweepAtoms>(a.cg, s_cell, s_pos, s_ab, a.rres, fr,
                                                                li.n_atoms, wl, atom, none, e_el, e_vd);
     else
       pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, none, e_el, e_vd);
