@@ -495,7 +495,7 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
                                            const RecRes* __restrict__ rres, const Frame& fr,
                                            int n_atoms, int* wl, AtomFn&& atom, GradOut&& gout,
                                            double& e_el, double& e_vd) {
-  static_assert(NA == 1 || (NA == 2 && PRELOAD), "two atoms per pass need the preload");
+  static_assert(NA == 1 || (NA >= 2 && NA <= 4 && PRELOAD), "several atoms per pass need the preload");
   const PairK pk = pair_k(fr.inv_s2, fr.inv_s, rres);
   const int lane = threadIdx.x & (kWave - 1);
   int* s_r0 = wl;                 // compacted rows: first receptor atom
@@ -506,10 +506,11 @@ __device__ __forceinline__ void atom_sweep(const CellGrid& cg, const int* __rest
   constexpr float R2 = R * R;
   // PRELOAD: the atoms of a 64-atom chunk are fetched together, one per lane, and
   // broadcast one at a time; in GRAD the lane of an atom keeps its gradient until the
-  // chunk ends, so no atom waits on its own load or gradient write.  NA = 2: two atoms
-  // per pass share one row listing (the rows of the union of their cutoff spheres, each
-  // row's x range the hull of the two) and every receptor atom read is paired with both;
-  // an odd last atom is paired with itself and its copy is dropped.
+  // chunk ends, so no atom waits on its own load or gradient write.  NA > 1: NA
+  // consecutive atoms per pass share one row listing (the rows of the union of their
+  // cutoff spheres, each row's x range the hull of theirs) and every receptor atom read
+  // is paired with each of them; a short last group repeats its first atom, whose copies
+  // are dropped.
   for (int jb = 0; jb < n_atoms; jb += (PRELOAD ? kWave : 1)) {
     LigPt mine = lig_pt_zero();
     double mq = 0.0, ma = 0.0, mb = 0.0;

@@ -345,7 +345,7 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
 
 // ----------------------------------------------------- inter kernel, pose-culled
 // ligand atoms per pass of the per-atom sweep
-constexpr int kPoseSweepAtoms = 2;
+constexpr int kPoseSweepAtoms = 4;
 
 // Shifted form for arbitrary poses (batches, gradients, minimisation): one wavefront per
 // pose; the receptor is swept per 8-atom sub-block of the pose (pose_sweep) or per atom

@@ -34,7 +34,7 @@ namespace mmo {
 // broadcasts them, rather than one dependent fetch per atom
 constexpr bool kMcPreload = true;
 // ligand atoms per pass of the chains' per-atom sweep
-constexpr int kMcSweepAtoms = 2;
+constexpr int kMcSweepAtoms = 4;
 
 // ------------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3" (SC 2011).
