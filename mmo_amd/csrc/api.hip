@@ -63,6 +63,7 @@ struct LigHost {
   std::vector<int> anum, type;
   std::vector<uint32_t> pair_ij;
   std::vector<int> rb_left, rb_right, rg_off, rg_atoms;
+  std::vector<int> sweep;   // order in which the per-atom sweeps take the atoms
 };
 
 }  // namespace
@@ -114,6 +115,7 @@ struct mmo_ctx {
   DevBuf<double> d_lig_xyz;
   DevBuf<uint32_t> d_pair_ij;
   DevBuf<double4> d_pair_par;
+  DevBuf<int> d_sweep;          // per ligand at atom_base: the sweeps' atom order
   DevBuf<int> d_rb_left, d_rb_right, d_rg_off, d_rg_atoms;
   // scratch for the plain (one ligand) forms
   DevBuf<double> s_xyz, s_e, s_g, s_part;
@@ -223,6 +225,41 @@ int upload(mmo_ctx* c, DevBuf<T>& b, const T* src, size_t n, hipStream_t s) {
   return MMO_OK;
 }
 
+// Order of the atoms for the per-atom sweeps, which take consecutive atoms in groups:
+// a depth-first walk of the bond graph (topological distance 1) that lists each atom's
+// terminal neighbours (hydrogens, halogens) right after it, so that a group of
+// consecutive atoms is a compact piece of the molecule.  Fragments follow one another.
+std::vector<int> sweep_order(const int32_t* dists, int n) {
+  std::vector<std::vector<int>> nb(n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (i != j && dists[i + (size_t)j * n] == 1) nb[i].push_back(j);
+  std::vector<int> order;
+  std::vector<char> seen(n, 0);
+  std::vector<int> stack;
+  for (int root = 0; root < n; ++root) {
+    if (seen[root]) continue;
+    seen[root] = 1;
+    stack.push_back(root);
+    while (!stack.empty()) {
+      const int a = stack.back();
+      stack.pop_back();
+      order.push_back(a);
+      for (int b : nb[a])   // terminal neighbours now
+        if (!seen[b] && nb[b].size() <= 1) {
+          seen[b] = 1;
+          order.push_back(b);
+        }
+      for (auto it = nb[a].rbegin(); it != nb[a].rend(); ++it)   // branches next, lowest first
+        if (!seen[*it]) {
+          seen[*it] = 1;
+          stack.push_back(*it);
+        }
+    }
+  }
+  return order;
+}
+
 std::vector<uint32_t> intra_pairs(const int32_t* dists, int n) {
   // src/mol.ml:203-208 (interacting = dists[i + j*n] >= 3), pairs i < j (src/mol.ml:885-890).
   // Listed by diagonal (d = j - i, then i) rather than by row: consecutive entries then
@@ -280,7 +317,7 @@ int sync_ligands(mmo_ctx* c) {
   std::vector<double> xyz;
   std::vector<uint32_t> pij;
   std::vector<double4> ppar;
-  std::vector<int> rbl, rbr, rgo, rga;
+  std::vector<int> rbl, rbr, rgo, rga, swp;
   c->max_atoms = 0;
   for (const LigHost& h : c->ligs) {
     LigInfo li{};
@@ -300,6 +337,7 @@ int sync_ligands(mmo_ctx* c) {
       atoms.push_back(la);
     }
     xyz.insert(xyz.end(), h.xyz.begin(), h.xyz.end());
+    swp.insert(swp.end(), h.sweep.begin(), h.sweep.end());
     std::vector<double4> pp;
     int r = pair_params(c, h.pair_ij, h.q.data(), h.anum.data(), pp);
     if (r) return r;
@@ -324,6 +362,7 @@ int sync_ligands(mmo_ctx* c) {
   if ((r = upload(c, c->d_rb_right, rbr.data(), rbr.size(), s))) return r;
   if ((r = upload(c, c->d_rg_off, rgo.data(), rgo.size(), s))) return r;
   if ((r = upload(c, c->d_rg_atoms, rga.data(), rga.size(), s))) return r;
+  if ((r = upload(c, c->d_sweep, swp.data(), swp.size(), s))) return r;
   HIPCHK(c, hipStreamSynchronize(s));
   c->lig_dirty = false;
   return MMO_OK;
@@ -480,7 +519,7 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     c->rpos.release(); c->rab.release(); c->rres.release(); c->grid.release();
     c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
-    c->d_pair_ij.release(); c->d_pair_par.release();
+    c->d_pair_ij.release(); c->d_pair_par.release(); c->d_sweep.release();
     c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
     c->s_xyz.release(); c->s_e.release(); c->s_g.release(); c->s_part.release();
     c->s_lig.release(); c->s_latom.release(); c->s_pij.release(); c->s_ppar.release();
@@ -868,6 +907,7 @@ int mmo_ligand_add(mmo_ctx* c, const double* xs, const double* ys, const double*
     h.type.assign(types, types + n);
   }
   h.pair_ij = intra_pairs(dists, n);
+  h.sweep = sweep_order(dists, n);
   for (int k = 0; k < nrb; ++k) {
     if (rb_left[k] < 0 || rb_left[k] >= n || rb_right[k] < 0 || rb_right[k] >= n ||
         rb_left[k] == rb_right[k])
@@ -1035,7 +1075,8 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     PoseCullArgs a{};
     a.xyz = xyz; a.atom_off = b->d_atom_off.p; a.pose_lig = b->d_pose_lig.p; a.npose = b->npose;
     a.rres = c->rres.p;
-    a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.e_out = e_out; a.e_stride = 4;
+    a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.atom_order = c->d_sweep.p;
+    a.e_out = e_out; a.e_stride = 4;
     const bool comp = (flags & MMO_F_COMPONENTS) != 0;
     const int waves = pose_cull_waves(c, b->npose);
     if ((r = deal_batch(b, waves, s))) return r;
@@ -1452,7 +1493,7 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
 static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   mmo_ctx* c = m->ctx;
   McArgs a{};
-  a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.rb_left = c->d_rb_left.p;
+  a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.atom_order = c->d_sweep.p; a.rb_left = c->d_rb_left.p;
   a.rb_right = c->d_rb_right.p; a.rg_off = c->d_rg_off.p; a.rg_atoms = c->d_rg_atoms.p;
   a.pair_ij = c->d_pair_ij.p; a.pair_par = c->d_pair_par.p; a.max_atoms = m->max_atoms;
   a.nwork = nwork; a.work = work; a.chain_lig = m->d_chain_lig.p; a.chain_atom = m->d_chain_atom.p;

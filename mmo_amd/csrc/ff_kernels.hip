@@ -371,13 +371,16 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
     const double* x = a.xyz + 3 * row0;
     int sat = 0;
     double e_el = 0.0, e_vd = 0.0;
+    // sweep position j -> ligand atom (the per-atom sweep's order, if given)
+    auto at = [&](int j) { return (ATOMWISE && a.atom_order) ? a.atom_order[li.atom_base + j] : j; };
     auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
-      l = lig_pt(x[3 * j], x[3 * j + 1], x[3 * j + 2], fr, &sat);
-      const LigAtom la = a.latoms[li.atom_base + j];
+      const int ja = at(j);
+      l = lig_pt(x[3 * ja], x[3 * ja + 1], x[3 * ja + 2], fr, &sat);
+      const LigAtom la = a.latoms[li.atom_base + ja];
       cq = la.keq; ca = la.a; cb = la.b;
     };
     auto gout = [&](int j, double gx, double gy, double gz) {
-      double* g = a.grad + 3 * (row0 + j);
+      double* g = a.grad + 3 * (row0 + at(j));
       g[0] += gx; g[1] += gy; g[2] += gz;
     };
     if (ATOMWISE)

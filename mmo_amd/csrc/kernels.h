@@ -133,6 +133,9 @@ struct PoseCullArgs {
   // nullable: the pose of each launch slot (slot = workgroup * waves + wavefront), a
   // permutation of 0..npose-1
   const int* order;
+  // nullable: per ligand at atom_base, the order in which the per-atom sweep takes the
+  // atoms (a permutation of 0..n_atoms-1)
+  const int* atom_order;
 };
 // atomwise: lanes over the receptor atoms near each ligand atom (atom_sweep) instead of
 // 8-atom sub-blocks (pose_sweep)

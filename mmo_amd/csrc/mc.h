@@ -44,6 +44,7 @@ struct McArgs {
   // ligands
   const LigInfo* ligs;
   const LigAtom* latoms;
+  const int* atom_order;        // nullable: per ligand at atom_base, the sweep's atom order
   const int* rb_left;
   const int* rb_right;
   const int* rg_off;

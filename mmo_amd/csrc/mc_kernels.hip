@@ -147,10 +147,12 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
   if (INTER != kMcInterF64) {
     double e_el = 0.0, e_vd = 0.0;
     auto atom = [&](int j, LigPt& l, double& cq, double& ca, double& cb) {
+      // sweep position j -> ligand atom (the per-atom sweep's order, if given)
+      const int ja = (INTER == kMcInterAtom && a.atom_order) ? a.atom_order[li.atom_base + j] : j;
       double x, y, z;
-      v.at(j, x, y, z);
+      v.at(ja, x, y, z);
       l = lig_pt(x, y, z, fr, &sat);
-      const LigAtom la = a.latoms[li.atom_base + j];
+      const LigAtom la = a.latoms[li.atom_base + ja];
       cq = la.keq; ca = la.a; cb = la.b;
     };
     auto none = [](int, double, double, double) {};
