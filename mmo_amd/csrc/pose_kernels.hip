@@ -1,6 +1,6 @@
 // pose_kernels.hip -- DOFs -> coordinates (Optim.apply_config, src/optim.ml:64-80) and
-// its adjoint (dE/dxyz -> dE/dDOF), one workgroup per pose (one wavefront; four when the
-// adjoint also evaluates E_intra), FP64 throughout.
+// its adjoint (dE/dxyz -> dE/dDOF), one workgroup per pose (one wavefront; kDofWaves
+// when the adjoint also evaluates E_intra), FP64 throughout.
 //
 // Forward: the registered centred conformer c0; for k = 0..R-1 the atoms of rotation
 // group k turn by tau_k about the axis right_k - left_k through right_k
@@ -26,7 +26,7 @@ namespace mmo {
 struct Rod { double m[9]; };
 
 // wavefronts per pose of the adjoint when it also evaluates E_intra
-constexpr int kDofWaves = 4;
+constexpr int kDofWaves = 2;
 
 // LDS of the DOF kernels: `arrays` FP64 coordinate arrays of max_atoms entries, then the
 // staged torsion tree (TorsionLds)
