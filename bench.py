@@ -455,8 +455,9 @@ def main():
     if tuned is None and not args.autotune:
         log("no recorded tuning for %s: library defaults" % scan_key)
     tune = dict(kv.split("=") for kv in args.tune.split(",") if kv)
-    if "cell_size" in tune:
-        ctx.tune(cell_size=float(tune.pop("cell_size")))
+    for key in ("cell_size", "cell_xsub"):   # cell grids are built by set_receptor
+        if key in tune:
+            ctx.tune(**{key: float(tune.pop(key))})
     ctx.set_receptor(rec)
     ctx.tune(**{k: float(v) for k, v in tune.items()})
     lid = ctx.add_ligand(lig)
