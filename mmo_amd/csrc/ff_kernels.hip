@@ -541,14 +541,14 @@ __global__ void __launch_bounds__(kIntraWaves * kWave) k_intra(IntraArgs a) {
     }
     const int i = ij & 0xffff, j = ij >> 16;
     const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-    const double r = rr < 0.01 ? 0.01 : rr;   // Math.non_zero_dist
-    const double ir = 1.0 / r;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const bool apart = r2 >= 1.0e-4;              // r >= 0.01 (Math.non_zero_dist)
+    const double ir = apart ? rsqrt(r2) : 100.0;  // 1 / max(r, 0.01)
     const double u = ir * ir;
     const double u3 = u * u * u;
     const double u6 = u3 * u3;
     e += pp.x * ir + pp.y * u6 - pp.z * u3;
-    if (GRAD && rr >= 0.01) {
+    if (GRAD && apart) {
       // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
       const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
       atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);

@@ -209,9 +209,9 @@ __device__ double mc_intra(const McArgs& a, const LigInfo& li, const double* sx,
     const int i = ij & 0xffff, j = ij >> 16;
     const double4 pp = a.pair_par[li.pair_base + p];
     const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-    const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-    const double r = rr < 0.01 ? 0.01 : rr;
-    const double ir = 1.0 / r, u = ir * ir, u3 = u * u * u, u6 = u3 * u3;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ir = r2 >= 1.0e-4 ? rsqrt(r2) : 100.0;   // 1 / max(r, 0.01)
+    const double u = ir * ir, u3 = u * u * u, u6 = u3 * u3;
     e += pp.x * ir + pp.y * u6 - pp.z * u3;
   }
   return wave_sum(e);
