@@ -155,7 +155,7 @@ struct mmo_mc {
   std::vector<std::pair<int, int>> groups;   // (first, count) in work_by_lig per ligand
   DevBuf<int> d_chain_lig, d_chain_atom, d_chain_rb, d_work, d_work_by_lig, d_queue;
   DevBuf<McState> d_st;
-  DevBuf<double> d_conf, d_best, d_dr;
+  DevBuf<double> d_conf, d_best, d_dr, d_undo;
   DevBuf<SwState> d_sw;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1474,6 +1474,7 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
       (r = upload(c, m->d_st, st.data(), st.size(), s)) ||
       (r = upload(c, m->d_conf, conf.data(), conf.size(), s)) ||
       (r = upload(c, m->d_best, conf.data(), conf.size(), s)) ||
+      (r = upload(c, m->d_undo, conf.data(), conf.size(), s)) ||
       (r = upload(c, m->d_dr, dr.data(), dr.size(), s)) ||
       (r = upload(c, m->d_sw, sw.data(), sw.size(), s))) {
     delete m;
@@ -1498,6 +1499,7 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.pair_ij = c->d_pair_ij.p; a.pair_par = c->d_pair_par.p; a.max_atoms = m->max_atoms;
   a.nwork = nwork; a.work = work; a.chain_lig = m->d_chain_lig.p; a.chain_atom = m->d_chain_atom.p;
   a.chain_rb = m->d_chain_rb.p; a.st = m->d_st.p; a.conf = m->d_conf.p; a.best_xyz = m->d_best.p;
+  a.undo = m->d_undo.p;
   a.rb_dr = m->d_dr.p; a.rb_sw = m->d_sw.p; a.queue = m->d_queue.p;
   a.nsteps = nsteps; a.total_frames = m->total_frames; a.flags = m->flags;
   a.seed_lo = (uint32_t)m->seed; a.seed_hi = (uint32_t)(m->seed >> 32);
@@ -1659,7 +1661,7 @@ void mmo_mc_destroy(mmo_mc* m) {
     if (m->graph) (void)hipGraphDestroy(m->graph);
     m->d_chain_lig.release(); m->d_chain_atom.release(); m->d_chain_rb.release();
     m->d_work.release(); m->d_work_by_lig.release(); m->d_queue.release(); m->d_st.release();
-    m->d_conf.release(); m->d_best.release(); m->d_dr.release(); m->d_sw.release();
+    m->d_conf.release(); m->d_best.release(); m->d_dr.release(); m->d_undo.release(); m->d_sw.release();
   }
   delete m;
 }

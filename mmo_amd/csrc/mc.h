@@ -61,6 +61,7 @@ struct McArgs {
   McState* st;
   double* conf;              // current conformers (rows)
   double* best_xyz;          // best poses (rows)
+  double* undo;              // per chain, like conf: a torsion proposal's moved atoms
   double* rb_dr;             // per chain and rbond: torsion step
   SwState* rb_sw;            // per chain and rbond: acceptance window
   int* queue;

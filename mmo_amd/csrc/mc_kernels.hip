@@ -8,7 +8,8 @@
 // Per frame (src/lds.ml:882-995):
 //   even frames: rigid move, rot' = R_axis(theta) rot (Move.rand_rot src/move.ml:20-31,
 //     passive axis matrices src/rot.ml:22-46), pos' = pos + dt U(-1,1)^3 (src/move.ml:46-54);
-//   odd frames: torsion move on a copy of the conformer, tweak U(-dr_i, dr_i) or, every
+//   odd frames: torsion move of the conformer (turned in place, the moved atoms restored
+//     from the chain's undo rows unless the move is kept), tweak U(-dr_i, dr_i) or, every
 //     100th conformer step, flip U(-pi, pi) (src/lds.ml:781-798, src/mol.ml:635-647);
 //     enclosing radius > 12 A (Mol.check_elongation_exn) ends the chain (the exception
 //     leaves the frame loop, src/lds.ml:996-998);
@@ -255,14 +256,13 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
   load_receptor<REC_LDS>(a.rpos, a.rab, a.nrec, a.cg, rv.pos, rv.ab, rv.cell, mc_lds);
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
-  double* cx_ = reinterpret_cast<double*>(mc_lds + rec_bytes) + (size_t)wv * 6 * a.max_atoms;
-  double* cy_ = cx_ + a.max_atoms;                // current conformer
+  // the chain's conformer; a torsion proposal turns it in place, with the moved atoms'
+  // previous coordinates in the chain's undo rows (HBM) until the decision
+  double* cx_ = reinterpret_cast<double*>(mc_lds + rec_bytes) + (size_t)wv * 3 * a.max_atoms;
+  double* cy_ = cx_ + a.max_atoms;
   double* cz_ = cy_ + a.max_atoms;
-  double* px_ = cz_ + a.max_atoms;                // proposed conformer
-  double* py_ = px_ + a.max_atoms;
-  double* pz_ = py_ + a.max_atoms;
   McState* s_st = reinterpret_cast<McState*>(
-      mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 6 * a.max_atoms * sizeof(double) + 15) &
+      mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 3 * a.max_atoms * sizeof(double) + 15) &
                             ~(size_t)15));
   rv.wl = reinterpret_cast<int*>(s_st + blockDim.x / kWave) + wv * kAtomSweepInts;
   const double pi = 3.14159265358979311600;       // 4 atan 1 (src/math.ml:13)
@@ -287,6 +287,17 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
     const int L = li.n_atoms, nrb = li.nrb;
     double* gconf = a.conf + 3 * (size_t)a.chain_atom[chain];
     double* gbest = a.best_xyz + 3 * (size_t)a.chain_atom[chain];
+    double* gundo = a.undo + 3 * (size_t)a.chain_atom[chain];
+    // group [g0, g1) of a torsion back to the coordinates saved in the undo rows (each
+    // lane restores the atoms it saved)
+    auto restore = [&](int g0, int g1) {
+      for (int m = g0 + lane; m < g1; m += kWave) {
+        const int at = a.rg_atoms[m];
+        const double* u = gundo + 3 * (size_t)(m - g0);
+        cx_[at] = u[0]; cy_[at] = u[1]; cz_[at] = u[2];
+      }
+      wave_sync();
+    };
     double* dr = a.rb_dr + a.chain_rb[chain];
     SwState* rsw = a.rb_sw + a.chain_rb[chain];
     for (int j = lane; j < L; j += kWave) {
@@ -332,6 +343,7 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
       const double* sy = cy_;
       const double* sz = cz_;
       int just_rotated = -1;
+      int mg0 = 0, mg1 = 0;   // torsion group turned in place by this proposal
       ccx = st.center[0]; ccy = st.center[1]; ccz = st.center[2];
       if (!rigid) {
         for (int k = 0; k < 9; ++k) nrot[k] = st.rot[k];
@@ -343,11 +355,9 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
           const bool flip = st.conf_step > 0 && st.conf_step % kMcBlock == 0 && !(a.flags & kMcNoFlip);
           const double amp = flip ? pi : dr[i];
           const double ang = u1 * (2.0 * amp) - amp;   // Move.rand_angle, src/move.ml:14-16
-          for (int j = lane; j < L; j += kWave) { px_[j] = cx_[j]; py_[j] = cy_[j]; pz_[j] = cz_[j]; }
-          wave_sync();
           const int left = a.rb_left[li.rb_base + i], right = a.rb_right[li.rb_base + i];
-          const double ox = px_[right], oy = py_[right], oz = pz_[right];
-          const double vx = ox - px_[left], vy = oy - py_[left], vz = oz - pz_[left];
+          const double ox = cx_[right], oy = cy_[right], oz = cz_[right];
+          const double vx = ox - cx_[left], vy = oy - cy_[left], vz = oz - cz_[left];
           const double mag = sqrt(vx * vx + vy * vy + vz * vz);
           double Rm[9];
           axis_angle(vx / mag, vy / mag, vz / mag, ang, Rm);
@@ -355,27 +365,32 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
           const int g0 = a.rg_off[li.rg_base + i], g1 = a.rg_off[li.rg_base + i + 1];
           for (int m = g0 + lane; m < g1; m += kWave) {
             const int at = a.rg_atoms[m];
-            const double zx = px_[at] - ox, zy = py_[at] - oy, zz = pz_[at] - oz;
-            px_[at] = (Rm[0] * zx + Rm[1] * zy + Rm[2] * zz) + ox;
-            py_[at] = (Rm[3] * zx + Rm[4] * zy + Rm[5] * zz) + oy;
-            pz_[at] = (Rm[6] * zx + Rm[7] * zy + Rm[8] * zz) + oz;
+            const double x0 = cx_[at], y0 = cy_[at], z0 = cz_[at];
+            double* u = gundo + 3 * (size_t)(m - g0);
+            u[0] = x0; u[1] = y0; u[2] = z0;
+            const double zx = x0 - ox, zy = y0 - oy, zz = z0 - oz;
+            cx_[at] = (Rm[0] * zx + Rm[1] * zy + Rm[2] * zz) + ox;
+            cy_[at] = (Rm[3] * zx + Rm[4] * zy + Rm[5] * zz) + oy;
+            cz_[at] = (Rm[6] * zx + Rm[7] * zy + Rm[8] * zz) + oz;
           }
           wave_sync();
           // update_center (A.favg) and enclosing radius (src/mol.ml:353-356, :576-591)
           double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-          for (int j = lane; j < L; j += kWave) { s0 += px_[j]; s1 += py_[j]; s2 += pz_[j]; }
+          for (int j = lane; j < L; j += kWave) { s0 += cx_[j]; s1 += cy_[j]; s2 += cz_[j]; }
           ccx = wave_sum(s0) / L; ccy = wave_sum(s1) / L; ccz = wave_sum(s2) / L;
           double rmax = 0.0;
           for (int j = lane; j < L; j += kWave) {
-            const double dx = ccx - px_[j], dy = ccy - py_[j], dz = ccz - pz_[j];
+            const double dx = ccx - cx_[j], dy = ccy - cy_[j], dz = ccz - cz_[j];
             rmax = fmax(rmax, 0.01 + sqrt(dx * dx + dy * dy + dz * dz));
           }
           rmax = wave_max(rmax);
           if (rmax > 12.0) {   // Mol.Too_long: the chain ends (src/lds.ml:794, :997)
+            restore(g0, g1);   // with its last accepted conformer
             st.status = kMcTooLong;
             break;
           }
-          sx = px_; sy = py_; sz = pz_;
+          mg0 = g0;
+          mg1 = g1;
           just_rotated = i;
         }
       } else {
@@ -445,12 +460,9 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
           for (int k = 0; k < 9; ++k) st.rot[k] = nrot[k];
           for (int k = 0; k < 3; ++k) st.pos[k] = npos[k];
           st.prev_e = curr_e;
-          if (sx != cx_) {   // conf := conf'
-            for (int j = lane; j < L; j += kWave) { cx_[j] = px_[j]; cy_[j] = py_[j]; cz_[j] = pz_[j]; }
+          if (mg1 > mg0) {   // conf := conf' (already in place): keep it
             st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
-            wave_sync();
-            sx = cx_; sy = cy_; sz = cz_;
-            v.sx = cx_; v.sy = cy_; v.sz = cz_;
+            mg0 = mg1 = 0;
           }
         }
         if (curr_e < st.best_e) {   // best pose, accepted or not (src/lds.ml:959-969)
@@ -479,6 +491,7 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
           }
         }
       }
+      if (mg1 > mg0) restore(mg0, mg1);   // the proposal was not taken: conf stays
       if (rigid) ++st.rigid_step;
       else ++st.conf_step;
     }
@@ -506,7 +519,7 @@ size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
   if (rec_lds)
     rec = ((size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
            (size_t)(a.cg.nx * a.cg.ny * a.cg.nz + 1) * sizeof(int) + 15) & ~(size_t)15;
-  const size_t conf = ((size_t)waves * 6 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
+  const size_t conf = ((size_t)waves * 3 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
   return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int));
 }
 
