@@ -243,11 +243,11 @@ __device__ __forceinline__ void wave_sync() {
 // A workgroup of `waves` wavefronts, each running its own chains; with REC_LDS the
 // workgroup first stages the receptor and its cell table in LDS, shared by all chains.
 // one instantiation per energy form, so that each carries only its own registers
-// Wavefronts per workgroup each form is compiled for.  16 caps the direct forms at 128
-// VGPRs (4 wavefronts per SIMD; the chain's control code then spills ~276 B per lane to
-// scratch), measured 15.4 M against 14.2 M steps/s for 8 (202 VGPRs, 2 per SIMD); the
-// grid form keeps 8.
-__host__ __device__ constexpr int mc_form_waves(int inter) { return inter == kMcInterGrid ? 8 : 16; }
+// Wavefronts per workgroup each form is compiled for.  16 caps every form at 128 VGPRs
+// (4 wavefronts per SIMD; the chain's control code then spills ~260-340 B per lane to
+// scratch): measured faster than 8 (about 200 VGPRs, 2 per SIMD) for the direct forms
+// (15.4 M against 14.2 M steps/s) and for the grid form (225 M against 203 M).
+__host__ __device__ constexpr int mc_form_waves(int) { return 16; }
 
 template <bool REC_LDS, int INTER>
 __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int rec_bytes) {
