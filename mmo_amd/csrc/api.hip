@@ -1063,8 +1063,10 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
       for (int t : h.type)
         if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
   }
-  // gradient buffer starts at zero; every producer adds
-  if (want_grad) HIPCHK(c, launch_fill(grad, 3 * (size_t)b->natoms, 0.0, s));
+  // every gradient producer adds to the buffer, except the pose kernel: when it runs it
+  // is the first producer and writes each atom's row once, so the buffer is not cleared
+  const bool pose_first = want_grad && (inter & MMO_F_INTER_SHIFTED) && c->pose_cull;
+  if (want_grad && !pose_first) HIPCHK(c, launch_fill(grad, 3 * (size_t)b->natoms, 0.0, s));
   HIPCHK(c, launch_fill(e_out, 4 * (size_t)b->npose, 0.0, s));
   if ((inter & MMO_F_INTER_SHIFTED) && c->pose_cull) {
     // one wavefront per pose over the culled receptor (per-atom or 8-atom sub-block sweep)
@@ -1089,6 +1091,7 @@ static int batch_score(mmo_batch* b, int flags, const double* xyz, double* e_out
     if (want_grad) {
       a.e_comp = nullptr;
       a.grad = grad;
+      a.grad_assign = 1;
       a.write_e = comp ? 0 : 1;
       HIPCHK(c, launch_pose_cull(L, a, true, waves, c->pose_sweep == 1, s));
     }

@@ -381,7 +381,11 @@ __global__ void __launch_bounds__(1024) k_pose_cull(PoseCullArgs a, Frame fr, co
     };
     auto gout = [&](int j, double gx, double gy, double gz) {
       double* g = a.grad + 3 * (row0 + at(j));
-      g[0] += gx; g[1] += gy; g[2] += gz;
+      if (a.grad_assign) {
+        g[0] = gx; g[1] = gy; g[2] = gz;
+      } else {
+        g[0] += gx; g[1] += gy; g[2] += gz;
+      }
     };
     if (ATOMWISE)
       atom_sweep<GRAD, decltype(atom)&, decltype(gout)&, true, true, kPoseSweepAtoms>(

@@ -128,6 +128,7 @@ struct PoseCullArgs {
   int e_stride;
   double* e_comp;           // nullable: [pose*e_stride + 2] elec, [+3] vdW
   double* grad;             // GRAD: dE/dxyz rows, added to what the buffer holds
+  int grad_assign;          // GRAD: write the rows instead (each atom's row is written once)
   int write_e;
   const RecRes* rres;       // receptor quantisation residuals
   // nullable: the pose of each launch slot (slot = workgroup * waves + wavefront), a
