@@ -173,7 +173,7 @@ struct mmo_batch {
   int order_waves = 0;
   DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm, d_order;
   DevBuf<Tile> d_tiles;
-  DevBuf<double> d_xyz, d_e, d_g, d_part;
+  DevBuf<double> d_xyz, d_e, d_g, d_part, d_conf;
 };
 
 namespace {
@@ -1027,6 +1027,7 @@ void mmo_batch_destroy(mmo_batch* b) {
     b->d_pose_lig.release(); b->d_atom_off.release(); b->d_dof_off.release(); b->d_perm.release();
     b->d_order.release();
     b->d_tiles.release(); b->d_xyz.release(); b->d_e.release(); b->d_g.release(); b->d_part.release();
+    b->d_conf.release();
   }
   delete b;
 }
@@ -1200,6 +1201,10 @@ int mmo_batch_score_dofs_dev(mmo_batch* b, int flags, const double* dofs, double
   a.npose = b->npose; a.max_atoms = b->max_atoms; a.max_rb = b->max_rb; a.max_rg = b->max_rg;
   a.dofs = dofs; a.xyz_out = xyz;
   a.radius_out = radius_out;
+  if (g) {   // the adjoint reads the conformers apply_config leaves
+    HIPCHK(c, b->d_conf.ensure(3 * (size_t)b->natoms));
+    a.conf = b->d_conf.p;
+  }
   if (dof_lds_bytes(a, g) > kMaxLds)
     return fail(c, MMO_ERR_BAD_ARG,
                 "ligands too large for the DOF kernels: %zu bytes of LDS per pose (limit %zu)",

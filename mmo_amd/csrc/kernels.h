@@ -88,6 +88,8 @@ struct DofArgs {
   const double* dofs;
   double* xyz_out;
   double* radius_out;           // nullable
+  double* conf;                 // nullable: the posed conformers before the rigid-body motion
+                                // (rows), written by apply_config and read by the adjoint
   const double* grad_xyz;       // dE/dxyz (for the adjoint)
   double* gdof;                 // dE/dDOF
   // nullable: the adjoint also evaluates E_intra and its gradient (pair list below) and

@@ -208,6 +208,11 @@ __global__ void __launch_bounds__(64) k_apply_config(DofArgs a) {
   double M[9];
   rot_xyz(t.sn + li.nrb, t.cs + li.nrb, M, nullptr, nullptr, nullptr);
   const int base = a.atom_off[pose];
+  if (a.conf)
+    for (int j = lane; j < L; j += kWave) {
+      double* c = a.conf + 3 * (size_t)(base + j);
+      c[0] = sx[j]; c[1] = sy[j]; c[2] = sz[j];
+    }
   for (int j = lane; j < L; j += kWave) {
     double* o = a.xyz_out + 3 * (size_t)(base + j);
     o[0] = (M[0] * sx[j] + M[1] * sy[j] + M[2] * sz[j]) + d[0];
@@ -240,10 +245,20 @@ __global__ void __launch_bounds__(INTRA ? kDofWaves * kWave : kWave) k_dof_grad(
   double* gd = a.gdof + a.dof_off[pose];
   const TorsionLds t =
       stage_torsions(a, li, d, smem + dof_coord_bytes(a.max_atoms, INTRA ? 6 + 3 * kDofWaves : 6));
-  load_conformer(a, li, sx, sy, sz);
   if (INTRA)
     for (int j = tid; j < 3 * L * kDofWaves; j += blockDim.x) gi[j] = 0.0;
-  forward_torsions(li, t, sx, sy, sz);   // ends with a barrier
+  if (a.conf) {
+    // the conformer apply_config left after the torsions
+    const int base0 = a.atom_off[pose];
+    for (int j = tid; j < L; j += blockDim.x) {
+      const double* c = a.conf + 3 * (size_t)(base0 + j);
+      sx[j] = c[0]; sy[j] = c[1]; sz[j] = c[2];
+    }
+    __syncthreads();
+  } else {
+    load_conformer(a, li, sx, sy, sz);
+    forward_torsions(li, t, sx, sy, sz);   // ends with a barrier
+  }
   if (INTRA) {
     double* gx = gi + 3 * L * (tid / kWave);
     double* gy = gx + L;
