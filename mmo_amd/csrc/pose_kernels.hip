@@ -117,9 +117,9 @@ __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double*
     __syncthreads();
     const int left = t.left[k], right = t.right[k];
     const double px = sx[right], py = sy[right], pz = sz[right];
-    double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
-    const double mag = sqrt(vx * vx + vy * vy + vz * vz);   // V3.normalize
-    const Rod R = rodrigues(vx / mag, vy / mag, vz / mag, t.sn[k], t.cs[k]);
+    const double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
+    const double im = rsqrt(vx * vx + vy * vy + vz * vz);   // V3.normalize: v / |v|
+    const Rod R = rodrigues(vx * im, vy * im, vz * im, t.sn[k], t.cs[k]);
     const int g0 = t.off[k], g1 = t.off[k + 1];
     __syncthreads();
     for (int m = g0 + tid; m < g1; m += nt) {
@@ -310,8 +310,8 @@ __global__ void __launch_bounds__(INTRA ? kDofWaves * kWave : kWave) k_dof_grad(
     const int left = t.left[k], right = t.right[k];
     const double px = sx[right], py = sy[right], pz = sz[right];
     const double vx = px - sx[left], vy = py - sy[left], vz = pz - sz[left];
-    const double mag = sqrt(vx * vx + vy * vy + vz * vz);
-    const double ax = vx / mag, ay = vy / mag, az = vz / mag;
+    const double im = rsqrt(vx * vx + vy * vy + vz * vz);   // 1 / |v|
+    const double ax = vx * im, ay = vy * im, az = vz * im;
     const double sn = t.sn[k], cs = t.cs[k];
     const Rod R = rodrigues(ax, ay, az, sn, cs);
     const double omc = 1.0 - cs;
@@ -345,8 +345,8 @@ __global__ void __launch_bounds__(INTRA ? kDofWaves * kWave : kWave) k_dof_grad(
     if (lane == 0) {
       gd[6 + k] = tb;
       const double adot = ax * abx + ay * aby + az * abz;
-      const double vbx = (abx - ax * adot) / mag, vby = (aby - ay * adot) / mag,
-                   vbz = (abz - az * adot) / mag;
+      const double vbx = (abx - ax * adot) * im, vby = (aby - ay * adot) * im,
+                   vbz = (abz - az * adot) * im;
       hx[right] += vbx + pbx; hy[right] += vby + pby; hz[right] += vbz + pbz;
       hx[left] -= vbx; hy[left] -= vby; hz[left] -= vbz;
     }
