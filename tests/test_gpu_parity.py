@@ -799,9 +799,17 @@ def test_mc_full_chain_count_forms_agree_and_follow_restatement():
         assert np.array_equal(sa[key], sb[key]), key
         assert np.array_equal(sa[key], sc[key]), key
     assert ((sa["frame"] == 40) | (sa["status"] == 2)).all()
-    for k in rng.choice(len(cl), 6, replace=False):
+    # conformers (turned in place by torsion moves, restored unless kept) agree bit for bit
+    ca, cb, cc = a.coords()[0], b.coords()[0], c.coords()[0]
+    assert all(np.array_equal(x, y) and np.array_equal(x, z) for x, y, z in zip(ca, cb, cc))
+    # sampled chains, among them chains that ended on a too-long conformer
+    too_long = np.flatnonzero(sa["status"] == 2)
+    sample = list(rng.choice(len(cl), 6, replace=False)) + list(too_long[:2])
+    for k in sample:
         ref = simulate(lib[ids.index(cl[k])], rec, rot0[k], pos0[k], 40, seed=77, chain=int(k),
                        hard_roi=(np.zeros(3), 6.0))
+        # the last kept conformer, also for a chain that stopped on a too-long proposal
+        assert np.allclose(ca[k], ref["conf"], rtol=0, atol=1e-9), k
         if ref["status"] != 0:
             assert sa["status"][k] == 2
             continue
