@@ -160,6 +160,39 @@ def lib_sys():
     return ctx, ligs, ids, rec, (grids, step, origin)
 
 
+def test_batch_gradient_rows_fully_written(lib_sys):
+    """The pose kernel writes every atom's gradient row (the buffer is not cleared
+    first): a device gradient buffer filled with NaN comes back finite, and a second call
+    on the same batch gives bit for bit what a fresh batch gives."""
+    import torch
+    ctx, ligs, ids, rec, _ = lib_sys
+    rng = np.random.default_rng(12)
+    pose_lig = np.asarray([ids[k % len(ids)] for k in range(37)])
+    b = ctx.batch(pose_lig)
+
+    def poses():
+        x = np.zeros((b.natoms, 3))
+        for p, l in enumerate(pose_lig):
+            _, xyz = _place(ligs[l], synth.random_rotations(rng, 1)[0], rng.uniform(-3, 3, 3))
+            x[b.atom_off[p]:b.atom_off[p + 1]] = xyz
+        return x
+    x1, x2 = poses(), poses()
+    flags = ff.F_INTER_SHIFTED | ff.F_INTRA | ff.F_GRAD
+    print("gradient rows", b.npose, flush=True)
+    dev = torch.device("cuda:0")
+    xt = torch.tensor(x2, device=dev)
+    et = torch.zeros((b.npose, 4), dtype=torch.float64, device=dev)
+    gt = torch.full((b.natoms, 3), float("nan"), dtype=torch.float64, device=dev)
+    b.score_dev(xt, et, grad=gt, flags=flags)
+    torch.cuda.synchronize()
+    assert torch.isfinite(gt).all()
+    b.score(x1, flags)
+    e_again, g_again = b.score(x2, flags)
+    e_fresh, g_fresh = ctx.batch(pose_lig).score(x2, flags)
+    assert np.array_equal(g_again, g_fresh) and np.array_equal(e_again, e_fresh)
+    assert np.array_equal(gt.cpu().numpy(), g_fresh)
+
+
 def test_batch_ragged_poses(lib_sys):
     ctx, ligs, ids, rec, (grids, step, origin) = lib_sys
     rng = np.random.default_rng(1)
