@@ -63,11 +63,11 @@ const char* mmo_last_error(const mmo_ctx* ctx);   /* ctx may be NULL: last globa
 /* library version string, e.g. "mmo_ff 1.0 gfx950" */
 const char* mmo_version(void);
 
-/* UFF per-element nonbond parameters.  Replaces UFF.anum_xi_Di (src/UFF.ml:30-42);
- * pair parameters follow the same geometric-mean rule (src/UFF.ml:62-71).  The
+/* UFF per-element nonbond parameters.  Replaces UFF.anum_xi_Di (src/UFF.ml:10-22);
+ * pair parameters follow the same geometric-mean rule (src/UFF.ml:39-51).  The
  * default table (set at creation) is the reference's: anum 0 (virtual), H, C, N, O,
  * F, Mg, P, S, Cl, Br, I.  Elements outside the table are rejected (the reference
- * reads NaN parameters for them, src/UFF.ml:57). */
+ * reads NaN parameters for them, src/UFF.ml:37). */
 int mmo_set_uff_table(mmo_ctx* ctx, const int32_t* anum, const double* x, const double* d, int n);
 
 /* Receptor (the `prot` Mol.t of src/mol.ml:796-960).  Uploaded once; replaces
@@ -130,7 +130,7 @@ int mmo_ene_inter_interp_grad(mmo_ctx* ctx, const double* xs, const double* ys,
  * centred conformer (Lds.preprocess_ligand, src/lds.ml:45-51), charges, atomic
  * numbers, the dists matrix (dists[i + j*n]), rotatable bonds with the axis
  * left->right and rotation groups rg_atoms[rg_off[k]..rg_off[k+1]) excluding
- * `right` (src/pqrs.ml:204-250), and grid types (nullable).  Returns an id >= 0. */
+ * `right` (src/mol.ml:385-402, src/pqrs.ml:80-87), and grid types (nullable).  Returns an id >= 0. */
 int mmo_ligand_add(mmo_ctx* ctx, const double* xs, const double* ys, const double* zs,
                    const double* q, const int32_t* anum, const int32_t* dists, int n,
                    const int32_t* rb_left, const int32_t* rb_right, const int32_t* rg_off,
@@ -299,7 +299,7 @@ double mmo_last_kernel_ms(mmo_ctx* ctx, const char* kernel);
 /* ------------------------------------------------------------- ligand input
  * Host-only (no GPU needed).  The reference prepares ligands with mol2pqrs and reads
  * them with Pqrs/Mol (src/mol2.ml:99-323, src/mol_graph.ml:53-192, src/mol2pqrs.ml:9-43,
- * src/pqrs.ml:17-90, src/mol.ml:420-440): a .mol2 file is parsed (lone pairs removed),
+ * src/pqrs.ml:17-87, src/mol.ml:420-440): a .mol2 file is parsed (lone pairs removed),
  * its topology derived (BFS topological distances; rotatable bonds: single, both ends of
  * degree > 1, not in a ring; rotation group: the smaller side), written as .pqrs text
  * and read back, so .mol2 and .pqrs inputs give the same molecules.  A molset holds the

@@ -29,7 +29,7 @@ std::string g_last_error;
 
 struct Uff { double x, d; };
 
-// src/UFF.ml:30-42
+// src/UFF.ml:10-22
 const std::map<int, Uff> kDefaultUff = {
     {0, {0.0, 0.0}},     {1, {2.886, 0.044}},  {6, {3.851, 0.105}},  {7, {3.660, 0.069}},
     {8, {3.500, 0.060}}, {9, {3.364, 0.050}},  {12, {3.021, 0.111}}, {15, {4.147, 0.305}},
@@ -1411,7 +1411,7 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
   m->flags = flags;
   m->total_frames = total_frames;
   m->seed = seed;
-  m->beta = 1.0 / (0.0019872041 * temperature);   // src/lds.ml:67-68, src/const.ml:97
+  m->beta = 1.0 / (0.0019872041 * temperature);   // src/lds.ml:66-68, src/const.ml:23-28
   if (roi_center) for (int k = 0; k < 3; ++k) m->roi[k] = roi_center[k];
   m->roi_r = roi_r;
   m->chain_lig.assign(chain_lig, chain_lig + nchain);

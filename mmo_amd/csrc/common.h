@@ -9,7 +9,7 @@
 //   poses     FP64 (x, y, z) rows, pose-major, atom_off[p] = first row of pose p.
 //
 // UFF pair parameters factorise: x_ij = sqrt(x_i x_j), D_ij = sqrt(D_i D_j)
-// (src/UFF.ml:62-71), so D_ij (x_ij^12/r^12 - 2 x_ij^6/r^6) = A_i A_j u^6 - 2 B_i B_j u^3
+// (src/UFF.ml:39-51), so D_ij (x_ij^12/r^12 - 2 x_ij^6/r^6) = A_i A_j u^6 - 2 B_i B_j u^3
 // with u = 1/r^2, A = sqrt(D) x^6, B = sqrt(D) x^3.  A lane owning ligand atom j can
 // therefore accumulate receptor-side sums  Sq = sum w q_i / r, S12 = sum w A_i u^6,
 // S6 = sum w B_i u^3  and apply j's parameters once per chunk:
@@ -21,7 +21,7 @@
 namespace mmo {
 
 constexpr int kWave = 64;
-constexpr double kElecWeight = 332.0637 / 4.0;   // src/UFF.ml:45
+constexpr double kElecWeight = 332.0637 / 4.0;   // src/UFF.ml:25
 constexpr float kCutoff2 = 144.0f;                // src/mol.ml:836
 constexpr float kMinR2 = 1.0e-4f;                 // Math.non_zero_dist 0.01 A, src/math.ml:58-62
 

@@ -2,7 +2,7 @@
 //
 // The reference prepares ligands with mol2pqrs and reads them back in lds:
 //   Mol2 parsing, lone pairs removed            src/mol2.ml:99-113, :150-323
-//   mol2 type -> element, vdW radii             src/ptable.ml:43-53, :169-227
+//   mol2 type -> element, vdW radii             src/ptable.ml:43-53, :169-225
 //   topological distances (BFS, unit weights)   src/mol_graph.ml:53-95
 //   rotatable bonds: single (types 1, am), both ends of degree > 1, not in a ring
 //                                               src/mol_graph.ml:120-160
@@ -11,7 +11,7 @@
 //   .pqrs block "n:nrb:name", atom lines "%g %g %g %g %g symb", "beg-end= b0 .. bn-1"
 //   lines, distance matrix rows                 src/mol2pqrs.ml:9-43
 //   .pqrs reader (axis from the fixed atom to the moving one, group without the axis
-//   end, column-major distances)                src/pqrs.ml:17-90, :204-250
+//   end, column-major distances)                src/pqrs.ml:17-87, src/mol.ml:368-402
 // A .mol2 file is converted to .pqrs text and read back, as the reference's two tools
 // do, so both inputs give identical molecules (coordinates rounded by %g included).
 #include <cctype>
@@ -40,7 +40,7 @@ struct InputError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
-// Ptable.anum_of_mol2_type (src/ptable.ml:169-227)
+// Ptable.anum_of_mol2_type (src/ptable.ml:169-225)
 const std::map<std::string, int> kMol2TypeAnum = {
     {"H", 1},      {"H.t3p", 1},  {"H.spc", 1},  {"Li", 3},     {"C.3", 6},    {"C.2", 6},
     {"C.1", 6},    {"C.ar", 6},   {"C.cat", 6},  {"N.3", 7},    {"N.2", 7},    {"N.1", 7},
@@ -57,7 +57,7 @@ const std::map<int, double> kVdwRadius = {{1, 1.2},  {6, 1.7},  {7, 1.6},  {8, 1
 // Mol2.bond_order (src/mol2.ml:140-150)
 const std::map<std::string, double> kBondOrder = {
     {"1", 1.0}, {"2", 2.0}, {"3", 3.0}, {"ar", 1.5}, {"am", 1.0}};
-// element symbols of .pqrs files (src/ptable.ml:88-107)
+// element symbols of .pqrs files (src/ptable.ml:86-107)
 const std::map<std::string, int> kSymbolAnum = {{"C", 6},  {"H", 1},  {"N", 7},   {"O", 8},
                                                 {"P", 15}, {"S", 16}, {"F", 9},   {"Cl", 17},
                                                 {"Br", 35}, {"I", 53}, {"Mg", 12}};

@@ -9,7 +9,7 @@
 // Mol.rotate_then_translate_copy src/mol.ml:669-672).
 //
 // Adjoint: reverse pass over k.  Neither `left` nor `right` of rbond k moves under
-// rotation k (the group excludes both ends, src/pqrs.ml:243-250), so c^k is recovered
+// rotation k (the group excludes both ends, src/pqrs.ml:80-87), so c^k is recovered
 // from c^{k+1} by the inverse rotation and no intermediate conformers are stored.
 // For y_i = p + R(a, tau)(x_i - p), with upstream adjoint h_i:
 //   dE/dtau  = sum_i h_i . (a x (y_i - p))

@@ -49,7 +49,7 @@ def mol_arrays(m):
 
 def dists_colmajor(m):
     """Topological distances as stored in Mol.t.dists_a: flat[i + j*n] = row i, column j
-    of the .pqrs matrix (src/pqrs.ml:225-240)."""
+    of the .pqrs matrix (src/pqrs.ml:62-77)."""
     d = _field(m, "dists", "dists_a")
     n = len(_field(m, "xs"))
     d = np.asarray(d, dtype=np.int32)
@@ -91,7 +91,7 @@ class Context:
 
     # ------------------------------------------------------------------ setup
     def set_uff_table(self, table):
-        """table: {anum: (x_i, D_i)}; replaces UFF.anum_xi_Di (src/UFF.ml:30-42)."""
+        """table: {anum: (x_i, D_i)}; replaces UFF.anum_xi_Di (src/UFF.ml:10-22)."""
         an = _i32(list(table))
         x = _f64([table[a][0] for a in table])
         d = _f64([table[a][1] for a in table])

@@ -22,7 +22,7 @@ INTRA_UFF = _lib.MC_INTRA_UFF
 METROPOLIS_ALWAYS = _lib.MC_METROPOLIS_ALWAYS
 NO_FLIP = _lib.MC_NO_FLIP
 FP64 = _lib.MC_FP64
-ROOM_TEMP_K = 293.15            # Const.room_temp_K, src/const.ml:94
+ROOM_TEMP_K = 293.15            # Const.room_temp_K, src/const.ml:23
 
 
 class MonteCarlo:

@@ -21,13 +21,13 @@ import numpy as np
 
 from . import ff
 
-# UFF nonbond parameters (x_i [A], D_i [kcal/mol]) by atomic number, src/UFF.ml:30-42
+# UFF nonbond parameters (x_i [A], D_i [kcal/mol]) by atomic number, src/UFF.ml:10-22
 ANUM_XI_DI = {0: (0.0, 0.0), 1: (2.886, 0.044), 6: (3.851, 0.105), 7: (3.660, 0.069),
               8: (3.500, 0.060), 9: (3.364, 0.050), 12: (3.021, 0.111), 15: (4.147, 0.305),
               16: (4.035, 0.274), 17: (3.947, 0.227), 35: (4.189, 0.251), 53: (4.500, 0.339)}
-ELEC_WEIGHT_PROTEIN = 332.0637 / 4.0      # src/UFF.ml:45
+ELEC_WEIGHT_PROTEIN = 332.0637 / 4.0      # src/UFF.ml:25
 SYM2ANUM = {"C": 6, "H": 1, "N": 7, "O": 8, "P": 15, "S": 16, "F": 9, "Cl": 17, "Br": 35,
-            "I": 53, "Mg": 12}            # src/ptable.ml:88-101
+            "I": 53, "Mg": 12}            # src/ptable.ml:86-100
 ANUM2SYM = {v: k for k, v in SYM2ANUM.items()}
 
 
@@ -36,7 +36,7 @@ class Mol:
 
     dists_a is the flat topological distance matrix, dists_a[i + j*n] (src/mol.ml:151);
     rbonds_a is a list of (left, right) with right on the moving side, rgroups_a the
-    moving atoms of each rbond without `right` (src/pqrs.ml:204-250); t_a are grid types.
+    moving atoms of each rbond without `right` (src/mol.ml:385-402, src/pqrs.ml:80-87); t_a are grid types.
     """
 
     def __init__(self, name, xs, ys, zs, q_a, elt_a, r_a=None, dists_a=None, rbonds_a=(),
@@ -127,7 +127,7 @@ class Mol:
 
 
 # ------------------------------------------------------------------------- .pqrs I/O
-# Parsing runs in the library (mmo_molset_*, C++): the .pqrs reader of src/pqrs.ml:17-90,
+# Parsing runs in the library (mmo_molset_*, C++): the .pqrs reader of src/pqrs.ml:17-87,
 # :204-250 and Mol.ligands_of_pqrs_file / receptor_of_pqrs_file (src/mol.ml:413-440).
 def molecules_of_text(text, kind, ligand=True):
     """Mol records of .pqrs ('pqrs') or .mol2 ('mol2') text, parsed by the library.

@@ -7,15 +7,15 @@ builds them deterministically from numpy's PCG64 with fixed seeds.
 
 Ligand records mirror ``Mol.t`` (src/mol.ml:17-35) as plain numpy arrays:
 ``xs, ys, zs, q, anum`` (SoA), ``dists`` (n x n topological distances,
-src/pqrs.ml:225-240), rotatable bonds ``rb_left, rb_right`` with the axis pointing
+src/pqrs.ml:62-77), rotatable bonds ``rb_left, rb_right`` with the axis pointing
 from the fixed atom to the moving one and rotation groups ``rg_off/rg_atoms``
-that exclude the ``right`` atom (src/pqrs.ml:204-250), as produced by mol2pqrs
+that exclude the ``right`` atom (src/mol.ml:385-402, src/pqrs.ml:80-87), as produced by mol2pqrs
 (smaller side of the cut, src/mol_graph.ml:157-192).
 
 Choices the configuration text leaves open, recorded here:
   * UFF types map to elements for LJ (C_3,C_R->C, N_3,N_R->N, O_2,O_3->O, S_3->S,
     H_->H): the reference looks parameters up by atomic number only
-    (src/UFF.ml:59-60, src/mol.ml:812).
+    (src/UFF.ml:39-40, src/mol.ml:810-812).
   * Conformers are grown depth-first with backtracking and rejected with the form of
     the reference's clash rule d^2 < (a (r_vdW,i + r_vdW,j))^2 (src/ptable.ml:62-80)
     on pairs four or more bonds apart, with a = 0.65 (0.5 for pairs with a hydrogen)
@@ -286,7 +286,7 @@ def make_ligand(rng, n_heavy, n_h, n_rot, name="lig"):
             left, right, group = b, a, side_a
         rb_left.append(left)
         rb_right.append(right)
-        members = sorted(i for i in group if i != right)   # src/pqrs.ml:243-250
+        members = sorted(i for i in group if i != right)   # src/pqrs.ml:80-87
         rg_atoms += members
         rg_off.append(len(rg_atoms))
     pos = pos - pos.mean(axis=0)

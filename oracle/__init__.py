@@ -32,7 +32,7 @@ _d = _ct.c_double
 _i = _ct.c_int
 _p = _ct.c_void_p
 
-ELEC_WEIGHT = 332.0637 / 4.0          # src/UFF.ml:45
+ELEC_WEIGHT = 332.0637 / 4.0          # src/UFF.ml:25
 UFF_TABLE = {0: (0.0, 0.0), 1: (2.886, 0.044), 6: (3.851, 0.105), 7: (3.660, 0.069),
              8: (3.500, 0.060), 9: (3.364, 0.050), 12: (3.021, 0.111), 15: (4.147, 0.305),
              16: (4.035, 0.274), 17: (3.947, 0.227), 35: (4.189, 0.251), 53: (4.500, 0.339)}
@@ -150,7 +150,7 @@ def ene_inter(prot, lig, shifted=True, components=False):
 
 def ene_intra(lig):
     """Mol.ene_intra_UFFNB_brute; lig['dists'] is the n x n matrix with dists[i, j] =
-    row i column j of the .pqrs matrix (stored column-major i + j*n, src/pqrs.ml:225-240)."""
+    row i column j of the .pqrs matrix (stored column-major i + j*n, src/pqrs.ml:62-77)."""
     k = _Keep()
     n = len(lig["xs"])
     dm = _np.asarray(lig["dists"], dtype=_np.int32).reshape(n, n)

@@ -30,7 +30,7 @@
 #include <omp.h>
 #endif
 
-/* src/UFF.ml:30-42 : (anum, (x_i [A], D_i [kcal/mol])) */
+/* src/UFF.ml:10-22 : (anum, (x_i [A], D_i [kcal/mol])) */
 static const struct { int anum; double x, d; } UFF_TABLE[12] = {
   { 0, 0.0,   0.0  }, { 1, 2.886, 0.044}, { 6, 3.851, 0.105}, { 7, 3.660, 0.069},
   { 8, 3.500, 0.060}, { 9, 3.364, 0.050}, {12, 3.021, 0.111}, {15, 4.147, 0.305},
@@ -39,16 +39,16 @@ static const struct { int anum; double x, d; } UFF_TABLE[12] = {
 #define CUTOFF2 144.0          /* src/mol.ml:836 (12*12) */
 #define MAX_E   100000.0       /* src/params.ml:26 */
 
-double orc_elec_weight(void) { return 332.0637 / 4.0; }   /* src/UFF.ml:45 */
+double orc_elec_weight(void) { return 332.0637 / 4.0; }   /* src/UFF.ml:25 */
 
 int orc_uff_params(int anum, double* x, double* d) {
   for (int k = 0; k < 12; ++k)
     if (UFF_TABLE[k].anum == anum) { *x = UFF_TABLE[k].x; *d = UFF_TABLE[k].d; return 0; }
-  *x = NAN; *d = NAN;  /* xidi_params is initialised to nan, src/UFF.ml:57 */
+  *x = NAN; *d = NAN;  /* xidi_params is initialised to nan, src/UFF.ml:37 */
   return -1;
 }
 
-/* src/UFF.ml:62-71 with FF.geo_mean src/FF.ml:14-15 */
+/* src/UFF.ml:39-51 with FF.geo_mean src/FF.ml:14-15 */
 void orc_uff_pair(int a1, int a2, double* x_ij, double* d_ij) {
   double x1, d1, x2, d2;
   int ok1 = orc_uff_params(a1, &x1, &d1), ok2 = orc_uff_params(a2, &x2, &d2);
@@ -64,7 +64,7 @@ double orc_shift_12A(double d) {                                        /* src/F
   return 0.0;
 }
 
-/* pair parameter cache (119x119 in the reference, src/UFF.ml:52-60) */
+/* pair parameter cache (119x119 in the reference, src/UFF.ml:32-40) */
 typedef struct { double x, d; } xd_t;
 static void pair_params(int a1, int a2, xd_t* p) { orc_uff_pair(a1, a2, &p->x, &p->d); }
 

@@ -24,9 +24,9 @@ extern "C" {
 #endif
 
 /* constants -------------------------------------------------------------- */
-double orc_elec_weight(void);          /* 332.0637 / 4.0          src/UFF.ml:45, src/const.ml:89 */
-int    orc_uff_params(int anum, double* x, double* d);  /* src/UFF.ml:30-42; -1 if unsupported */
-void   orc_uff_pair(int a1, int a2, double* x_ij, double* d_ij); /* src/UFF.ml:57-71 (NaN if unsupported) */
+double orc_elec_weight(void);          /* 332.0637 / 4.0          src/UFF.ml:25, src/const.ml:18 */
+int    orc_uff_params(int anum, double* x, double* d);  /* src/UFF.ml:10-22; -1 if unsupported */
+void   orc_uff_pair(int a1, int a2, double* x_ij, double* d_ij); /* src/UFF.ml:37-51 (NaN if unsupported) */
 
 /* scalar helpers ---------------------------------------------------------- */
 double orc_non_zero_dist(double r);    /* src/math.ml:58-62 */

@@ -3,7 +3,7 @@ input path, the checker of the library's C++ implementation (mmo_molset_*).
 
 Follows Mol2 parsing (src/mol2.ml:150-323: atom lines "id name x y z type subst_id
 subst_name charge", bond lines "id src dst type", lone pairs removed), Ptable's mol2
-type -> element map and vdW radii (src/ptable.ml:43-53, :169-227), and Mol_graph
+type -> element map and vdW radii (src/ptable.ml:43-53, :169-225), and Mol_graph
 (src/mol_graph.ml:53-192):
 
 - topological distances: all-pairs shortest paths of the bond graph (unit weights;
@@ -16,7 +16,7 @@ type -> element map and vdW radii (src/ptable.ml:43-53, :169-227), and Mol_graph
 
 writes the .pqrs block of Mol2pqrs.process_ligand (src/mol2pqrs.ml:9-43): "n:nrb:name",
 one "x y z q r elt" line per atom (%g), one "src-dst= b_0 .. b_{n-1}" line per rotatable
-bond, then the distance matrix rows; and reads .pqrs text back (src/pqrs.ml:17-90,
+bond, then the distance matrix rows; and reads .pqrs text back (src/pqrs.ml:17-87,
 :204-250) into plain dict records.
 """
 from collections import deque
@@ -27,7 +27,7 @@ import numpy as np
 MOLECULE_TAG, ATOM_TAG, BOND_TAG = "@<TRIPOS>MOLECULE", "@<TRIPOS>ATOM", "@<TRIPOS>BOND"
 UNITY_TAG = "@<TRIPOS>UNITY_ATOM_ATTR"
 
-# Ptable.anum_of_mol2_type (src/ptable.ml:169-227)
+# Ptable.anum_of_mol2_type (src/ptable.ml:169-225)
 MOL2_TYPE_ANUM = {
     "H": 1, "H.t3p": 1, "H.spc": 1, "Li": 3,
     "C.3": 6, "C.2": 6, "C.1": 6, "C.ar": 6, "C.cat": 6,
@@ -199,7 +199,7 @@ def write_pqrs_from_mol2(mol2_fn, pqrs_fn):
 
 
 SYM2ANUM = {"C": 6, "H": 1, "N": 7, "O": 8, "P": 15, "S": 16, "F": 9, "Cl": 17, "Br": 35,
-            "I": 53, "Mg": 12}            # src/ptable.ml:88-107
+            "I": 53, "Mg": 12}            # src/ptable.ml:86-107
 
 
 def _parse_header(line):
@@ -239,7 +239,7 @@ def _read_one(lines, pos, ligand):
         left, right = (beg, end) if b[end] else (end, beg)
         rec["rb_left"].append(left)
         rec["rb_right"].append(right)
-        rec["groups"].append([i for i in range(n) if b[i] and i != right])   # src/pqrs.ml:82-90
+        rec["groups"].append([i for i in range(n) if b[i] and i != right])   # src/pqrs.ml:80-87
     dm = np.zeros((n, n), dtype=np.int32)
     for i in range(n):
         row = [int(v) for v in lines[pos].split()]

@@ -2,7 +2,7 @@
 
 Written as vectorised pair matrices (not loops) so that it shares no code with the
 C oracle; tests cross-check the two.  Formulas: src/mol.ml:796-903 (BrG, BrL, intra),
-src/G3D.ml:97-157 (trilinear), src/UFF.ml:30-71 (pair parameters).
+src/G3D.ml:97-157 (trilinear), src/UFF.ml:10-51 (pair parameters).
 """
 import numpy as np
 

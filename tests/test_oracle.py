@@ -63,7 +63,7 @@ def test_lj_minimum_and_zero(a1, a2):
 
 
 def test_coulomb_term():
-    """anum 0 is the virtual element without vdW (src/UFF.ml:31): a pure Coulomb pair."""
+    """anum 0 is the virtual element without vdW (src/UFF.ml:11): a pure Coulomb pair."""
     p = mol([0, 0, 0], [0.5], [0])
     l = mol([0, 0, 2.0], [-0.25], [0])
     e, el, vd = oracle.ene_inter(p, l, shifted=False, components=True)
@@ -151,7 +151,7 @@ def test_inter_permutation_invariance(config0):
 
 # ------------------------------------------------------------------ intra
 def test_intra_topology_index_convention():
-    """dists[i + j*n] is row i, column j of the .pqrs matrix (src/pqrs.ml:233,
+    """dists[i + j*n] is row i, column j of the .pqrs matrix (src/pqrs.ml:70,
     src/mol.ml:151-152) and only i < j pairs count (src/mol.ml:885-890).  An
     asymmetric matrix tells the two index orders apart."""
     xyz = [[0, 0, 0], [3.0, 0, 0], [0, 4.0, 0]]
