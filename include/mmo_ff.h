@@ -17,9 +17,11 @@
  *   - Coordinates are FP64 in the caller's frame.  The library re-origins them (in
  *     FP64) and keeps receptor/ligand positions as 32-bit fixed point, so pair
  *     differences are exact; pair arithmetic is FP32, per-pose sums are FP64.
- *     Host-buffer entry points reject non-finite ligand input (MMO_ERR_BAD_ARG); in
- *     device batches a non-finite atom is treated like an atom beyond the fixed-point
- *     range (no contribution to the shifted form, NaN energy in the global form).
+ *     Atoms beyond the fixed-point window (~64 A from the receptor centre) take no part
+ *     in the shifted form (they are beyond every cutoff) and are summed with FP64
+ *     pairs in the global form.  Host-buffer entry points reject non-finite ligand
+ *     input (MMO_ERR_BAD_ARG); in device batches a non-finite atom contributes nothing
+ *     to the shifted form and makes the global energy NaN.
  *   - A context belongs to the process that created it: calls from a forked child
  *     fail with MMO_ERR_FORK (HIP state does not survive fork()).
  *   - A context must be used by one thread at a time.
@@ -45,7 +47,7 @@ enum {
   MMO_ERR_NO_RECEPTOR = -5,  /* mmo_set_receptor() has not been called                */
   MMO_ERR_NO_GRID = -6,      /* mmo_set_grids() has not been called                   */
   MMO_ERR_FORK = -7,         /* context used from a process other than its creator    */
-  MMO_ERR_RANGE = -8         /* an atom lies outside the fixed-point range (GLOBAL)   */
+  MMO_ERR_RANGE = -8         /* reserved (no entry point returns it)                  */
 };
 
 /* energy forms */
@@ -161,8 +163,9 @@ enum {
 /* Scores every pose of the batch from coordinates.  xyz: device FP64
  * (n_atoms_total x 3); e_out: device FP64 (npose x 4: inter, intra, elec, vdW);
  * grad: device FP64 (n_atoms_total x 3) or NULL.  Out-of-grid poses (INTERP) get
- * NaN in e_out[p*4+0]; poses with atoms outside the fixed-point range in GLOBAL mode
- * get NaN as well. */
+ * NaN in e_out[p*4+0].  GLOBAL sums atoms anywhere (those outside the fixed-point
+ * window, ~64 A from the receptor centre, through FP64 pairs); non-finite atoms give
+ * NaN. */
 int mmo_batch_score_dev(mmo_batch* b, int flags, const double* xyz, double* e_out,
                         double* grad, void* stream);
 /* Host-pointer variant (copies in and out; synchronous). */

@@ -16,6 +16,8 @@ import torch
 
 from . import ff
 
+# Mol.check_elongation_exn with Const.charged_cutoff (src/mol.ml:589-591, src/const.ml:12)
+TOO_LONG_RADIUS = 12.0
 
 def _pose_index(offsets, device):
     off = torch.as_tensor(offsets, dtype=torch.int64)
@@ -23,27 +25,14 @@ def _pose_index(offsets, device):
     return torch.repeat_interleave(torch.arange(len(counts)), counts).to(device)
 
 
-class _Cache:
-    """Per-batch row -> pose maps (atom rows and DOF entries)."""
-
-    def __init__(self):
-        self.atom = {}
-        self.dof = {}
-
-    def atom_pose(self, batch, device):
-        key = (id(batch), str(device))
-        if key not in self.atom:
-            self.atom[key] = _pose_index(batch.atom_off, device)
-        return self.atom[key]
-
-    def dof_pose(self, batch, device):
-        key = (id(batch), str(device))
-        if key not in self.dof:
-            self.dof[key] = _pose_index(batch.dof_off, device)
-        return self.dof[key]
-
-
-_cache = _Cache()
+def _row_pose(batch, which, device):
+    """Row -> pose map of `batch` (``which`` = "atom" rows or "dof" entries) on `device`,
+    kept on the batch object itself so that it lives and dies with the batch."""
+    maps = batch.__dict__.setdefault("_row_pose", {})
+    key = (which, str(device))
+    if key not in maps:
+        maps[key] = _pose_index(batch.atom_off if which == "atom" else batch.dof_off, device)
+    return maps[key]
 
 
 def _check_input(t, rows, what):
@@ -73,31 +62,39 @@ class PoseEnergy(torch.autograd.Function):
         (g,) = ctx.saved_tensors
         if g is None:
             return None, None, None
-        w = gout[_cache.atom_pose(ctx.batch, g.device)]
+        w = gout[_row_pose(ctx.batch, "atom", g.device)]
         return (w[:, None] * g).view(-1, 3), None, None
 
 
 class DofEnergy(torch.autograd.Function):
-    """E_p of the pose built from DOF vector p (rigid body + torsions)."""
+    """E_p of the pose built from DOF vector p (rigid body + torsions).
+
+    Optim.apply_config raises Mol.Too_long when the torsions make the ligand longer
+    than 12 A (src/optim.ml:72-75) and its callers then score the worst value; here
+    such a pose gets E = +inf and a zero gradient."""
 
     @staticmethod
     def forward(ctx, dofs, batch, flags):
         _check_input(dofs, int(batch.dof_off[-1]), "dofs")
         d = dofs.detach().contiguous()
         e = torch.empty(batch.npose, 4, dtype=torch.float64, device=d.device)
+        rad = torch.empty(batch.npose, dtype=torch.float64, device=d.device)
         need = ctx.needs_input_grad[0]
         gd = torch.empty_like(d) if need else None
-        batch.score_dofs_dev(d, e, gdof=gd, flags=flags | (ff.F_GRAD if need else 0))
+        batch.score_dofs_dev(d, e, gdof=gd, radius=rad, flags=flags | (ff.F_GRAD if need else 0))
+        too_long = rad > TOO_LONG_RADIUS
+        if need:
+            gd.masked_fill_(too_long[_row_pose(batch, "dof", gd.device)], 0.0)
         ctx.batch = batch
         ctx.save_for_backward(gd)
-        return e[:, 0] + e[:, 1]
+        return torch.where(too_long, torch.full_like(rad, float("inf")), e[:, 0] + e[:, 1])
 
     @staticmethod
     def backward(ctx, gout):
         (gd,) = ctx.saved_tensors
         if gd is None:
             return None, None, None
-        return gout[_cache.dof_pose(ctx.batch, gd.device)] * gd, None, None
+        return gout[_row_pose(ctx.batch, "dof", gd.device)] * gd, None, None
 
 
 def pose_energy(batch, xyz, flags=ff.F_INTER_SHIFTED | ff.F_INTRA):

@@ -753,7 +753,6 @@ static int plain_inter(mmo_ctx* c, int mode, const double* xs, const double* ys,
     HIPCHK(c, hipMemcpyAsync(g.data(), c->s_g.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   }
   HIPCHK(c, hipStreamSynchronize(s));
-  if (std::isnan(h[0])) return fail(c, MMO_ERR_RANGE, "ligand atom outside the fixed-point range");
   *e = h[0];
   if (e_elec) *e_elec = h[2];
   if (e_vdw) *e_vdw = h[3];
@@ -1502,7 +1501,8 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
 static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   mmo_ctx* c = m->ctx;
   McArgs a{};
-  a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.atom_order = c->d_sweep.p; a.rb_left = c->d_rb_left.p;
+  a.ligs = c->d_ligs.p; a.latoms = c->d_latoms.p; a.lig_xyz = c->d_lig_xyz.p;
+  a.atom_order = c->d_sweep.p; a.rb_left = c->d_rb_left.p;
   a.rb_right = c->d_rb_right.p; a.rg_off = c->d_rg_off.p; a.rg_atoms = c->d_rg_atoms.p;
   a.pair_ij = c->d_pair_ij.p; a.pair_par = c->d_pair_par.p; a.max_atoms = m->max_atoms;
   a.nwork = nwork; a.work = work; a.chain_lig = m->d_chain_lig.p; a.chain_atom = m->d_chain_atom.p;

@@ -128,6 +128,42 @@ __device__ __forceinline__ void pair_grad(const RecPos& p, const RecAB& ab, cons
   gz = fmaf(coef, dz, gz);
 }
 
+// Global form for a ligand atom outside the fixed-point window (more than ~2^(30-k) A
+// from the frame origin): FP64 pairs over the receptor tile from the atom's caller-frame
+// position relative to the frame origin, like the reference's all-pairs loop
+// (src/mol.ml:796-818, no cutoff, r = max(r, 0.01)).  Such an atom is far from every
+// receptor atom, so the pair terms are smooth and need no fixed-point differences.
+// With GRAD, (gx, gy, gz) receive dE/dx of the atom in A^-1.
+template <bool GRAD>
+__device__ void far_atom_pairs(const RecPos* __restrict__ s_pos, const RecAB* __restrict__ s_ab,
+                               int nt, double inv_scale, double xr, double yr, double zr,
+                               double keq, double aj, double bj, double& e_el, double& e_vd,
+                               double& gx, double& gy, double& gz) {
+  double el = 0.0, vd = 0.0;
+  for (int i = 0; i < nt; ++i) {
+    const RecPos p = s_pos[i];
+    const RecAB ab = s_ab[i];
+    const double dx = (double)p.x * inv_scale - xr, dy = (double)p.y * inv_scale - yr,
+                 dz = (double)p.z * inv_scale - zr;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double rr = sqrt(r2);
+    const bool apart = rr >= 0.01;
+    const double r = apart ? rr : 0.01;
+    const double ir = 1.0 / r, u = ir * ir, u3 = u * u * u;
+    const double eq = (double)p.q * ir, e12 = aj * (double)ab.a * u3 * u3,
+                 e6 = bj * (double)ab.b * u3;
+    el += eq;
+    vd += e12 - 2.0 * e6;
+    if (GRAD && apart) {
+      // dE/dr / r, times (x_lig - x_rec) = -d
+      const double c = -(keq * eq + 12.0 * e12 - 12.0 * e6) * u;
+      gx -= c * dx; gy -= c * dy; gz -= c * dz;
+    }
+  }
+  e_el = fma(keq, el, e_el);
+  e_vd += vd;
+}
+
 // ------------------------------------------------------------------ inter kernel
 template <int MODE, int K, bool GRAD, class Src>
 __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
@@ -151,7 +187,6 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   const Frame fr = a.fr;
   const PairK pk = pair_k(fr.inv_s2, fr.inv_s, a.rres);
   double e_el = 0.0, e_vd = 0.0;
-  int sat = 0;
   const int r_begin = blockIdx.y * a.rslice;
   const int r_end = min(a.nrec, r_begin + a.rslice);
   const bool split_rec = gridDim.y > 1;
@@ -170,22 +205,47 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
       double cq[K], ca[K], cb[K];
       float c1[K], fa[K], fb[K];
       int row[K];
+      bool far[K];      // GLOBAL: outside the fixed-point window, summed by far_atom_pairs
+      bool any_far = false;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int j = aslot + split * (blk * K + k);
+        far[k] = false;
         if (pose_ok && j < li.n_atoms) {
           double x, y, z;
           a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
-          l[k] = lig_pt(x, y, z, fr, &sat);
+          int satk = 0;
+          l[k] = lig_pt(x, y, z, fr, &satk);
           const LigAtom la = a.latoms[li.atom_base + j];
           cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
           row[k] = GRAD ? a.src.row(pose, j) : 0;
+          if (MODE == MMO_GLOBAL && satk) {
+            // the saturated position takes no part in the fixed-point pairs
+            far[k] = any_far = true;
+            cq[k] = ca[k] = cb[k] = 0.0;
+          }
         } else {
           l[k] = lig_pt_zero();
           cq[k] = ca[k] = cb[k] = 0.0;
           row[k] = -1;
         }
         c1[k] = (float)cq[k]; fa[k] = (float)ca[k]; fb[k] = (float)cb[k];
+      }
+      // GLOBAL: FP64 pairs of the far atoms over this receptor tile (wave-uniform branch)
+      double far_gx[K], far_gy[K], far_gz[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) far_gx[k] = far_gy[k] = far_gz[k] = 0.0;
+      if (MODE == MMO_GLOBAL && __ballot(any_far) != 0ull) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!far[k]) continue;
+          const int j = aslot + split * (blk * K + k);
+          double x, y, z;
+          a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
+          const LigAtom la = a.latoms[li.atom_base + j];
+          far_atom_pairs<GRAD>(s_pos, s_ab, nt, 1.0 / fr.scale, x - fr.ox, y - fr.oy, z - fr.oz,
+                               la.keq, la.a, la.b, e_el, e_vd, far_gx[k], far_gy[k], far_gz[k]);
+        }
       }
       if (!GRAD) {
         for (int c0 = 0; c0 < nt; c0 += kChunk) {
@@ -235,6 +295,9 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
         const double gs = -2.0 * (double)fr.inv_s;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
+          if (MODE == MMO_GLOBAL && far[k]) {   // A^-1 -> the units scaled by gs below
+            gx[k] += far_gx[k] / gs; gy[k] += far_gy[k] / gs; gz[k] += far_gz[k] / gs;
+          }
           if (row[k] >= 0) {
             double* g = a.grad + 3 * (size_t)row[k];
             if (split_rec) {   // grad pre-zeroed, slices add concurrently
@@ -254,12 +317,10 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs<Src> a) {
   if (!tile_ok) return;
   e_el = group_sum(e_el, split);
   e_vd = group_sum(e_vd, split);
-  sat = group_or(sat, split);
   if (pose_ok && aslot == 0) {
-    double e = e_el + e_vd;
-    // out of the fixed-point range: exact for the shifted form (saturated atoms stay
-    // beyond the cutoff), undefined for the global form
-    if (MODE == MMO_GLOBAL && sat) e = __builtin_nan("");
+    // saturated atoms stay beyond the cutoff of every receptor atom in the shifted form;
+    // the global form sums them through far_atom_pairs
+    const double e = e_el + e_vd;
     // receptor slices write partial sums to slice-major buffers reduced afterwards
     const size_t o = (size_t)blockIdx.y * a.npose_total * a.e_stride + (size_t)pose * a.e_stride;
     if (!GRAD || a.write_e) a.e_out[o] = e;

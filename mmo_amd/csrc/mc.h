@@ -44,6 +44,7 @@ struct McArgs {
   // ligands
   const LigInfo* ligs;
   const LigAtom* latoms;
+  const double* lig_xyz;        // registered centred conformers (rows, at atom_base)
   const int* atom_order;        // nullable: per ligand at atom_base, the sweep's atom order
   const int* rb_left;
   const int* rb_right;

@@ -158,10 +158,10 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
     };
     auto none = [](int, double, double, double) {};
     if (INTER == kMcInterAtom)
-      // the chains skip the residual correction of pairs closer than 0.5 A: it changes
-      // only energies above 1e7 kcal/mol (by up to ~2e-5 relative)
-      atom_sweep<false, decltype(atom)&, decltype(none)&, false, kMcPreload, kMcSweepAtoms>(a.cg, s_cell, s_pos, s_ab, a.rres, fr,
-                                                               li.n_atoms, wl, atom, none, e_el, e_vd);
+      // pairs closer than 0.5 A add both atoms' quantisation residuals (without them a
+      // clashing pose's energy, above 1e7 kcal/mol, is off by up to ~2e-5 relative)
+      atom_sweep<false, decltype(atom)&, decltype(none)&, true, kMcPreload, kMcSweepAtoms>(
+          a.cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, wl, atom, none, e_el, e_vd);
     else
       pose_sweep<false>(a.cg, s_cell, s_pos, s_ab, a.rres, fr, li.n_atoms, atom, none, e_el, e_vd);
     return wave_sum(e_el + e_vd);
@@ -438,11 +438,24 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
       }
       if (metropolis && st.prev_e_inter == 0.0) { reset = true; metropolis = false; }
       if (reset) {
-        // reset_run_params (src/lds.ml:632-648): the conformer is kept
+        // reset_run_params (src/lds.ml:632-648): the conformer is kept; the best pose
+        // becomes the registered conformer at the start pose (best_lig := prev_lig :=
+        // rotate_then_translate_copy centered_lig rot0 pos0, src/lds.ml:645-646)
         st.max_rot = a.max_rot0;
         st.max_trans = a.max_trans0;
         for (int k = 0; k < 9; ++k) { st.rot[k] = st.rot0[k]; st.best_rot[k] = (k % 4 == 0) ? 1.0 : 0.0; }
         for (int k = 0; k < 3; ++k) { st.pos[k] = st.pos0[k]; st.best_pos[k] = 0.0; }
+        {
+          const double* c0 = a.lig_xyz + 3 * (size_t)li.atom_base;
+          const double* R0 = st.rot0;
+          const double* p0 = st.pos0;
+          for (int j = lane; j < L; j += kWave) {
+            const double x0 = c0[3 * j], y0 = c0[3 * j + 1], z0 = c0[3 * j + 2];
+            gbest[3 * j] = (R0[0] * x0 + R0[1] * y0 + R0[2] * z0) + p0[0];
+            gbest[3 * j + 1] = (R0[3] * x0 + R0[4] * y0 + R0[5] * z0) + p0[1];
+            gbest[3 * j + 2] = (R0[6] * x0 + R0[7] * y0 + R0[8] * z0) + p0[2];
+          }
+        }
         st.prev_e = __builtin_inf();
         st.best_e = __builtin_inf();
         sw_reset(st.sw);

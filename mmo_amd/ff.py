@@ -76,6 +76,7 @@ class Context:
         self.nrec = 0
         self.grid_shape = None
         self._ligs = []
+        self._nrb = []
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -207,10 +208,14 @@ class Context:
                                        _ptr(rga if len(rga) else None), len(rbl), _ptr(t),
                                        ct.byref(out)))
         self._ligs.append(n)
+        self._nrb.append(len(rbl))
         return out.value
 
     def ligand_natoms(self, lig_id):
         return self._ligs[lig_id]
+
+    def ligand_nrb(self, lig_id):
+        return self._nrb[lig_id]
 
     def batch(self, pose_lig):
         return Batch(self, pose_lig)

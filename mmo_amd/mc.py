@@ -43,6 +43,8 @@ class MonteCarlo:
         self.natoms = lib.mmo_mc_natoms(h)
         sizes = np.array([ctx.ligand_natoms(int(l)) for l in cl], dtype=np.int64)
         self.chain_atom = np.concatenate([[0], np.cumsum(sizes)])
+        nrb = np.array([ctx.ligand_nrb(int(l)) for l in cl], dtype=np.int64)
+        self.chain_rb = np.concatenate([[0], np.cumsum(nrb)])
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -83,6 +85,12 @@ class MonteCarlo:
         self.ctx._check(lib.mmo_mc_coords(self.h, _ptr(conf), _ptr(best), None))
         split = lambda a: [a[self.chain_atom[k]:self.chain_atom[k + 1]] for k in range(self.nchain)]
         return split(conf), split(best)
+
+    def bond_steps(self):
+        """Each chain's per-rotatable-bond torsion step sizes (radians)."""
+        dr = np.zeros(max(int(self.chain_rb[-1]), 1))
+        self.ctx._check(lib.mmo_mc_coords(self.h, None, None, _ptr(dr)))
+        return [dr[self.chain_rb[k]:self.chain_rb[k + 1]] for k in range(self.nchain)]
 
     def rescore(self):
         """E_inter of every chain's best pose by the direct shifted sum, whatever form the

@@ -496,6 +496,15 @@ def library(n_ligands=1024, seed=None, median_heavy=30, sigma=0.35, h_ratio=0.87
     # worker processes are started fresh ("spawn"): nothing GPU-related is inherited
     if processes and processes > 1 and n_ligands >= 32:
         import multiprocessing as mp
-        with mp.get_context("spawn").Pool(min(processes, 8)) as pool:
-            return pool.map(_lib_one, jobs, chunksize=4)
+        # close() + join() let the workers exit on their own; the context manager's
+        # terminate() would end them with SIGTERM
+        pool = mp.get_context("spawn").Pool(min(processes, 8))
+        try:
+            out = pool.map(_lib_one, jobs, chunksize=4)
+            pool.close()
+            pool.join()
+            return out
+        except BaseException:
+            pool.terminate()
+            raise
     return [_lib_one(j) for j in jobs]

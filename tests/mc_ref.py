@@ -32,10 +32,28 @@ class SW:
         self.events = []
 
 
+def _decision_margin(cur, prev_e, best_e, u, beta, scale):
+    """Distance of this frame's two energy comparisons from their boundaries, relative
+    to `scale` (the magnitudes of the E_inter values involved: only E_inter carries the
+    GPU's FP32 pair error; E_intra is FP64 on both sides).  The proposal is accepted iff
+    cur - prev < T = -ln(u) / beta (which covers cur <= prev) and becomes the best pose
+    iff cur < best; an E_inter error below this fraction of `scale` changes neither."""
+    m = math.inf
+    if math.isfinite(prev_e) and math.isfinite(cur):
+        T = math.inf if (beta == 0.0 or u == 0.0) else -math.log(u) / beta
+        if math.isfinite(T):
+            m = abs((cur - prev_e) - T) / scale
+    if math.isfinite(best_e) and math.isfinite(cur):
+        m = min(m, abs(cur - best_e) / scale)
+    return m
+
+
 def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True, hard_roi=None,
              metropolis_always=False, beta=1.0 / (0.0019872041 * 293.15), inter_fn=None,
-             trace=None):
-    """Runs one chain for `nframes`; `trace`, when a list, receives one dict per frame."""
+             trace=None, no_flip=False):
+    """Runs one chain for `nframes`; `trace`, when a list, receives one dict per frame.
+    best_xyz is best_lig: the pose of the best proposal, or after a reset the registered
+    conformer at the start pose (src/lds.ml:645-646)."""
     n = len(lig["xs"])
     conf = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1).astype(np.float64)
     center = np.zeros(3)
@@ -57,11 +75,14 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
             return inter_fn(x)
         return oracle.ene_inter(rec, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]), shifted=True)
 
+    conf0 = conf.copy()
     start = conf @ rot.T + pos
+    best_xyz = start.copy()
     prev_intra = e_intra(conf)
     prev_inter = e_inter(start)
     prev_e = prev_inter + prev_intra
     best_e = prev_e
+    prev_e_inter = best_inter = prev_inter   # E_inter of the current state / best pose
     best_rot, best_pos = rot.copy(), pos.copy()
     rigid_step = conf_step = 0
     accepted = resets = ooroi = 0
@@ -71,10 +92,11 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
         rigid = frame % 2 == 0
         conf2, c2, jr, ndr = conf, center, -1, list(dr)
         rot2, pos2 = rot, pos
+        ang = None
         if not rigid:
             if flex:
                 i = min(int(u[0] * nrb), nrb - 1)
-                flip = conf_step > 0 and conf_step % 100 == 0
+                flip = conf_step > 0 and conf_step % 100 == 0 and not no_flip
                 amp = PI if flip else ndr[i]
                 ang = u[1] * (2.0 * amp) - amp
                 conf2 = oracle.rotate_bond(conf, int(lig["rb_left"][i]), int(lig["rb_right"][i]),
@@ -98,6 +120,7 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
         cur = prev_inter + prev_intra
         metropolis = False
         reset = False
+        margin = math.inf
         if hard_roi is not None:
             if np.linalg.norm(np.asarray(hard_roi[0]) - pos2) > hard_roi[1]:
                 reset = True
@@ -112,12 +135,15 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
             max_rot, max_trans = max_rot0, max_trans0
             rot, pos = np.array(rot0, dtype=float).reshape(3, 3), np.array(pos0, dtype=float)
             best_rot, best_pos = np.eye(3), np.zeros(3)
+            best_xyz = conf0 @ rot.T + pos
             prev_e = math.inf
             best_e = math.inf
             sw.reset()
             resets += 1
         elif metropolis:
             acc = (cur <= prev_e) or (u[5] < math.exp(-(cur - prev_e) * beta))
+            margin = _decision_margin(cur, prev_e, best_e, u[5], beta,
+                                      abs(prev_inter) + abs(prev_e_inter) + abs(best_inter) + 1e-300)
             if rigid:
                 sw.process(acc)
             elif jr > -1:
@@ -125,9 +151,12 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
             if acc:
                 accepted += 1
                 rot, pos, prev_e = rot2, pos2, cur
+                prev_e_inter = prev_inter
                 conf, center = conf2, c2
             if cur < best_e:
                 best_e, best_rot, best_pos = cur, rot2, pos2
+                best_inter = prev_inter
+                best_xyz = pose.copy()
             if rigid and rigid_step > 0 and rigid_step % 100 == 0:
                 ar = sw.ratio()
                 if ar <= 0.45:
@@ -153,8 +182,12 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
             trace.append({"frame": frame, "e_inter": prev_inter, "e_intra": prev_intra, "e_prop": cur,
                           "e": prev_e, "best_e": best_e, "accepted": accepted,
                           "resets": resets, "pos": pos.copy(), "rot": rot.copy(),
-                          "conf": conf.copy(), "center": np.array(center), "pose": pose.copy()})
+                          "conf": conf.copy(), "center": np.array(center), "pose": pose.copy(),
+                          "bond": jr, "angle": ang, "max_rot": max_rot, "max_trans": max_trans,
+                          "dr": list(dr), "best_xyz": best_xyz.copy(), "reset": reset,
+                          "margin": margin})
     return {"best_e": best_e, "e": prev_e, "e_inter": prev_inter, "e_intra": prev_intra,
             "rot": rot, "pos": pos, "conf": conf, "center": np.array(center), "accepted": accepted, "resets": resets,
             "ooroi": ooroi, "status": status, "max_rot": max_rot, "max_trans": max_trans,
-            "best_rot": best_rot, "best_pos": best_pos, "dr": dr}
+            "best_rot": best_rot, "best_pos": best_pos, "best_xyz": best_xyz, "dr": dr,
+            "rigid_step": rigid_step, "conf_step": conf_step}

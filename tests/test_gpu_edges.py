@@ -9,7 +9,8 @@ import pytest
 
 import oracle
 from mmo_amd import ff, synth
-from mmo_amd._lib import (ERR_BAD_ARG, ERR_NO_GRID, ERR_NO_RECEPTOR, ERR_OUT_OF_GRID, ERR_RANGE,
+from tests.parity import assert_energy, inter_abs_sum
+from mmo_amd._lib import (ERR_BAD_ARG, ERR_NO_GRID, ERR_NO_RECEPTOR, ERR_OUT_OF_GRID,
                           MMOError)
 
 pytestmark = pytest.mark.gpu
@@ -64,13 +65,17 @@ def test_single_atom_ligand_every_form(system):
 
 
 def test_saturated_atoms(system):
-    """Atoms beyond the fixed-point range: exactly 0 in the shifted form, an error in
-    the global form (whose sum has no cutoff)."""
-    ctx, lig, _ = system
+    """Atoms beyond the fixed-point range: exactly 0 in the shifted form; the global
+    form (no cutoff) sums them like the oracle."""
+    ctx, lig, rec = system
     far = dict(lig, xs=np.asarray(lig["xs"]) + 1e6)
     print("edge: saturated coordinates", flush=True)
     assert ctx.ene_inter(far, ff.SHIFTED) == 0.0
-    assert _code(lambda: ctx.ene_inter(far, ff.GLOBAL)) == ERR_RANGE
+    e = ctx.ene_inter(far, ff.GLOBAL)
+    ref = oracle.ene_inter(rec, far, shifted=False)
+    # the net-neutral ligand's pair terms cancel at 1e6 A: the conditioned measure
+    xyz = np.stack([far["xs"], far["ys"], far["zs"]], 1)
+    assert_energy(e, ref, inter_abs_sum(rec, xyz, far, False), what="global, 1e6 A")
     # non-finite input: rejected by the plain forms; in batches it counts as out of range
     nan = dict(lig, xs=np.full(len(lig["xs"]), np.nan))
     assert _code(lambda: ctx.ene_inter(nan, ff.SHIFTED)) == ERR_BAD_ARG

@@ -60,13 +60,31 @@ def test_plain_inter_components(sys0):
     assert abs(el - rel_) / s < 1e-5 and abs(vd - rvd) / s < 1e-5
 
 
-def test_plain_inter_far_away_is_exactly_zero(sys0):
+def test_plain_inter_far_away(sys0):
+    """A ligand 200 A away (beyond the fixed-point window too): exactly 0 in the shifted
+    form; the global form (no cutoff, src/mol.ml:796-818) sums every pair like the
+    oracle, energy and gradient, also in a batch next to an in-window pose."""
     ctx, lig, rec = sys0
-    l, _ = _place(lig, np.eye(3), [200.0, 0.0, 0.0])   # beyond the fixed-point range too
+    l, xyz = _place(lig, np.eye(3), [200.0, 0.0, 0.0])
+    print("far away: plain forms", flush=True)
     assert ctx.ene_inter(l, ff.SHIFTED) == 0.0
-    with pytest.raises(MMOError) as e:
-        ctx.ene_inter(l, ff.GLOBAL)
-    assert e.value.code == -8
+    e = ctx.ene_inter(l, ff.GLOBAL)
+    ref = oracle.ene_inter(rec, l, shifted=False)
+    s = inter_abs_sum(rec, xyz, l, False)
+    assert math.isfinite(e)
+    assert_energy(e, ref, s, what="global, 200 A")
+    eg, g = ctx.ene_inter_grad(l, ff.GLOBAL)
+    assert_grad(g, oracle.grad_inter(rec, l, shifted=False), what="global grad, 200 A")
+    print("far away: batch of a near and a far pose", flush=True)
+    lid = ctx.add_ligand(lig)
+    b = ctx.batch([lid, lid])
+    near, nxyz = _place(lig, np.eye(3), [0.5, -1.0, 2.0])
+    eb, gb = b.score(np.concatenate([nxyz, xyz]), ff.F_INTER_GLOBAL | ff.F_GRAD)
+    assert_energy(eb[0, 0], oracle.ene_inter(rec, near, shifted=False),
+                  inter_abs_sum(rec, nxyz, near, False), what="global batch, near")
+    assert_energy(eb[1, 0], ref, s, what="global batch, far")
+    n = len(lig["xs"])
+    assert_grad(gb[n:], oracle.grad_inter(rec, l, shifted=False), what="global batch grad, far")
 
 
 def test_plain_intra(sys0):
@@ -779,6 +797,161 @@ def test_mc_config0_full_length_fp64():
         assert st[key][0] == pytest.approx(ref[key], rel=1e-6, abs=1e-4), key
     assert np.allclose(st["max_rot"][0], ref["max_rot"]) and np.allclose(st["max_trans"][0], ref["max_trans"])
     assert np.allclose(conf[0], ref["conf"], atol=1e-8)
+
+
+def _long_chain_system(nlig=6, chains=2, seed=41, max_rb=4):
+    """Library ligands with 1..max_rb rotatable bonds (so that 1,000 frames pass conformer
+    step 100 nrb), `chains` chains each, starting on the receptor's outer surface, where
+    energies are moderate and few decisions hinge on the last digits of E."""
+    lib = [l for l in synth.library(64, seed=seed) if 1 <= len(l["rb_left"]) <= max_rb][:nlig]
+    _, rec = synth.config0()
+    rng = np.random.default_rng(seed)
+    ncl = len(lib) * chains
+    rot0 = synth.random_rotations(rng, ncl)
+    pos0 = SURFACE + rng.uniform(-1.0, 1.0, (ncl, 3))
+    return lib, rec, rot0, pos0
+
+
+LONG_ROI = (SURFACE, 4.0)
+
+
+def _compare_long_chains(m, lig_of_chain, refs, min_margin, e_rel):
+    """GPU chains against restatement runs (with traces): chains whose every decision in
+    the restatement lies farther than min_margin (relative) from its boundary must take
+    the same decisions, so their counters, step sizes, per-bond steps, conformers and
+    best poses agree; returns the indices of the chains compared."""
+    st = m.state()
+    conf, best = m.coords()
+    dr = m.bond_steps()
+    compared = []
+    for k, (ref, tr) in enumerate(refs):
+        if min(t["margin"] for t in tr) <= min_margin:
+            continue
+        compared.append(k)
+        lig = lig_of_chain[k]
+        assert st["status"][k] == (1 if ref["status"] == 0 else 2), k
+        assert st["frame"][k] == len(tr) or ref["status"] != 0, k
+        assert st["accepted"][k] == ref["accepted"], (k, st["accepted"][k], ref["accepted"])
+        assert st["resets"][k] == ref["resets"] and st["ooroi"][k] == ref["ooroi"], k
+        assert st["rigid_step"][k] == ref["rigid_step"] and st["conf_step"][k] == ref["conf_step"]
+        assert st["max_rot"][k] == ref["max_rot"] and st["max_trans"][k] == ref["max_trans"], k
+        assert np.array_equal(dr[k], np.asarray(ref["dr"])), (k, dr[k], ref["dr"])
+        assert np.allclose(conf[k], ref["conf"], rtol=0, atol=1e-9), k
+        assert np.allclose(best[k], ref["best_xyz"], rtol=0, atol=1e-8), k
+        for key in ("best_e", "e"):
+            if math.isinf(ref[key]):
+                assert math.isinf(st[key][k]), (k, key)
+            else:
+                assert st[key][k] == pytest.approx(ref[key], rel=e_rel, abs=1e-6), (k, key)
+    return compared
+
+
+def _assert_rules_exercised(refs, compared, lig_of_chain):
+    """The compared chains went past the adaptive rigid steps (rigid step 100), the
+    flips (conformer step 100) and the per-bond steps (conformer step 100 nrb)."""
+    assert len(compared) >= len(refs) // 2, "too many chains with a close decision"
+    full = [k for k in compared if refs[k][0]["status"] == 0]
+    assert len(full) >= 2, "too few compared chains ran all their frames"
+    assert all(refs[k][0]["rigid_step"] >= 200 for k in full)
+    assert all(refs[k][0]["conf_step"] >= 100 for k in full)
+    assert any(refs[k][0]["conf_step"] >= 100 * len(lig_of_chain[k]["rb_left"]) for k in full)
+    assert any(refs[k][0]["accepted"] > 0 and refs[k][0]["max_trans"] != 0.15 for k in full)
+
+
+def test_mc_atom_sweep_1000_frames_follow_restatement():
+    """The benchmarked chains (FP32 per-atom sweep, receptor in LDS) for the workload's
+    full 1,000 frames against the FP64 restatement, past every adaptive rule."""
+    from tests.mc_ref import simulate
+    from mmo_amd import mc as mcmod
+    lib, rec, rot0, pos0 = _long_chain_system()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    ids = [ctx.add_ligand(l) for l in lib]
+    cl = np.repeat(np.asarray(ids), 2)
+    lig_of = [lib[ids.index(l)] for l in cl]
+    roi = LONG_ROI
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    m = mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.SHIFTED, total_frames=1000, flags=flags,
+                         roi_center=roi[0], roi_r=roi[1], seed=1234)
+    print("mc atom sweep 1000 frames, %d chains" % len(cl), flush=True)
+    m.run_graph(250, nreplay=4)
+    refs = []
+    for k in range(len(cl)):
+        tr = []
+        refs.append((simulate(lig_of[k], rec, rot0[k], pos0[k], 1000, seed=1234, chain=k,
+                              hard_roi=roi, trace=tr), tr))
+        print("  restatement chain %d done" % k, flush=True)
+    compared = _compare_long_chains(m, lig_of, refs, min_margin=1e-5, e_rel=1e-5)
+    print("compared chains", compared, flush=True)
+    _assert_rules_exercised(refs, compared, lig_of)
+
+
+def test_mc_grid_1000_frames_follow_restatement():
+    """The grid-interpolation chains (10 per-type FP32 grids of 100^3 at 0.375 A computed
+    on the GPU) for 1,000 frames against the restatement on the same grids."""
+    from tests.mc_ref import simulate
+    from mmo_amd import mc as mcmod
+    lib, rec, rot0, pos0 = _long_chain_system(seed=43)
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    probe_a, probe_q, types = synth.grid_types(lib, 10)
+    origin, dims, step = synth.grid_spec(SURFACE, 100, 0.375)
+    grids = ctx.compute_grids(probe_a, probe_q, step, dims, origin)
+    ids = [ctx.add_ligand(l, types=t) for l, t in zip(lib, types)]
+    cl = np.repeat(np.asarray(ids), 2)
+    lig_of = [lib[ids.index(l)] for l in cl]
+    typ_of = [types[ids.index(l)] for l in cl]
+    roi = LONG_ROI
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    m = mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.INTERP, total_frames=1000, flags=flags,
+                         roi_center=roi[0], roi_r=roi[1], seed=4321)
+    print("mc grid 1000 frames, %d chains" % len(cl), flush=True)
+    m.run_graph(250, nreplay=4)
+    g = oracle.Grid.make(step, dims)
+    gl = list(grids)
+    refs = []
+    for k in range(len(cl)):
+        tr = []
+
+        def inter(x, k=k):
+            return oracle.ene_inter_interp(g, gl, x - origin, typ_of[k])
+        refs.append((simulate(lig_of[k], rec, rot0[k], pos0[k], 1000, seed=4321, chain=k,
+                              hard_roi=roi, inter_fn=inter, trace=tr), tr))
+    compared = _compare_long_chains(m, lig_of, refs, min_margin=1e-9, e_rel=1e-9)
+    print("compared chains", compared, flush=True)
+    _assert_rules_exercised(refs, compared, lig_of)
+
+
+def test_mc_chains_in_contact_keep_energies_within_tolerance():
+    """Chains started inside the receptor's atoms (clashing poses, energies far above
+    1e7 kcal/mol): every proposal's E_inter of the FP32 sweep stays within 1e-5 of the
+    FP64 restatement's (pairs closer than 0.5 A add the quantisation residuals)."""
+    from tests.mc_ref import simulate
+    from mmo_amd import mc as mcmod
+    lib, rec, _, _ = _long_chain_system(nlig=3, chains=1, seed=45)
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    ids = [ctx.add_ligand(l) for l in lib]
+    rxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    pos0 = rxyz[[5, 500, 1500]] + 0.05          # ligand centres on receptor atoms
+    rot0 = synth.random_rotations(np.random.default_rng(45), 3)
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    nf = 24
+    m = mcmod.MonteCarlo(ctx, ids, rot0, pos0, ff.SHIFTED, total_frames=nf, flags=flags,
+                         roi_center=np.zeros(3), roi_r=40.0, seed=77)
+    got = []
+    for f in range(nf):
+        m.run(1)
+        got.append(m.state()["e_inter"].copy())
+    print("contact chains: E_inter of the first proposals", got[0], flush=True)
+    for k in range(3):
+        tr = []
+        simulate(lib[k], rec, rot0[k], pos0[k], nf, seed=77, chain=k,
+                 hard_roi=(np.zeros(3), 40.0), trace=tr)
+        ref = np.array([t["e_inter"] for t in tr])
+        gpu = np.array([g[k] for g in got])
+        assert ref.max() > 1e7, "chain %d did not start in contact" % k
+        assert np.all(np.abs(gpu - ref) <= 1e-5 * np.abs(ref)), (k, np.abs(gpu / ref - 1).max())
 
 
 def test_mc_launch_forms_agree():
