@@ -74,7 +74,9 @@ def test_plain_inter_far_away(sys0):
     assert math.isfinite(e)
     assert_energy(e, ref, s, what="global, 200 A")
     eg, g = ctx.ene_inter_grad(l, ff.GLOBAL)
-    assert_grad(g, oracle.grad_inter(rec, l, shifted=False), what="global grad, 200 A")
+    e64, g64 = oracle.grad_inter(rec, l, shifted=False)
+    assert_energy(eg, e64, s, what="global, 200 A, gradient call")
+    assert_grad(g, g64, what="global grad, 200 A")
     print("far away: batch of a near and a far pose", flush=True)
     lid = ctx.add_ligand(lig)
     b = ctx.batch([lid, lid])
@@ -84,7 +86,7 @@ def test_plain_inter_far_away(sys0):
                   inter_abs_sum(rec, nxyz, near, False), what="global batch, near")
     assert_energy(eb[1, 0], ref, s, what="global batch, far")
     n = len(lig["xs"])
-    assert_grad(gb[n:], oracle.grad_inter(rec, l, shifted=False), what="global batch grad, far")
+    assert_grad(gb[n:], g64, what="global batch grad, far")
 
 
 def test_plain_intra(sys0):
