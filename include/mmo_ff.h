@@ -226,6 +226,15 @@ int mmo_scan_filter_dev(mmo_ctx* ctx, int lig_id, const double* rots, int nrot, 
  * (k > n gives n).  e_out, idx_out: device.  Enqueued on stream. */
 int mmo_topk_dev(mmo_ctx* ctx, const double* e, int64_t n, int k, double* e_out, int64_t* idx_out,
                  void* stream);
+/* mmo_topk_dev without any allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops).  The
+ * kernels' scratch (radix state, histogram, counters, per-chunk counts) is a static
+ * device array of 132 KiB per GPU, set up by mmo_ctx_create, so n <= 134,217,728
+ * (larger n -> MMO_ERR_BAD_ARG); same results as mmo_topk_dev.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other contexts,
+ * other threads) race and must be ordered by the caller. */
+int mmo_topk_dev_noalloc(mmo_ctx* ctx, const double* e, int64_t n, int k, double* e_out,
+                         int64_t* idx_out, void* stream);
 
 /* -------------------------------------------------------------- Monte Carlo */
 /* Batched MC chains (Lds.simulate_lig, src/lds.ml:741-1000) on the GPU: chain k

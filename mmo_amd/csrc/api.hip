@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,33 @@ using namespace mmo;
 namespace {
 
 std::string g_last_error;
+
+// ----------------------------------------------- no-allocation top-k (mmo_topk_dev_noalloc)
+// Scratch of the top-k kernels in a file-scope device array instead of library
+// allocations: radix state (3 x u64) at byte 0, histogram (256 x u32) at 256, counter
+// (u32) at 1280, per-chunk counts (u32) from 1536.  Each GPU holds its own copy for the
+// life of the process; host code uses the per-device address from hipGetSymbolAddress,
+// never the array's name.  Every call on one GPU reuses its copy: the kernels clear the
+// state, histogram and counter themselves and write every chunk count before reading it.
+constexpr size_t kTopkChunk = 4096;                       // energies per chunk count
+constexpr size_t kTopkNoallocChunks = 32768;              // -> at most 134,217,728 energies
+constexpr size_t kTopkNoallocBytes = 1536 + 4 * kTopkNoallocChunks;
+static __device__ __align__(256) unsigned char scratch[kTopkNoallocBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+// the calling thread's current device must be `device`
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
 
 struct Uff { double x, d; };
 
@@ -493,6 +521,7 @@ int mmo_ctx_create(int device, mmo_ctx** out) {
     return fail(nullptr, MMO_ERR_BAD_ARG, "device %d out of range (%d devices)", device, ndev);
   e = hipSetDevice(device);
   if (e != hipSuccess) return fail(nullptr, MMO_ERR_DEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+  scratch_setup(device);
   mmo_ctx* c = new mmo_ctx();
   c->device = device;
   c->pid = getpid();
@@ -1835,11 +1864,44 @@ int mmo_topk_dev(mmo_ctx* c, const double* e, int64_t n, int k, double* e_out, i
     return fail(c, MMO_ERR_BAD_ARG, "bad top-k arguments (n %lld, k %d, k <= %d)", (long long)n,
                 k, kTopkMax);
   const int kk = (int)std::min<int64_t>(k, n);
-  const size_t chunk = 4096;
+  const size_t chunk = kTopkChunk;
   const size_t nb = ((size_t)n + chunk - 1) / chunk;
   HIPCHK(c, c->s_tk_st.ensure(3));
   HIPCHK(c, c->s_tk_u.ensure(256 + 1 + nb));
   TopkScratch w{c->s_tk_st.p, c->s_tk_u.p, c->s_tk_u.p + 256, c->s_tk_u.p + 257, chunk};
+  HIPCHK(c, launch_topk(e, (size_t)n, kk, e_out, idx_out, w, (hipStream_t)stream));
+  return MMO_OK;
+}
+
+int mmo_topk_dev_noalloc(mmo_ctx* c, const double* e, int64_t n, int k, double* e_out,
+                         int64_t* idx_out, void* stream) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!e || n <= 0 || k <= 0 || k > kTopkMax || !e_out || !idx_out)
+    return fail(c, MMO_ERR_BAD_ARG, "bad top-k arguments (n %lld, k %d, k <= %d)", (long long)n,
+                k, kTopkMax);
+  const uint64_t max_n = (uint64_t)kTopkNoallocChunks * kTopkChunk;
+  if ((uint64_t)n > max_n)
+    return fail(c, MMO_ERR_BAD_ARG,
+                "top-k without allocation: n = %lld exceeds the scratch limit of %llu energies",
+                (long long)n, (unsigned long long)max_n);
+  void* base = nullptr;
+  hipError_t se = hipErrorInvalidDevice;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) {
+      base = g_scratch_ptr[c->device];
+      se = g_scratch_err[c->device];
+    }
+  }
+  if (!base)
+    return fail(c, MMO_ERR_DEVICE, "top-k scratch of device %d unavailable: %s", c->device,
+                hipGetErrorString(se));
+  unsigned char* b = static_cast<unsigned char*>(base);
+  const int kk = (int)std::min<int64_t>(k, n);
+  TopkScratch w{reinterpret_cast<uint64_t*>(b), reinterpret_cast<unsigned*>(b + 256),
+                reinterpret_cast<unsigned*>(b + 1280), reinterpret_cast<unsigned*>(b + 1536),
+                kTopkChunk};
   HIPCHK(c, launch_topk(e, (size_t)n, kk, e_out, idx_out, w, (hipStream_t)stream));
   return MMO_OK;
 }

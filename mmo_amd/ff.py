@@ -280,6 +280,16 @@ class Context:
                                      _stream_ptr(stream)))
         return vals, idx
 
+    def topk_noalloc(self, e, k, vals, idx, stream=None):
+        """topk into caller tensors (vals FP64, idx int64, min(k, n) each) through the
+        entry point that allocates nothing (per-GPU static scratch; calls that overlap
+        on one GPU race)."""
+        n = e.numel()
+        self._check(lib.mmo_topk_dev_noalloc(self.h, ct.c_void_p(e.data_ptr()), ct.c_int64(n),
+                                             int(k), ct.c_void_p(vals.data_ptr()),
+                                             ct.c_void_p(idx.data_ptr()), _stream_ptr(stream)))
+        return vals, idx
+
     def tune(self, **kv):
         for k, v in kv.items():
             self._check(lib.mmo_tune_set(self.h, k.encode(), float(v)))
