@@ -25,6 +25,8 @@
  *   - A context belongs to the process that created it: calls from a forked child
  *     fail with MMO_ERR_FORK (HIP state does not survive fork()).
  *   - A context must be used by one thread at a time.
+ *   - Batches, Monte Carlo runs and minimisers may be destroyed before or after their
+ *     context (or batch): a destroy call reads nothing but its own handle.
  */
 #ifndef MMO_FF_H
 #define MMO_FF_H
@@ -269,6 +271,20 @@ int mmo_mc_run(mmo_mc* mc, int nsteps, void* stream);
 int mmo_mc_run_graph(mmo_mc* mc, int nsteps, int nreplay, void* stream);
 /* Plain form for comparison: one launch per ligand (its chains only). */
 int mmo_mc_run_per_ligand(mmo_mc* mc, int nsteps, void* stream);
+/* Monte Carlo-plus-minimisation: after every proposal (rigid or torsion move) the chain
+ * runs `iters` L-BFGS energy + gradient evaluations (the batched minimiser's algorithm,
+ * history pairs `history`, first steps capped at max_step, convergence at gtol) over the
+ * proposal's DOF vector [t, alpha, beta, gamma, tau..] (torsions only with
+ * MMO_MC_TWEAK_RBONDS), starting from the proposal itself, and the Metropolis test, the
+ * E_inter == 0 and hard-ROI resets, the best pose and the step adaptation then apply to
+ * the minimised pose (the ROI test to its centre).  The reference has no such mode: its
+ * MC (Lds.simulate_lig, src/lds.ml:741-1000) and its minimiser (Lds.minimize_ligand,
+ * src/lds.ml:1138-1297) are separate; this composes them per frame in the persistent
+ * launch.  iters = 0 switches it off.  The start pose is scored, not minimised. */
+int mmo_mc_set_minimizer(mmo_mc* mc, int iters, int history, double max_step, double gtol);
+/* Energy + gradient evaluations of the local minimisations so far: per chain (nullable)
+ * and in total (nullable). */
+int mmo_mc_min_evals(mmo_mc* mc, int32_t* evals, int64_t* total);
 int mmo_mc_natoms(const mmo_mc* mc);
 /* Per chain (host arrays, each nullable): best_e[k]; cur_e[3k..] = E, E_inter,
  * E_intra of the current state; best pose rotation and position; info[8k..] = frame,

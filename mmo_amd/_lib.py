@@ -59,6 +59,8 @@ SIGNATURES = {
     "mmo_mc_run": (_i, [_p, _i, _p]),
     "mmo_mc_run_graph": (_i, [_p, _i, _i, _p]),
     "mmo_mc_run_per_ligand": (_i, [_p, _i, _p]),
+    "mmo_mc_set_minimizer": (_i, [_p, _i, _i, _d, _d]),
+    "mmo_mc_min_evals": (_i, [_p, _p, _p]),
     "mmo_mc_natoms": (_i, [_p]),
     "mmo_mc_get": (_i, [_p, _p, _p, _p, _p, _p, _p]),
     "mmo_mc_coords": (_i, [_p, _p, _p, _p]),

@@ -122,6 +122,10 @@ struct mmo_ctx {
   int cull_k = 1;
   int mc_waves_per_cu = 16;
   int mc_wg_waves = 16;       // Monte Carlo chains (wavefronts) per workgroup
+  int min_fused = 1;          // minimiser: all iterations in one persistent launch
+  int min_f64 = 0;            // fused minimiser: FP64 pair arithmetic (parity mode)
+  int min_waves = 16;         // fused minimiser: wavefronts per workgroup
+  int min_rec_lds = 1;        // fused minimiser: receptor in LDS when >= 8 wavefronts fit
   int cull_waves = 16;         // rigid scan: wavefronts per workgroup
   int pose_waves = 16;         // batches (one wavefront per pose): wavefronts per workgroup
   CellGrid cg{};              // scan and grid kernels: cubic cells
@@ -137,6 +141,7 @@ struct mmo_ctx {
   // ligand registry
   std::vector<LigHost> ligs;
   bool lig_dirty = true;
+  int lig_version = 0;         // bumped by every registry upload (device pointers may move)
   int max_atoms = 0;
   DevBuf<LigInfo> d_ligs;
   DevBuf<LigAtom> d_latoms;
@@ -175,6 +180,8 @@ struct mmo_ctx {
 
 struct mmo_mc {
   mmo_ctx* ctx = nullptr;
+  int device = 0;      // the context's, kept so that destruction never reads the context
+  pid_t pid = 0;
   int nchain = 0, natoms = 0, nrb = 0, max_atoms = 0, inter_mode = MMO_SHIFTED, flags = 0;
   int total_frames = 0;
   uint64_t seed = 0;
@@ -188,10 +195,17 @@ struct mmo_mc {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
+  int graph_version = -1;      // ligand registry version the graph was captured with
+  // local minimisation after every proposal (mmo_mc_set_minimizer)
+  int min_iters = 0, min_m = 8;
+  double min_max_step = 0.2, min_gtol = 1e-6;
+  DevBuf<double> d_min_base, d_min_S, d_min_Y, d_min_rho;
 };
 
 struct mmo_batch {
   mmo_ctx* ctx = nullptr;
+  int device = 0;      // the context's, kept so that destruction never reads the context
+  pid_t pid = 0;
   int npose = 0, natoms = 0, ndof = 0, ntiles = 0, max_atoms = 0, max_rb = 0, max_rg = 0;
   std::vector<int> pose_lig, atom_off, dof_off, perm;
   std::vector<Tile> tiles;
@@ -199,7 +213,7 @@ struct mmo_batch {
   // for workgroups of order_waves wavefronts (0: not dealt yet)
   std::vector<int> by_size, order;
   int order_waves = 0;
-  DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm, d_order;
+  DevBuf<int> d_pose_lig, d_atom_off, d_dof_off, d_perm, d_order, d_by_size;
   DevBuf<Tile> d_tiles;
   DevBuf<double> d_xyz, d_e, d_g, d_part, d_conf;
 };
@@ -393,6 +407,7 @@ int sync_ligands(mmo_ctx* c) {
   if ((r = upload(c, c->d_sweep, swp.data(), swp.size(), s))) return r;
   HIPCHK(c, hipStreamSynchronize(s));
   c->lig_dirty = false;
+  ++c->lig_version;
   return MMO_OK;
 }
 
@@ -529,7 +544,7 @@ int mmo_ctx_create(int device, mmo_ctx** out) {
     delete c;
     return fail(nullptr, MMO_ERR_DEVICE, "stream creation failed");
   }
-  for (const char* k : {"inter", "interp", "intra", "grid", "mc"}) {
+  for (const char* k : {"inter", "interp", "intra", "grid", "mc", "min"}) {
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
       delete c;
@@ -995,6 +1010,8 @@ int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch**
   *out = nullptr;
   mmo_batch* b = new mmo_batch();
   b->ctx = c;
+  b->device = c->device;
+  b->pid = c->pid;
   b->npose = npose;
   b->pose_lig.assign(pose_lig, pose_lig + npose);
   b->atom_off.resize(npose + 1);
@@ -1050,10 +1067,10 @@ int mmo_batch_create(mmo_ctx* c, const int32_t* pose_lig, int npose, mmo_batch**
 
 void mmo_batch_destroy(mmo_batch* b) {
   if (!b) return;
-  if (getpid() == b->ctx->pid) {
-    (void)hipSetDevice(b->ctx->device);
+  if (getpid() == b->pid) {
+    (void)hipSetDevice(b->device);
     b->d_pose_lig.release(); b->d_atom_off.release(); b->d_dof_off.release(); b->d_perm.release();
-    b->d_order.release();
+    b->d_order.release(); b->d_by_size.release();
     b->d_tiles.release(); b->d_xyz.release(); b->d_e.release(); b->d_g.release(); b->d_part.release();
     b->d_conf.release();
   }
@@ -1434,6 +1451,8 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
   *out = nullptr;
   mmo_mc* m = new mmo_mc();
   m->ctx = c;
+  m->device = c->device;
+  m->pid = c->pid;
   m->nchain = nchain;
   m->inter_mode = inter_mode;
   m->flags = flags;
@@ -1548,6 +1567,10 @@ static McArgs mc_args(mmo_mc* m, int nsteps, int nwork, const int* work) {
   a.rres = c->rres.p;
   a.nrec = c->nrec; a.sweep = c->mc_sweep;
   a.cg = c->cgp; a.g = c->g;
+  a.min_iters = m->min_iters; a.min_m = m->min_m; a.min_max_step = m->min_max_step;
+  a.min_min_step = 1e-10; a.min_gtol = m->min_gtol; a.min_c1 = 1e-4;
+  a.min_base = m->d_min_base.p; a.min_S = m->d_min_S.p; a.min_Y = m->d_min_Y.p;
+  a.min_rho = m->d_min_rho.p;
   return a;
 }
 
@@ -1579,6 +1602,7 @@ int mmo_mc_run(mmo_mc* m, int nsteps, void* stream) {
   int r = check_ctx(c);
   if (r) return r;
   if (nsteps <= 0) return fail(c, MMO_ERR_BAD_ARG, "nsteps must be positive");
+  if ((r = sync_ligands(c))) return r;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   McArgs a = mc_args(m, nsteps, m->nchain, m->d_work.p);
   timer_begin(c, "mc", s);
@@ -1593,8 +1617,10 @@ int mmo_mc_run_graph(mmo_mc* m, int nsteps, int nreplay, void* stream) {
   int r = check_ctx(c);
   if (r) return r;
   if (nsteps <= 0 || nreplay <= 0) return fail(c, MMO_ERR_BAD_ARG, "bad graph arguments");
+  if ((r = sync_ligands(c))) return r;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  if (m->gexec && m->graph_steps != nsteps) {
+  // a graph holds the registry's device pointers: capture again when they may have moved
+  if (m->gexec && (m->graph_steps != nsteps || m->graph_version != c->lig_version)) {
     (void)hipGraphExecDestroy(m->gexec);
     (void)hipGraphDestroy(m->graph);
     m->gexec = nullptr;
@@ -1612,6 +1638,7 @@ int mmo_mc_run_graph(mmo_mc* m, int nsteps, int nreplay, void* stream) {
     m->graph = g;
     HIPCHK(c, hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
     m->graph_steps = nsteps;
+    m->graph_version = c->lig_version;
   }
   if (s != c->stream) {
     // order the replays after the caller's prior work
@@ -1640,6 +1667,7 @@ int mmo_mc_run_per_ligand(mmo_mc* m, int nsteps, void* stream) {
   int r = check_ctx(c);
   if (r) return r;
   if (nsteps <= 0) return fail(c, MMO_ERR_BAD_ARG, "nsteps must be positive");
+  if ((r = sync_ligands(c))) return r;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   timer_begin(c, "mc", s);
   for (const auto& gr : m->groups) {
@@ -1647,6 +1675,60 @@ int mmo_mc_run_per_ligand(mmo_mc* m, int nsteps, void* stream) {
     HIPCHK(c, mc_launch(m, a, gr.second, s));
   }
   timer_end(c, "mc", s);
+  return MMO_OK;
+}
+
+int mmo_mc_set_minimizer(mmo_mc* m, int iters, int history, double max_step, double gtol) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  if (iters < 0 || history < 1 || history > kMinMaxHist)
+    return fail(c, MMO_ERR_BAD_ARG, "minimiser: iters %d, history %d outside 1..%d", iters,
+                history, kMinMaxHist);
+  if (iters > 0 && m->inter_mode == MMO_SHIFTED && !c->cullp_ok)
+    return fail(c, MMO_ERR_BAD_ARG, "minimisation in the direct form needs the receptor cell grid");
+  if (iters > 0) {
+    McArgs a{};
+    a.max_atoms = m->max_atoms;
+    a.min_iters = iters;
+    if (mc_lds_bytes(a, 1, false) > kMaxLds)
+      return fail(c, MMO_ERR_BAD_ARG, "ligands of %d atoms exceed the minimising chains' LDS",
+                  m->max_atoms);
+    const size_t nc = std::max(m->nchain, 1);
+    hipError_t e = m->d_min_base.ensure(3 * (size_t)std::max(m->natoms, 1));
+    if (e == hipSuccess) e = m->d_min_S.ensure(nc * history * kWave);
+    if (e == hipSuccess) e = m->d_min_Y.ensure(nc * history * kWave);
+    if (e == hipSuccess) e = m->d_min_rho.ensure(nc * history);
+    if (e != hipSuccess) return fail(c, MMO_ERR_NOMEM, "minimiser buffers: %s", hipGetErrorString(e));
+  }
+  m->min_iters = iters;
+  m->min_m = history;
+  if (max_step > 0.0) m->min_max_step = max_step;
+  if (gtol > 0.0) m->min_gtol = gtol;
+  if (m->gexec) {   // the captured launch no longer matches
+    (void)hipGraphExecDestroy(m->gexec);
+    (void)hipGraphDestroy(m->graph);
+    m->gexec = nullptr;
+    m->graph = nullptr;
+  }
+  return MMO_OK;
+}
+
+int mmo_mc_min_evals(mmo_mc* m, int32_t* evals, int64_t* total) {
+  if (!m) return fail(nullptr, MMO_ERR_BAD_ARG, "NULL Monte Carlo handle");
+  mmo_ctx* c = m->ctx;
+  int r = check_ctx(c);
+  if (r) return r;
+  std::vector<McState> st(m->nchain);
+  HIPCHK(c, hipDeviceSynchronize());
+  HIPCHK(c, hipMemcpy(st.data(), m->d_st.p, st.size() * sizeof(McState), hipMemcpyDeviceToHost));
+  int64_t sum = 0;
+  for (int k = 0; k < m->nchain; ++k) {
+    if (evals) evals[k] = st[k].min_evals;
+    sum += st[k].min_evals;
+  }
+  if (total) *total = sum;
   return MMO_OK;
 }
 
@@ -1691,14 +1773,15 @@ int mmo_mc_coords(mmo_mc* m, double* conf, double* best_xyz, double* rb_dr) {
 
 void mmo_mc_destroy(mmo_mc* m) {
   if (!m) return;
-  if (getpid() == m->ctx->pid) {
-    (void)hipSetDevice(m->ctx->device);
+  if (getpid() == m->pid) {
+    (void)hipSetDevice(m->device);
     (void)hipDeviceSynchronize();
     if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
     m->d_chain_lig.release(); m->d_chain_atom.release(); m->d_chain_rb.release();
     m->d_work.release(); m->d_work_by_lig.release(); m->d_queue.release(); m->d_st.release();
     m->d_conf.release(); m->d_best.release(); m->d_dr.release(); m->d_undo.release(); m->d_sw.release();
+    m->d_min_base.release(); m->d_min_S.release(); m->d_min_Y.release(); m->d_min_rho.release();
   }
   delete m;
 }
@@ -1909,10 +1992,13 @@ int mmo_topk_dev_noalloc(mmo_ctx* c, const double* e, int64_t n, int k, double* 
 // ------------------------------------------------------------- L-BFGS minimiser
 struct mmo_min {
   mmo_batch* b = nullptr;
+  int device = 0;      // kept so that destruction never reads the batch or the context
+  pid_t pid = 0;
   int m = 8, flags = 0;
   double max_step = 1.0, gtol = 1e-6;
   DevBuf<double> x, g, d, xt, gt, e, rad, S, Y, rho, lo, hi;
   DevBuf<MinPose> st;
+  DevBuf<int> queue;
 };
 
 int mmo_min_create(mmo_batch* b, int history, int flags, const double* lo, const double* hi,
@@ -1931,6 +2017,8 @@ int mmo_min_create(mmo_batch* b, int history, int flags, const double* lo, const
       return fail(c, MMO_ERR_BAD_ARG, "pose %d has more than %d DOFs", p, kWave);
   mmo_min* z = new mmo_min;
   z->b = b;
+  z->device = c->device;
+  z->pid = c->pid;
   z->m = history;
   z->flags = flags;
   if (max_step > 0.0) z->max_step = max_step;
@@ -1946,6 +2034,12 @@ int mmo_min_create(mmo_batch* b, int history, int flags, const double* lo, const
   if (e == hipSuccess) e = z->Y.ensure((size_t)history * nd);
   if (e == hipSuccess) e = z->rho.ensure((size_t)history * np);
   if (e == hipSuccess) e = z->st.ensure(np);
+  if (e == hipSuccess) e = z->queue.ensure(1);
+  if (e == hipSuccess && b->d_by_size.n < (size_t)b->npose)
+    e = b->d_by_size.ensure(b->npose) == hipSuccess
+            ? hipMemcpyAsync(b->d_by_size.p, b->by_size.data(), b->npose * sizeof(int),
+                             hipMemcpyHostToDevice, s)
+            : hipErrorOutOfMemory;
   if (e == hipSuccess && lo) e = z->lo.ensure(nd);
   if (e == hipSuccess && hi) e = z->hi.ensure(nd);
   if (e == hipSuccess && lo) e = hipMemcpyAsync(z->lo.p, lo, (size_t)b->ndof * sizeof(double), hipMemcpyHostToDevice, s);
@@ -1966,11 +2060,9 @@ int mmo_min_run_dev(mmo_min* z, double* dofs, int iters, double* e_out, void* st
   int r = check_ctx(c);
   if (r) return r;
   if (!dofs || iters < 0) return fail(c, MMO_ERR_BAD_ARG, "bad minimiser arguments");
+  // the kernels read the ligand registry: upload it if ligands were added since
+  if ((r = sync_ligands(c))) return r;
   hipStream_t s = (hipStream_t)stream;
-  HIPCHK(c, hipMemcpyAsync(z->xt.p, dofs, (size_t)b->ndof * sizeof(double),
-                           hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(z->x.p, dofs, (size_t)b->ndof * sizeof(double),
-                           hipMemcpyDeviceToDevice, s));
   HIPCHK(c, launch_min_reset(z->st.p, b->npose, s));
   MinArgs a{};
   a.npose = b->npose; a.m = z->m; a.ndof_total = b->ndof; a.dof_off = b->d_dof_off.p;
@@ -1978,6 +2070,54 @@ int mmo_min_run_dev(mmo_min* z, double* dofs, int iters, double* e_out, void* st
   a.rad = z->rad.p; a.S = z->S.p; a.Y = z->Y.p; a.rho = z->rho.p; a.st = z->st.p;
   a.lo = z->lo.p; a.hi = z->hi.p;
   a.max_step = z->max_step; a.min_step = 1e-10; a.gtol = z->gtol; a.c1 = 1e-4;
+  const int inter = z->flags & (MMO_F_INTER_SHIFTED | MMO_F_INTER_GLOBAL | MMO_F_INTER_INTERP);
+  if (c->min_fused && inter != 0 && inter != MMO_F_INTER_GLOBAL &&
+      (inter != MMO_F_INTER_SHIFTED || c->cullp_ok) &&
+      (inter != MMO_F_INTER_INTERP || c->has_grid)) {
+    // every iteration of every pose in one persistent launch (one wavefront per pose)
+    if (iters == 0) return MMO_OK;
+    MinPersistArgs p{};
+    p.ligs = c->d_ligs.p; p.latoms = c->d_latoms.p; p.lig_xyz = c->d_lig_xyz.p;
+    p.atom_order = c->d_sweep.p; p.rb_left = c->d_rb_left.p; p.rb_right = c->d_rb_right.p;
+    p.rg_off = c->d_rg_off.p; p.rg_atoms = c->d_rg_atoms.p; p.pair_ij = c->d_pair_ij.p;
+    p.pair_par = c->d_pair_par.p;
+    p.npose = b->npose; p.max_atoms = b->max_atoms; p.ndof_total = b->ndof; p.iters = iters;
+    p.m = z->m; p.pose_lig = b->d_pose_lig.p; p.dof_off = b->d_dof_off.p; p.order = b->d_by_size.p;
+    p.x = dofs; p.lo = z->lo.p; p.hi = z->hi.p; p.S = z->S.p; p.Y = z->Y.p; p.rho = z->rho.p;
+    p.st = z->st.p; p.e_out = e_out; p.queue = z->queue.p; p.intra = (z->flags & MMO_F_INTRA) != 0;
+    p.max_step = a.max_step; p.min_step = a.min_step; p.gtol = a.gtol; p.c1 = a.c1;
+    p.fr = c->fr; p.rpos = c->rpos.p; p.rab = c->rab.p; p.rres = c->rres.p; p.nrec = c->nrec;
+    p.cg = c->cgp; p.g = c->g;
+    const int kind = inter == MMO_F_INTER_INTERP ? kDofInterGrid
+                     : c->min_f64 ? kDofInterF64 : kDofInterAtom;
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    int waves = c->min_waves;
+    while (waves > 1 && min_persist_lds_bytes(p, waves, false) > kMaxLds) --waves;
+    if (min_persist_lds_bytes(p, 1, false) > kMaxLds)
+      return fail(c, MMO_ERR_BAD_ARG, "ligands of %d atoms exceed the minimiser's LDS", b->max_atoms);
+    bool rec_lds = false;
+    if (kind != kDofInterGrid && c->min_rec_lds) {
+      int wl = waves;
+      while (wl > 1 && min_persist_lds_bytes(p, wl, true) > kMaxLds) --wl;
+      if (wl >= 8 && min_persist_lds_bytes(p, wl, true) <= kMaxLds) {
+        rec_lds = true;
+        waves = wl;
+      }
+    }
+    const int blocks = std::max(1, std::min((b->npose + waves - 1) / waves, ncu * std::max(1, 16 / waves)));
+    HIPCHK(c, launch_fill_int(z->queue.p, 1, 0, s));
+    timer_begin(c, "min", s);
+    HIPCHK(c, launch_min_persist(p, kind, blocks, waves, rec_lds, s));
+    timer_end(c, "min", s);
+    return MMO_OK;
+  }
+  // one launch sequence per iteration: batched evaluation, then the update
+  HIPCHK(c, hipMemcpyAsync(z->xt.p, dofs, (size_t)b->ndof * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(z->x.p, dofs, (size_t)b->ndof * sizeof(double),
+                           hipMemcpyDeviceToDevice, s));
   for (int it = 0; it < iters; ++it) {
     r = mmo_batch_score_dofs_dev(b, z->flags | MMO_F_GRAD, z->xt.p, nullptr, z->e.p, z->gt.p,
                                  z->rad.p, stream);
@@ -2015,13 +2155,14 @@ int mmo_min_get(mmo_min* z, int32_t* info, double* e_start) {
 
 void mmo_min_destroy(mmo_min* z) {
   if (!z) return;
-  if (getpid() == z->b->ctx->pid) {
-    (void)hipSetDevice(z->b->ctx->device);
+  if (getpid() == z->pid) {
+    (void)hipSetDevice(z->device);
     (void)hipDeviceSynchronize();
     for (DevBuf<double>* v : {&z->x, &z->g, &z->d, &z->xt, &z->gt, &z->e, &z->rad, &z->S, &z->Y,
                               &z->rho, &z->lo, &z->hi})
       v->release();
     z->st.release();
+    z->queue.release();
   }
   delete z;
 }
@@ -2066,6 +2207,10 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "mc_sweep" && (iv == 0 || iv == 1)) c->mc_sweep = iv;
   else if (k == "row_clip" && iv >= 0 && iv <= (1 << 20)) c->row_clip = c->cg.clip_min = iv;
   else if (k == "cell_xsub" && iv >= 1 && iv <= 16) c->cell_xsub = iv;
+  else if (k == "min_fused" && (iv == 0 || iv == 1)) c->min_fused = iv;
+  else if (k == "min_f64" && (iv == 0 || iv == 1)) c->min_f64 = iv;
+  else if (k == "min_waves" && iv >= 1 && iv <= 16) c->min_waves = iv;
+  else if (k == "min_rec_lds" && (iv == 0 || iv == 1)) c->min_rec_lds = iv;
   else return fail(c, MMO_ERR_BAD_ARG, "bad tuning key/value %s=%g", key, v);
   return MMO_OK;
 }
@@ -2089,6 +2234,10 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "mc_sweep") return c->mc_sweep;
   if (k == "row_clip") return c->row_clip;
   if (k == "cell_xsub") return c->cell_xsub;
+  if (k == "min_fused") return c->min_fused;
+  if (k == "min_f64") return c->min_f64;
+  if (k == "min_waves") return c->min_waves;
+  if (k == "min_rec_lds") return c->min_rec_lds;
   return -1.0;
 }
 

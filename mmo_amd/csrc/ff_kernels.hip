@@ -16,6 +16,7 @@
 
 #include "common.h"
 #include "cull.h"
+#include "dof.h"
 #include "kernels.h"
 
 namespace mmo {
@@ -656,39 +657,6 @@ struct InterpArgs {
   int grad_accumulate;
 };
 
-template <bool GRAD>
-__device__ __forceinline__ double trilin(const GridDev& g, const float* arr, double px, double py,
-                                         double pz, bool& ok, double* dg) {
-  // src/G3D.ml:97-157: i0 = int_of_float (x * one_div_step) (truncation)
-  const double fx = px * g.ods, fy = py * g.ods, fz = pz * g.ods;
-  // cell test in floating point before the conversion (also rejects NaN and values
-  // that would saturate the integer conversion); equals 0 <= i0 < nx - 1 with i0 the
-  // truncation of a non-negative fx
-  if (!(px >= 0.0 && py >= 0.0 && pz >= 0.0 && fx < (double)(g.nx - 1) &&
-        fy < (double)(g.ny - 1) && fz < (double)(g.nz - 1))) {
-    ok = false;
-    return 0.0;
-  }
-  const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-  const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
-  const size_t b = i0 + j0 * nx + k0 * nxy;
-  const double A = arr[b], B = arr[b + 1], C = arr[b + 1 + nx], D = arr[b + nx];
-  const double E = arr[b + nxy], F = arr[b + 1 + nxy], G = arr[b + 1 + nx + nxy],
-               H = arr[b + nx + nxy];
-  const double wlx = (px - i0 * g.step) * g.ods, wly = (py - j0 * g.step) * g.ods,
-               wlz = (pz - k0 * g.step) * g.ods;
-  const double whx = 1.0 - wlx, why = 1.0 - wly, whz = 1.0 - wlz;
-  if (GRAD) {
-    dg[0] = g.ods * ((B - A) * why * whz + (C - D) * wly * whz + (F - E) * why * wlz + (G - H) * wly * wlz);
-    dg[1] = g.ods * ((D - A) * whx * whz + (C - B) * wlx * whz + (H - E) * whx * wlz + (G - F) * wlx * wlz);
-    dg[2] = g.ods * ((E - A) * whx * why + (F - B) * wlx * why + (G - C) * wlx * wly + (H - D) * whx * wly);
-  }
-  // corner order a..h, src/G3D.ml:144-151
-  return A * (whx * why * whz) + B * (wlx * why * whz) + C * (wlx * wly * whz) +
-         D * (whx * wly * whz) + E * (whx * why * wlz) + F * (wlx * why * wlz) +
-         G * (wlx * wly * wlz) + H * (whx * wly * wlz);
-}
-
 template <bool GRAD, class Src>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs<Src> a) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -723,6 +691,12 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs<Src> a) {
   e = group_sum(e, split);
   bad = group_or(bad, split);
   if (pose_ok && aslot == 0) a.e_out[(size_t)pose * a.e_stride] = bad ? __builtin_nan("") : e;
+}
+
+__global__ void k_fill_int(int* p, size_t n, int v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 __global__ void k_fill(double* p, size_t n, double v) {
@@ -926,6 +900,13 @@ hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int* p, size_t n, int v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_int, dim3(blocks), dim3(256), 0, s, p, n, v);
   return hipGetLastError();
 }
 

@@ -117,6 +117,7 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
                               const double* trans, const int* tperm, int ntrans, double* e_out,
                               hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
+hipError_t launch_fill_int(int* p, size_t n, int v, hipStream_t s);
 hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s);
 // one pose per wavefront, shifted form, receptor culled per 8-atom sub-block
 struct PoseCullArgs {
@@ -206,6 +207,51 @@ struct MinArgs {
   double max_step, min_step, gtol, c1;
 };
 hipError_t launch_lbfgs(const MinArgs& a, hipStream_t s);
+
+// energy forms of the DOF evaluator (dofmin.h; the Monte Carlo kernel's numbering)
+enum { kDofInterGrid = 0, kDofInterAtom = 1, kDofInterF64 = 3 };
+
+// The whole minimisation of a batch in one persistent launch: each wavefront takes poses
+// from a queue and runs all iterations of one pose (evaluation and update) without
+// leaving the kernel.  inter: kDofInterAtom (shifted, FP32 per-atom sweep), kDofInterF64
+// or kDofInterGrid (dofmin.h); the batch's DOF vectors apply to the registered conformers.
+struct MinPersistArgs {
+  const LigInfo* ligs;
+  const LigAtom* latoms;
+  const double* lig_xyz;        // registered centred conformers (the base of every pose)
+  const int* atom_order;
+  const int* rb_left;
+  const int* rb_right;
+  const int* rg_off;
+  const int* rg_atoms;
+  const uint32_t* pair_ij;
+  const double4* pair_par;
+  int npose, max_atoms, ndof_total, iters, m;
+  const int* pose_lig;
+  const int* dof_off;
+  const int* order;             // nullable: queue position -> pose (largest ligands first)
+  double* x;                    // DOF vectors: start in, best point out
+  const double* lo;             // box bounds (ndof_total) or null
+  const double* hi;
+  double* S;                    // history: S[slot * ndof_total + dof], rho[slot * npose + pose]
+  double* Y;
+  double* rho;
+  MinPose* st;
+  double* e_out;                // nullable: best energy per pose
+  int* queue;
+  int intra;
+  double max_step, min_step, gtol, c1;
+  Frame fr;
+  const RecPos* rpos;
+  const RecAB* rab;
+  const RecRes* rres;
+  int nrec;
+  CellGrid cg;
+  GridDev g;
+};
+size_t min_persist_lds_bytes(const MinPersistArgs& a, int waves, bool rec_lds);
+hipError_t launch_min_persist(const MinPersistArgs& a, int inter, int nblocks, int waves,
+                              bool rec_lds, hipStream_t s);
 hipError_t launch_min_reset(MinPose* st, int npose, hipStream_t s);
 hipError_t launch_min_energy(const MinPose* st, int npose, double* e_out, hipStream_t s);
 

@@ -37,6 +37,7 @@ struct McState {
   int frame, rigid_step, conf_step;
   int status, initialized;
   int accepted, resets, ooroi;
+  int min_evals;                    // local minimisation: energy + gradient evaluations
 };
 static_assert(sizeof(McState) % 8 == 0, "McState is copied as 32-bit words into 8-aligned LDS slots");
 
@@ -81,6 +82,15 @@ struct McArgs {
   int sweep;                 // 1: atom_sweep, 0: pose_sweep (direct sum, FP32)
   CellGrid cg;
   GridDev g;
+  // local minimisation after every proposal (Monte Carlo-plus-minimisation): min_iters
+  // L-BFGS evaluations over the proposal's DOFs; per chain the base rows (like conf),
+  // history S, Y (min_m slots of 64 lanes) and rho (min_m)
+  int min_iters, min_m;
+  double min_max_step, min_min_step, min_gtol, min_c1;
+  double* min_base;
+  double* min_S;
+  double* min_Y;
+  double* min_rho;
 };
 
 // waves: chains per workgroup (one wavefront each); rec_lds: the workgroup first copies

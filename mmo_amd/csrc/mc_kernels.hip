@@ -26,6 +26,7 @@
 // Random numbers: Philox4x32-10 keyed by the seed, counter (chain, frame, draw).
 #include "common.h"
 #include "cull.h"
+#include "dofmin.h"
 #include "kernels.h"
 #include "mc.h"
 
@@ -239,6 +240,71 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// ------------------------------------------------- Monte Carlo-plus-minimisation
+// LDS of one chain's minimiser (after the chain slots): the minimised conformer and the
+// adjoint, 6 x max_atoms FP64
+struct MinLds {
+  double *tx, *ty, *tz, *hx, *hy, *hz;
+};
+
+// Local minimisation of the proposal (conformer sx.. with centre c, rotation nrot,
+// position npos): the base rows are the proposal pose centred at the origin,
+// b_j = nrot (x_j - c), so that the DOF vector [npos, 0, 0, 0, 0..] is the proposal
+// itself; min_iters L-BFGS evaluations of E_inter + E_intra (the chain's energy form)
+// over [t, alpha, beta, gamma, tau_0..] (torsions only for flexible ligands).  Returns
+// the energy of the best point and leaves its conformer (before the rigid motion) in
+// m.tx.., its rigid motion in M, T and the pose centre (M mean(conformer) + T) in cpos;
+// E_inter / E_intra of that point go to the chain state.
+template <int INTER>
+__device__ double mc_minimize(const McArgs& a, const LigInfo& li, int chain, bool flex,
+                              bool use_intra, const PoseView& v, const RecView& rv, const MinLds& m,
+                              McState& st, double* M, double* T, double* cpos) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int L = li.n_atoms;
+  double* gbase = a.min_base + 3 * (size_t)a.chain_atom[chain];
+  for (int j = lane; j < L; j += kWave) {   // lane j later reads back the rows it wrote
+    double x, y, z;
+    v.at(j, x, y, z);
+    gbase[3 * j] = x - v.px; gbase[3 * j + 1] = y - v.py; gbase[3 * j + 2] = z - v.pz;
+  }
+  DofEval ev;
+  ev.li = li; ev.latoms = a.latoms; ev.atom_order = a.atom_order; ev.rb_left = a.rb_left;
+  ev.rb_right = a.rb_right; ev.rg_off = a.rg_off; ev.rg_atoms = a.rg_atoms;
+  ev.pair_ij = a.pair_ij; ev.pair_par = a.pair_par; ev.ntors = flex ? li.nrb : 0;
+  ev.intra = use_intra; ev.fr = a.fr; ev.s_pos = rv.pos; ev.s_ab = rv.ab; ev.s_cell = rv.cell;
+  ev.rres = a.rres; ev.cg = a.cg; ev.wl = rv.wl; ev.g = a.g;
+  const int n = 6 + ev.ntors;
+  const bool act = lane < n;
+  const MinCfg cfg{a.min_m, a.min_max_step, a.min_min_step, a.min_gtol, a.min_c1};
+  double* S = a.min_S + (size_t)chain * a.min_m * kWave + lane;
+  double* Y = a.min_Y + (size_t)chain * a.min_m * kWave + lane;
+  double* rho = a.min_rho + (size_t)chain * a.min_m;
+  MinPose s{};
+  double xt = lane == 0 ? v.px : lane == 1 ? v.py : lane == 2 ? v.pz : 0.0;
+  double x = 0.0, g = 0.0, d = 0.0, e_inter = 0.0, e_intra = 0.0;
+  for (int it = 0; it < a.min_iters && __builtin_amdgcn_readfirstlane(s.status) == kMinRunning; ++it) {
+    double gt, ei, ea, rad;
+    const double f = dof_eval<INTER>(ev, gbase, xt, m.tx, m.ty, m.tz, m.hx, m.hy, m.hz, gt, ei, ea, rad);
+    const int acc0 = s.n_acc, init0 = s.init;
+    xt = lbfgs_wave(s, x, g, d, xt, f, act ? gt : 0.0, rad, -__builtin_inf(), __builtin_inf(), act,
+                    S, Y, (size_t)kWave, rho, 1, cfg);
+    if (!init0 || s.n_acc != acc0) { e_inter = ei; e_intra = ea; }   // the accepted point
+  }
+  st.min_evals += s.n_eval;
+  st.prev_e_inter = e_inter;
+  st.prev_e_intra = e_intra;
+  // the best point's conformer and rigid motion
+  double sn, cs;
+  dof_pose(ev, gbase, x, m.tx, m.ty, m.tz, M, nullptr, nullptr, nullptr, T, sn, cs);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int j = lane; j < L; j += kWave) { s0 += m.tx[j]; s1 += m.ty[j]; s2 += m.tz[j]; }
+  const double cx = wave_sum(s0) / L, cy = wave_sum(s1) / L, cz = wave_sum(s2) / L;
+  cpos[0] = (M[0] * cx + M[1] * cy + M[2] * cz) + T[0];
+  cpos[1] = (M[3] * cx + M[4] * cy + M[5] * cz) + T[1];
+  cpos[2] = (M[6] * cx + M[7] * cy + M[8] * cz) + T[2];
+  return s.f;
+}
+
 // --------------------------------------------------------------------- the kernel
 // A workgroup of `waves` wavefronts, each running its own chains; with REC_LDS the
 // workgroup first stages the receptor and its cell table in LDS, shared by all chains.
@@ -247,10 +313,10 @@ __device__ __forceinline__ void wave_sync() {
 // (4 wavefronts per SIMD; the chain's control code then spills ~260-340 B per lane to
 // scratch): measured faster than 8 (about 200 VGPRs, 2 per SIMD) for the direct forms
 // (15.4 M against 14.2 M steps/s) and for the grid form (225 M against 203 M).
-__host__ __device__ constexpr int mc_form_waves(int) { return 16; }
+__host__ __device__ constexpr int mc_form_waves(int, bool min) { return min ? 8 : 16; }
 
-template <bool REC_LDS, int INTER>
-__global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int rec_bytes) {
+template <bool REC_LDS, int INTER, bool MIN>
+__global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a, int rec_bytes) {
   extern __shared__ __align__(16) char mc_lds[];
   RecView rv;
   load_receptor<REC_LDS>(a.rpos, a.rab, a.nrec, a.cg, rv.pos, rv.ab, rv.cell, mc_lds);
@@ -265,12 +331,23 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
       mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 3 * a.max_atoms * sizeof(double) + 15) &
                             ~(size_t)15));
   rv.wl = reinterpret_cast<int*>(s_st + blockDim.x / kWave) + wv * kAtomSweepInts;
+  MinLds ml{};
+  if (MIN) {
+    // after the scratch of every wavefront: 6 x max_atoms FP64 per wavefront
+    double* b = reinterpret_cast<double*>(reinterpret_cast<int*>(s_st + blockDim.x / kWave) +
+                                          (blockDim.x / kWave) * kAtomSweepInts) +
+                (size_t)wv * 6 * a.max_atoms;
+    ml = MinLds{b, b + a.max_atoms, b + 2 * a.max_atoms, b + 3 * a.max_atoms,
+                b + 4 * a.max_atoms, b + 5 * a.max_atoms};
+  }
   const double pi = 3.14159265358979311600;       // 4 atan 1 (src/math.ml:13)
 
   for (;;) {
+    // the next chain: lane 0 takes it from the queue; readlane makes it a scalar, so the
+    // loop exit is uniform
     int qv = 0;
     if (lane == 0) qv = atomicAdd(a.queue, 1);
-    const int q = __shfl(qv, 0, kWave);
+    const int q = __builtin_amdgcn_readlane(qv, 0);
     if (q >= a.nwork) break;
     const int chain = a.work[q];
     // the chain's state in this wavefront's LDS slot (wave-uniform values)
@@ -423,10 +500,17 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
       // ---- energies (stored before the acceptance test, src/lds.ml:900-904)
       PoseView v{sx, sy, sz, {}, ccx, ccy, ccz, npos[0], npos[1], npos[2]};
       for (int k = 0; k < 9; ++k) v.m[k] = nrot[k];
-      if (!rigid && use_intra && flex) st.prev_e_intra = mc_intra(a, li, sx, sy, sz);
-      st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
-      const double curr_e = st.prev_e_inter + st.prev_e_intra;
-      const double dxr = a.roi_x - npos[0], dyr = a.roi_y - npos[1], dzr = a.roi_z - npos[2];
+      double curr_e, cpos[3] = {npos[0], npos[1], npos[2]};
+      double mM[9], mT[3];   // MIN: rigid motion of the minimised pose
+      if constexpr (MIN) {
+        // the proposal, locally minimised; its centre is the pose centre of the ROI test
+        curr_e = mc_minimize<INTER>(a, li, chain, flex, use_intra, v, rv, ml, st, mM, mT, cpos);
+      } else {
+        if (!rigid && use_intra && flex) st.prev_e_intra = mc_intra(a, li, sx, sy, sz);
+        st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
+        curr_e = st.prev_e_inter + st.prev_e_intra;
+      }
+      const double dxr = a.roi_x - cpos[0], dyr = a.roi_y - cpos[1], dzr = a.roi_z - cpos[2];
       const double dist_roi = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
       const bool hard = (a.flags & kMcHardRoi) != 0;
       bool reset = false, metropolis = false;
@@ -470,22 +554,59 @@ __global__ void __launch_bounds__(mc_form_waves(INTER) * 64) k_mc(McArgs a, int 
         else if (just_rotated > -1) sw_process(rsw[just_rotated], accept);
         if (accept) {
           ++st.accepted;
-          for (int k = 0; k < 9; ++k) st.rot[k] = nrot[k];
-          for (int k = 0; k < 3; ++k) st.pos[k] = npos[k];
           st.prev_e = curr_e;
-          if (mg1 > mg0) {   // conf := conf' (already in place): keep it
-            st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
-            mg0 = mg1 = 0;
+          if constexpr (MIN) {
+            // the minimised pose: conformer nrot^T c* (the chain's frame), centre nrot^T
+            // mean(c*), rotation M* nrot, position its pose centre
+            double cm[3] = {0.0, 0.0, 0.0};
+            for (int j = lane; j < L; j += kWave) {
+              const double x0 = ml.tx[j], y0 = ml.ty[j], z0 = ml.tz[j];
+              cx_[j] = nrot[0] * x0 + nrot[3] * y0 + nrot[6] * z0;
+              cy_[j] = nrot[1] * x0 + nrot[4] * y0 + nrot[7] * z0;
+              cz_[j] = nrot[2] * x0 + nrot[5] * y0 + nrot[8] * z0;
+              cm[0] += x0; cm[1] += y0; cm[2] += z0;
+            }
+            for (int k = 0; k < 3; ++k) cm[k] = wave_sum(cm[k]) / L;
+            for (int k = 0; k < 3; ++k)
+              st.center[k] = nrot[k] * cm[0] + nrot[3 + k] * cm[1] + nrot[6 + k] * cm[2];
+            for (int r = 0; r < 3; ++r)
+              for (int cc = 0; cc < 3; ++cc)
+                st.rot[3 * r + cc] = mM[3 * r] * nrot[cc] + mM[3 * r + 1] * nrot[3 + cc] +
+                                     mM[3 * r + 2] * nrot[6 + cc];
+            for (int k = 0; k < 3; ++k) st.pos[k] = cpos[k];
+            mg0 = mg1 = 0;   // conf holds the minimised conformer: nothing to restore
+            wave_sync();
+          } else {
+            for (int k = 0; k < 9; ++k) st.rot[k] = nrot[k];
+            for (int k = 0; k < 3; ++k) st.pos[k] = npos[k];
+            if (mg1 > mg0) {   // conf := conf' (already in place): keep it
+              st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
+              mg0 = mg1 = 0;
+            }
           }
         }
         if (curr_e < st.best_e) {   // best pose, accepted or not (src/lds.ml:959-969)
           st.best_e = curr_e;
-          for (int k = 0; k < 9; ++k) st.best_rot[k] = nrot[k];
-          for (int k = 0; k < 3; ++k) st.best_pos[k] = npos[k];
-          for (int j = lane; j < L; j += kWave) {
-            double x, y, z;
-            v.at(j, x, y, z);
-            gbest[3 * j] = x; gbest[3 * j + 1] = y; gbest[3 * j + 2] = z;
+          if constexpr (MIN) {
+            for (int r = 0; r < 3; ++r)
+              for (int cc = 0; cc < 3; ++cc)
+                st.best_rot[3 * r + cc] = mM[3 * r] * nrot[cc] + mM[3 * r + 1] * nrot[3 + cc] +
+                                          mM[3 * r + 2] * nrot[6 + cc];
+            for (int k = 0; k < 3; ++k) st.best_pos[k] = cpos[k];
+            for (int j = lane; j < L; j += kWave) {
+              const double x0 = ml.tx[j], y0 = ml.ty[j], z0 = ml.tz[j];
+              gbest[3 * j] = (mM[0] * x0 + mM[1] * y0 + mM[2] * z0) + mT[0];
+              gbest[3 * j + 1] = (mM[3] * x0 + mM[4] * y0 + mM[5] * z0) + mT[1];
+              gbest[3 * j + 2] = (mM[6] * x0 + mM[7] * y0 + mM[8] * z0) + mT[2];
+            }
+          } else {
+            for (int k = 0; k < 9; ++k) st.best_rot[k] = nrot[k];
+            for (int k = 0; k < 3; ++k) st.best_pos[k] = npos[k];
+            for (int j = lane; j < L; j += kWave) {
+              double x, y, z;
+              v.at(j, x, y, z);
+              gbest[3 * j] = x; gbest[3 * j + 1] = y; gbest[3 * j + 2] = z;
+            }
           }
         }
         // adaptive rigid steps every 100 rigid frames (src/lds.ml:971-974, :586-600)
@@ -533,13 +654,18 @@ size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
     rec = ((size_t)a.nrec * (sizeof(RecPos) + sizeof(RecAB)) +
            (size_t)(a.cg.nx * a.cg.ny * a.cg.nz + 1) * sizeof(int) + 15) & ~(size_t)15;
   const size_t conf = ((size_t)waves * 3 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
-  return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int));
+  const size_t mins = a.min_iters > 0 ? (size_t)waves * 6 * a.max_atoms * sizeof(double) : 0;
+  return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int)) + mins;
 }
 
 template <bool R, int I>
 static hipError_t mc_allow_lds(int cap) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<R, I>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<R, I, false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  if (e == hipSuccess && I != kMcInterSub)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mc<R, I, I != kMcInterSub>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  return e;
 }
 
 hipError_t mc_prepare() {
@@ -560,10 +686,11 @@ hipError_t mc_prepare() {
 int mc_inter_kind(const McArgs& a) {
   if (a.inter_mode == MMO_INTERP) return kMcInterGrid;
   if (a.flags & kMcFp64) return kMcInterF64;
-  return a.sweep == 1 ? kMcInterAtom : kMcInterSub;
+  // the minimiser's gradients come from the per-atom sweep
+  return (a.sweep == 1 || a.min_iters > 0) ? kMcInterAtom : kMcInterSub;
 }
 
-int mc_max_waves(const McArgs& a) { return mc_form_waves(mc_inter_kind(a)); }
+int mc_max_waves(const McArgs& a) { return mc_form_waves(mc_inter_kind(a), a.min_iters > 0); }
 
 hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipStream_t s) {
   if (waves < 1 || waves > mc_max_waves(a)) return hipErrorInvalidValue;
@@ -573,14 +700,21 @@ hipError_t launch_mc(const McArgs& a, int nblocks, int waves, bool rec_lds, hipS
   const size_t lds = mc_lds_bytes(a, waves, rec_lds);
   const int rec_bytes = (int)(mc_lds_bytes(a, waves, rec_lds) - mc_lds_bytes(a, waves, false));
   const dim3 g(nblocks), b(waves * kWave);
-  switch (mc_inter_kind(a) + (rec_lds ? 8 : 0)) {
-    case kMcInterAtom + 8: hipLaunchKernelGGL((k_mc<true, kMcInterAtom>), g, b, lds, s, a, rec_bytes); break;
-    case kMcInterSub + 8: hipLaunchKernelGGL((k_mc<true, kMcInterSub>), g, b, lds, s, a, rec_bytes); break;
-    case kMcInterF64 + 8: hipLaunchKernelGGL((k_mc<true, kMcInterF64>), g, b, lds, s, a, rec_bytes); break;
-    case kMcInterAtom: hipLaunchKernelGGL((k_mc<false, kMcInterAtom>), g, b, lds, s, a, 0); break;
-    case kMcInterSub: hipLaunchKernelGGL((k_mc<false, kMcInterSub>), g, b, lds, s, a, 0); break;
-    case kMcInterF64: hipLaunchKernelGGL((k_mc<false, kMcInterF64>), g, b, lds, s, a, 0); break;
-    case kMcInterGrid: hipLaunchKernelGGL((k_mc<false, kMcInterGrid>), g, b, lds, s, a, 0); break;
+  const int key = mc_inter_kind(a) + (rec_lds ? 8 : 0) + (a.min_iters > 0 ? 16 : 0);
+  switch (key) {
+    case kMcInterAtom + 8: hipLaunchKernelGGL((k_mc<true, kMcInterAtom, false>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterSub + 8: hipLaunchKernelGGL((k_mc<true, kMcInterSub, false>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterF64 + 8: hipLaunchKernelGGL((k_mc<true, kMcInterF64, false>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterAtom: hipLaunchKernelGGL((k_mc<false, kMcInterAtom, false>), g, b, lds, s, a, 0); break;
+    case kMcInterSub: hipLaunchKernelGGL((k_mc<false, kMcInterSub, false>), g, b, lds, s, a, 0); break;
+    case kMcInterF64: hipLaunchKernelGGL((k_mc<false, kMcInterF64, false>), g, b, lds, s, a, 0); break;
+    case kMcInterGrid: hipLaunchKernelGGL((k_mc<false, kMcInterGrid, false>), g, b, lds, s, a, 0); break;
+    // with local minimisation after every proposal
+    case kMcInterAtom + 24: hipLaunchKernelGGL((k_mc<true, kMcInterAtom, true>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterF64 + 24: hipLaunchKernelGGL((k_mc<true, kMcInterF64, true>), g, b, lds, s, a, rec_bytes); break;
+    case kMcInterAtom + 16: hipLaunchKernelGGL((k_mc<false, kMcInterAtom, true>), g, b, lds, s, a, 0); break;
+    case kMcInterF64 + 16: hipLaunchKernelGGL((k_mc<false, kMcInterF64, true>), g, b, lds, s, a, 0); break;
+    case kMcInterGrid + 16: hipLaunchKernelGGL((k_mc<false, kMcInterGrid, true>), g, b, lds, s, a, 0); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -18,20 +18,10 @@
 // Lanes hold the DOF components (at most 64 per pose); dot products are wavefront
 // reductions in FP64.
 #include "common.h"
+#include "dofmin.h"
 #include "kernels.h"
 
 namespace mmo {
-
-namespace {
-
-__device__ __forceinline__ double dot(double a, double b) { return wave_sum(a * b); }
-
-__device__ __forceinline__ double wave_absmax(double v) {
-  v = fabs(v);
-  return wave_max(v);
-}
-
-}  // namespace
 
 __global__ void __launch_bounds__(256) k_lbfgs(MinArgs a) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -43,105 +33,20 @@ __global__ void __launch_bounds__(256) k_lbfgs(MinArgs a) {
   const int n = a.dof_off[p + 1] - o;
   const bool act = lane < n;
   const size_t i = (size_t)o + (act ? lane : 0);
-  const size_t nt = (size_t)a.ndof_total;
-
-  double ft = a.et[4 * (size_t)p] + a.et[4 * (size_t)p + 1];
-  if (!(a.rad[p] <= 12.0) || !isfinite(ft)) ft = __builtin_inf();
+  const double ft = a.et[4 * (size_t)p] + a.et[4 * (size_t)p + 1];
   const double gti = act ? a.gt[i] : 0.0;
   const double xti = act ? a.xt[i] : 0.0;
   const double lo = (act && a.lo) ? a.lo[i] : -__builtin_inf();
   const double hi = (act && a.hi) ? a.hi[i] : __builtin_inf();
-  s.n_eval += 1;
-
-  double x, g, d;
-  if (!s.init) {
-    // first evaluation: the start point itself
-    x = xti;
-    g = gti;
-    s.f = ft;
-    s.f0 = ft;
-    s.init = 1;
-    s.hist = 0;
-    s.head = 0;
-    if (!isfinite(ft)) s.status = kMinFailed;
-    d = -g;
-    const double dm = wave_absmax(act ? d : 0.0);
-    s.alpha = dm > 0.0 ? fmin(1.0, a.max_step / dm) : 0.0;
-    if (dm <= a.gtol) s.status = kMinConverged;
-  } else {
-    x = act ? a.x[i] : 0.0;
-    g = act ? a.g[i] : 0.0;
-    d = act ? a.d[i] : 0.0;
-    const double sv = xti - x;                 // the step actually taken (after projection)
-    const double gs = dot(act ? g : 0.0, act ? sv : 0.0);
-    if (gs < 0.0 && ft <= s.f + a.c1 * gs) {
-      // accept; curvature pair
-      const double yv = gti - g;
-      const double sy = dot(act ? sv : 0.0, act ? yv : 0.0);
-      const double yy = dot(act ? yv : 0.0, act ? yv : 0.0);
-      if (sy > 1e-12 * yy && sy > 0.0) {
-        const int slot = s.head;
-        if (act) {
-          a.S[(size_t)slot * nt + i] = sv;
-          a.Y[(size_t)slot * nt + i] = yv;
-        }
-        if (lane == 0) a.rho[(size_t)slot * a.npose + p] = 1.0 / sy;
-        s.head = (s.head + 1) % a.m;
-        s.hist = min(s.hist + 1, a.m);
-        s.gamma = sy / yy;
-      }
-      x = xti;
-      g = gti;
-      s.f = ft;
-      s.n_acc += 1;
-      // two-loop recursion, newest pair first
-      double q = act ? g : 0.0;
-      double al[kMinMaxHist];
-      for (int k = 0; k < s.hist; ++k) {
-        const int idx = (s.head - 1 - k + 2 * a.m) % a.m;
-        const double sk = act ? a.S[(size_t)idx * nt + i] : 0.0;
-        const double yk = act ? a.Y[(size_t)idx * nt + i] : 0.0;
-        const double rk = a.rho[(size_t)idx * a.npose + p];
-        al[k] = rk * dot(sk, q);
-        q -= al[k] * yk;
-      }
-      double r = (s.hist > 0 ? s.gamma : 1.0) * q;
-      for (int k = s.hist - 1; k >= 0; --k) {
-        const int idx = (s.head - 1 - k + 2 * a.m) % a.m;
-        const double sk = act ? a.S[(size_t)idx * nt + i] : 0.0;
-        const double yk = act ? a.Y[(size_t)idx * nt + i] : 0.0;
-        const double rk = a.rho[(size_t)idx * a.npose + p];
-        const double b = rk * dot(yk, r);
-        r += sk * (al[k] - b);
-      }
-      d = act ? -r : 0.0;
-      if (dot(d, act ? g : 0.0) >= 0.0) {   // not a descent direction: restart
-        s.hist = 0;
-        s.n_reset += 1;
-        d = act ? -g : 0.0;
-      }
-      const double dm = wave_absmax(d);
-      const double gm = wave_absmax(act ? g : 0.0);
-      s.alpha = dm > 0.0 ? fmin(1.0, a.max_step / dm) : 0.0;
-      if (gm <= a.gtol) s.status = kMinConverged;
-    } else {
-      // backtrack
-      s.alpha *= 0.5;
-      const double dm = wave_absmax(act ? d : 0.0);
-      if (s.alpha * dm < a.min_step) {
-        if (s.hist == 0) {
-          s.status = kMinConverged;        // no decrease along steepest descent
-        } else {                           // drop the history and restart
-          s.hist = 0;
-          s.n_reset += 1;
-          d = act ? -g : 0.0;
-          const double dm2 = wave_absmax(d);
-          s.alpha = dm2 > 0.0 ? fmin(1.0, a.max_step / dm2) : 0.0;
-        }
-      }
-    }
+  double x = 0.0, g = 0.0, d = 0.0;
+  if (s.init && act) {
+    x = a.x[i];
+    g = a.g[i];
+    d = a.d[i];
   }
-  const double xn = (s.status == kMinRunning) ? fmin(hi, fmax(lo, x + s.alpha * d)) : x;
+  const MinCfg cfg{a.m, a.max_step, a.min_step, a.gtol, a.c1};
+  const double xn = lbfgs_wave(s, x, g, d, xti, ft, gti, a.rad[p], lo, hi, act, a.S + i,
+                               a.Y + i, (size_t)a.ndof_total, a.rho + p, (size_t)a.npose, cfg);
   if (act) {
     a.x[i] = x;
     a.g[i] = g;
@@ -162,6 +67,108 @@ __global__ void k_min_reset(MinPose* st, int npose) {
   if (p < npose) {
     MinPose z{};
     st[p] = z;
+  }
+}
+
+// ---------------------------------------------------------- persistent minimiser
+// LDS per wavefront: the conformer and the adjoint (6 x max_atoms FP64) and the
+// atom-sweep scratch; before them, with REC_LDS, the receptor and its cell table.
+__host__ __device__ inline size_t min_wave_lds_bytes(int max_atoms) {
+  return ((size_t)6 * max_atoms * sizeof(double) + kAtomSweepInts * sizeof(int) + 15) & ~(size_t)15;
+}
+
+constexpr int kMinPersistWaves = 16;
+
+template <bool REC_LDS, int INTER>
+__global__ void __launch_bounds__(kMinPersistWaves * 64) k_min_persist(MinPersistArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  const RecPos* s_pos;
+  const RecAB* s_ab;
+  const int* s_cell;
+  load_receptor<REC_LDS>(a.rpos, a.rab, a.nrec, a.cg, s_pos, s_ab, s_cell, smem);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  char* mine = smem + receptor_lds_bytes<REC_LDS>(a.nrec, a.cg) + wv * min_wave_lds_bytes(a.max_atoms);
+  double* tx = reinterpret_cast<double*>(mine);
+  double* ty = tx + a.max_atoms;
+  double* tz = ty + a.max_atoms;
+  double* hx = tz + a.max_atoms;
+  double* hy = hx + a.max_atoms;
+  double* hz = hy + a.max_atoms;
+  int* wl = reinterpret_cast<int*>(hz + a.max_atoms);
+  const MinCfg cfg{a.m, a.max_step, a.min_step, a.gtol, a.c1};
+  for (;;) {
+    // the next pose: lane 0 takes it from the queue, readlane makes it a scalar, so the
+    // loop exit is uniform (with a vector broadcast the compiler treats the exit as
+    // divergent and may send the other lanes round the loop without lane 0's atomic)
+    int qv = 0;
+    if (lane == 0) qv = atomicAdd(a.queue, 1);
+    const int q = __builtin_amdgcn_readlane(qv, 0);
+    if (q >= a.npose) break;
+    const int p = a.order ? a.order[q] : q;
+    const LigInfo li = a.ligs[a.pose_lig[p]];
+    const int o = a.dof_off[p];
+    const int n = a.dof_off[p + 1] - o;
+    const bool act = lane < n;
+    const size_t i = (size_t)o + (act ? lane : 0);
+    DofEval ev;
+    ev.li = li; ev.latoms = a.latoms; ev.atom_order = a.atom_order; ev.rb_left = a.rb_left;
+    ev.rb_right = a.rb_right; ev.rg_off = a.rg_off; ev.rg_atoms = a.rg_atoms;
+    ev.pair_ij = a.pair_ij; ev.pair_par = a.pair_par; ev.ntors = n - 6; ev.intra = a.intra != 0;
+    ev.fr = a.fr; ev.s_pos = s_pos; ev.s_ab = s_ab; ev.s_cell = s_cell; ev.rres = a.rres;
+    ev.cg = a.cg; ev.wl = wl; ev.g = a.g;
+    const double* base = a.lig_xyz + 3 * (size_t)li.atom_base;
+    const double lo = (act && a.lo) ? a.lo[i] : -__builtin_inf();
+    const double hi = (act && a.hi) ? a.hi[i] : __builtin_inf();
+    MinPose s{};
+    double xt = act ? a.x[i] : 0.0, x = 0.0, g = 0.0, d = 0.0;
+    for (int it = 0; it < a.iters && __builtin_amdgcn_readfirstlane(s.status) == kMinRunning; ++it) {
+      double gt, ei, ea, rad;
+      const double f = dof_eval<INTER>(ev, base, xt, tx, ty, tz, hx, hy, hz, gt, ei, ea, rad);
+      xt = lbfgs_wave(s, x, g, d, xt, f, act ? gt : 0.0, rad, lo, hi, act, a.S + i, a.Y + i,
+                      (size_t)a.ndof_total, a.rho + p, (size_t)a.npose, cfg);
+    }
+    if (a.iters > 0) {
+      if (act) a.x[i] = x;
+      if (lane == 0) {
+        a.st[p] = s;
+        if (a.e_out) a.e_out[p] = s.f;
+      }
+    }
+    // all lanes meet here before the next pose (a convergent point: without it the
+    // compiler joins this lane-0 block to the next queue fetch, and the other lanes
+    // would start the next pose without lane 0)
+    wave_lds_sync();
+  }
+}
+
+size_t min_persist_lds_bytes(const MinPersistArgs& a, int waves, bool rec_lds) {
+  return (rec_lds ? receptor_lds_bytes<true>(a.nrec, a.cg) : 0) +
+         (size_t)waves * min_wave_lds_bytes(a.max_atoms);
+}
+
+template <bool R, int I>
+static hipError_t min_persist_go(const MinPersistArgs& a, int nblocks, int waves, hipStream_t s) {
+  const size_t lds = min_persist_lds_bytes(a, waves, R);
+  if (lds > kMaxLds) return hipErrorInvalidConfiguration;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_min_persist<R, I>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_min_persist<R, I>), dim3(nblocks), dim3(waves * kWave), lds, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_min_persist(const MinPersistArgs& a, int inter, int nblocks, int waves,
+                              bool rec_lds, hipStream_t s) {
+  if (a.npose == 0) return hipSuccess;
+  if (waves < 1 || waves > kMinPersistWaves) return hipErrorInvalidValue;
+  switch (inter + (rec_lds ? 8 : 0)) {
+    case kDofInterAtom + 8: return min_persist_go<true, kDofInterAtom>(a, nblocks, waves, s);
+    case kDofInterF64 + 8: return min_persist_go<true, kDofInterF64>(a, nblocks, waves, s);
+    case kDofInterAtom: return min_persist_go<false, kDofInterAtom>(a, nblocks, waves, s);
+    case kDofInterF64: return min_persist_go<false, kDofInterF64>(a, nblocks, waves, s);
+    case kDofInterGrid: return min_persist_go<false, kDofInterGrid>(a, nblocks, waves, s);
+    default: return hipErrorInvalidValue;
   }
 }
 

@@ -19,11 +19,10 @@
 //   v_bar    = (I - a a^T) a_bar / |v|,  v = x_right - x_left
 //   right_bar += v_bar + p_bar,  left_bar -= v_bar.
 #include "common.h"
+#include "dof.h"
 #include "kernels.h"
 
 namespace mmo {
-
-struct Rod { double m[9]; };
 
 // wavefronts per pose of the adjoint when it also evaluates E_intra
 constexpr int kDofWaves = 2;
@@ -35,37 +34,6 @@ __host__ __device__ constexpr size_t dof_coord_bytes(int max_atoms, int arrays) 
 }
 constexpr size_t torsion_lds_bytes(int max_rb, int max_rg) {
   return 2 * (size_t)(max_rb + 3) * sizeof(double) + (size_t)(3 * max_rb + 1 + max_rg) * sizeof(int);
-}
-
-// Rot.of_axis_angle, src/rot.ml:121-146, from the sine and cosine of the angle
-__device__ __forceinline__ Rod rodrigues(double x, double y, double z, double s, double c) {
-  const double o = 1.0 - c;
-  Rod r;
-  r.m[0] = c + x * x * o;     r.m[1] = x * y * o - z * s; r.m[2] = x * z * o + y * s;
-  r.m[3] = x * y * o + z * s; r.m[4] = c + y * y * o;     r.m[5] = y * z * o - x * s;
-  r.m[6] = x * z * o - y * s; r.m[7] = y * z * o + x * s; r.m[8] = c + z * z * o;
-  return r;
-}
-
-// Rot.r_xyz, src/rot.ml:52-66, and its partial derivatives; sn/cs hold the sines and
-// cosines of (alpha, beta, gamma)
-__device__ __forceinline__ void rot_xyz(const double* sn, const double* cs, double* m, double* da,
-                                        double* db, double* dg) {
-  const double as = sn[0], ac = cs[0], bs = sn[1], bc = cs[1], gs = sn[2], gc = cs[2];
-  m[0] = bc * gc;  m[1] = gc * as * bs - ac * gs;  m[2] = as * gs + ac * gc * bs;
-  m[3] = bc * gs;  m[4] = ac * gc + as * bs * gs;  m[5] = ac * bs * gs - gc * as;
-  m[6] = -bs;      m[7] = bc * as;                 m[8] = ac * bc;
-  if (da) {
-    da[0] = 0.0;      da[1] = gc * ac * bs + as * gs; da[2] = ac * gs - as * gc * bs;
-    da[3] = 0.0;      da[4] = -as * gc + ac * bs * gs; da[5] = -as * bs * gs - gc * ac;
-    da[6] = 0.0;      da[7] = bc * ac;               da[8] = -as * bc;
-    db[0] = -bs * gc; db[1] = gc * as * bc;          db[2] = ac * gc * bc;
-    db[3] = -bs * gs; db[4] = as * bc * gs;          db[5] = ac * bc * gs;
-    db[6] = -bc;      db[7] = -bs * as;              db[8] = -ac * bs;
-    dg[0] = -bc * gs; dg[1] = -gs * as * bs - ac * gc; dg[2] = as * gc - ac * gs * bs;
-    dg[3] = bc * gc;  dg[4] = -ac * gs + as * bs * gc; dg[5] = ac * bs * gc + gs * as;
-    dg[6] = 0.0;      dg[7] = 0.0;                   dg[8] = 0.0;
-  }
 }
 
 // The pose's torsion tree staged in LDS before the serial torsion loop, so that the

@@ -6,6 +6,7 @@ CPU path.  Molecules are passed as ``mmo_amd.mol.Mol`` objects or plain dicts wi
 ``Mol.t`` fields (``xs, ys, zs, q, anum``; ``dists`` for E_intra).
 """
 import ctypes as ct
+import sys
 
 import numpy as np
 
@@ -80,7 +81,8 @@ class Context:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        # at interpreter exit the HIP runtime may already be gone: the process frees it all
+        if h and not sys.is_finalizing():
             lib.mmo_ctx_destroy(h)
             self.h = None
 
@@ -323,7 +325,8 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        # at interpreter exit the HIP runtime may already be gone: the process frees it all
+        if h and not sys.is_finalizing():
             lib.mmo_batch_destroy(h)
             self.h = None
 

@@ -8,6 +8,7 @@ All chains of all ligands run in one launch; ``run_graph`` replays that launch f
 captured hipGraph and ``run_per_ligand`` launches once per ligand (the plain form).
 """
 import ctypes as ct
+import sys
 import math
 
 import numpy as np
@@ -48,9 +49,23 @@ class MonteCarlo:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        # at interpreter exit the HIP runtime may already be gone: the process frees it all
+        if h and not sys.is_finalizing():
             lib.mmo_mc_destroy(h)
             self.h = None
+
+    def set_minimizer(self, iters, history=8, max_step=0.2, gtol=1e-6):
+        """Monte Carlo-plus-minimisation: `iters` L-BFGS evaluations after every proposal
+        (0 switches it off)."""
+        self.ctx._check(lib.mmo_mc_set_minimizer(self.h, int(iters), int(history),
+                                                 float(max_step), float(gtol)))
+
+    def min_evals(self):
+        """Energy + gradient evaluations of the local minimisations: (per chain, total)."""
+        per = np.zeros(self.nchain, dtype=np.int32)
+        tot = ct.c_int64()
+        self.ctx._check(lib.mmo_mc_min_evals(self.h, _ptr(per), ct.byref(tot)))
+        return per, tot.value
 
     def run(self, nsteps, stream=None):
         self.ctx._check(lib.mmo_mc_run(self.h, int(nsteps), _stream_ptr(stream) if stream else None))

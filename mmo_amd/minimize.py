@@ -10,6 +10,7 @@ central-difference gradients inside the bounds of src/optim.ml:34-60, this runs 
 on the analytic adjoint gradient for every pose of the batch at once.
 """
 import ctypes as ct
+import sys
 import math
 
 import numpy as np
@@ -52,7 +53,8 @@ class Minimizer:
 
     def __del__(self):
         h = getattr(self, "h", None)
-        if h:
+        # at interpreter exit the HIP runtime may already be gone: the process frees it all
+        if h and not sys.is_finalizing():
             lib.mmo_min_destroy(h)
             self.h = None
 

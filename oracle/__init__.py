@@ -83,6 +83,7 @@ _sig("orc_center", None, [_p, _p, _p, _i, _p])
 _sig("orc_radius", _d, [_p, _p, _p, _i, _p])
 _sig("orc_center_rotate_translate", None, [_p, _p, _p, _i, _p, _p, _p, _p, _p, _p])
 _sig("orc_apply_config", _d, [_p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p])
+_sig("orc_apply_config_jvp", _d, [_p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p])
 _sig("orc_grad_inter", _d, [_i] + _mol + _mol + [_p, _p, _p])
 _sig("orc_grad_intra", _d, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p])
 _sig("orc_grad_interp", _d, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p])
@@ -288,6 +289,41 @@ def apply_config(centered_xyz, rb_left, rb_right, rg_off, rg_atoms, dofs):
                                k.i(rb_right), k.i(rg_off), k.i(rg_atoms), len(rb_left),
                                k.f(dofs), _ptr(o[0]), _ptr(o[1]), _ptr(o[2]))
     return o.T.copy(), rad
+
+
+def apply_config_jvp(centered_xyz, rb_left, rb_right, rg_off, rg_atoms, dofs):
+    """apply_config with forward-mode derivatives: (coords (n,3), radius, tangents
+    (6 + nrb, n, 3)) where tangents[i, j] = d coords[j] / d dofs[i]."""
+    k = _Keep()
+    x = _np.asarray(centered_xyz, dtype=_np.float64)
+    n = len(x)
+    nrb = len(rb_left)
+    o = _np.zeros((3, n))
+    tan = _np.zeros((6 + nrb, n, 3))
+    rad = lib.orc_apply_config_jvp(k.f(x[:, 0]), k.f(x[:, 1]), k.f(x[:, 2]), n, k.i(rb_left),
+                                   k.i(rb_right), k.i(rg_off), k.i(rg_atoms), nrb, k.f(dofs),
+                                   _ptr(o[0]), _ptr(o[1]), _ptr(o[2]), _ptr(tan))
+    return o.T.copy(), rad, tan
+
+
+def dof_energy_grad(base_xyz, lig, dofs, inter_grad, intra=True):
+    """E_inter + E_intra of the pose Optim.apply_config(base_xyz, dofs) and dE/dDOF,
+    FP64: dE/dxyz from inter_grad(x) -> (E_inter, (n,3)) and grad_intra, chained through
+    the tangent pass (gdof_i = sum_j g_j . d x_j / d dof_i).  dofs has 6 + R entries:
+    the first R rotatable bonds of `lig` are turned.  Returns (e_inter, e_intra, gdof,
+    radius after the torsions)."""
+    dofs = _np.asarray(dofs, dtype=_np.float64)
+    r = len(dofs) - 6
+    rg_off = _np.asarray(lig["rg_off"])[:r + 1]
+    x, rad, tan = apply_config_jvp(base_xyz, _np.asarray(lig["rb_left"])[:r],
+                                   _np.asarray(lig["rb_right"])[:r], rg_off,
+                                   _np.asarray(lig["rg_atoms"])[:int(rg_off[-1])], dofs)
+    e_inter, g = inter_grad(x)
+    e_intra = 0.0
+    if intra:
+        e_intra, gi = grad_intra(dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]))
+        g = g + gi
+    return e_inter, e_intra, _np.einsum("ijc,jc->i", tan, g), rad
 
 
 def grad_inter(prot, lig, shifted=True):

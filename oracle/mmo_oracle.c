@@ -421,6 +421,137 @@ double orc_apply_config(const double* cxs, const double* cys, const double* czs,
   return rad;
 }
 
+/* Forward-mode (tangent) derivative of orc_apply_config: tan[(i*n + j)*3 + c] is
+ * d out_j[c] / d dofs[i] for every DOF i (6 + nrb of them).  Each torsion
+ * y = p + R(a, tau)(x - p), p = x_right, a = (x_right - x_left)/|.|, carries the tangents
+ * dy = dp + R(dx - dp) + dtau dR/dtau w + s [da]x w + (1 - c)(da (a.w) + a (da.w)),
+ * w = x - p, da = (I - a a^T) dv / |v|, dR/dtau w = -s w + c a x w + s a (a.w)
+ * (Rodrigues: R = c I + s [a]x + (1 - c) a a^T); the rigid part r_xyz = Rz(g) Ry(b) Rx(a)
+ * (src/rot.ml:48-66) is differentiated factor by factor.  This is this build's exact
+ * DOF gradient for the CPU checks (the reference differentiates numerically,
+ * src/lds.ml:1118-1136); tests pin it against central differences. */
+static void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static void mat3_mult(const double* A, const double* B, double* C) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      C[3 * r + c] = A[3 * r] * B[c] + A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c];
+}
+/* active axis rotations and their derivatives */
+static void act_rx(double t, double* m, double* d) {
+  const double c = cos(t), s = sin(t);
+  const double M[9] = {1, 0, 0, 0, c, -s, 0, s, c}, D[9] = {0, 0, 0, 0, -s, -c, 0, c, -s};
+  memcpy(m, M, sizeof M); memcpy(d, D, sizeof D);
+}
+static void act_ry(double t, double* m, double* d) {
+  const double c = cos(t), s = sin(t);
+  const double M[9] = {c, 0, s, 0, 1, 0, -s, 0, c}, D[9] = {-s, 0, c, 0, 0, 0, -c, 0, -s};
+  memcpy(m, M, sizeof M); memcpy(d, D, sizeof D);
+}
+static void act_rz(double t, double* m, double* d) {
+  const double c = cos(t), s = sin(t);
+  const double M[9] = {c, -s, 0, s, c, 0, 0, 0, 1}, D[9] = {-s, -c, 0, c, -s, 0, 0, 0, 0};
+  memcpy(m, M, sizeof M); memcpy(d, D, sizeof D);
+}
+double orc_apply_config_jvp(const double* cxs, const double* cys, const double* czs, int n,
+                            const int32_t* rb_left, const int32_t* rb_right,
+                            const int32_t* rg_off, const int32_t* rg_atoms, int nrb,
+                            const double* dofs,
+                            double* oxs, double* oys, double* ozs, double* tan) {
+  const int nd = 6 + nrb;
+  double* x = (double*)malloc(sizeof(double) * 3 * n);
+  double* dx = (double*)calloc((size_t)nd * n * 3, sizeof(double));
+  double* dp = (double*)malloc(sizeof(double) * 3 * nd);
+  double* da = (double*)malloc(sizeof(double) * 3 * nd);
+  for (int j = 0; j < n; ++j) { x[3 * j] = cxs[j]; x[3 * j + 1] = cys[j]; x[3 * j + 2] = czs[j]; }
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < nrb; ++k) {
+    const int L = rb_left[k], Rr = rb_right[k];
+    const double* p = x + 3 * Rr;
+    const double v[3] = {p[0] - x[3 * L], p[1] - x[3 * L + 1], p[2] - x[3 * L + 2]};
+    const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double a[3] = {v[0] / nv, v[1] / nv, v[2] / nv};
+    const double tau = dofs[6 + k], cs = cos(tau), sn = sin(tau);
+    double R[9];
+    orc_rot_of_axis_angle(a[0], a[1], a[2], tau, R);
+    for (int i = 0; i < nd; ++i) {
+      const double* dxi = dx + (size_t)i * n * 3;
+      double dv[3];
+      for (int q = 0; q < 3; ++q) {
+        dp[3 * i + q] = dxi[3 * Rr + q];
+        dv[q] = dxi[3 * Rr + q] - dxi[3 * L + q];
+      }
+      const double adv = a[0] * dv[0] + a[1] * dv[1] + a[2] * dv[2];
+      for (int q = 0; q < 3; ++q) da[3 * i + q] = (dv[q] - a[q] * adv) / nv;
+    }
+    const double px = p[0], py = p[1], pz = p[2];
+    for (int m = rg_off[k]; m < rg_off[k + 1]; ++m) {
+      const int j = rg_atoms[m];
+      const double w[3] = {x[3 * j] - px, x[3 * j + 1] - py, x[3 * j + 2] - pz};
+      const double aw = a[0] * w[0] + a[1] * w[1] + a[2] * w[2];
+      double axw[3];
+      cross3(a, w, axw);
+      for (int i = 0; i < nd; ++i) {
+        double* dxi = dx + (size_t)i * n * 3;
+        const double* dpi = dp + 3 * i;
+        const double* dai = da + 3 * i;
+        const double dw[3] = {dxi[3 * j] - dpi[0], dxi[3 * j + 1] - dpi[1], dxi[3 * j + 2] - dpi[2]};
+        double Rdw[3], daxw[3];
+        rotate_v(R, dw[0], dw[1], dw[2], Rdw);
+        cross3(dai, w, daxw);
+        const double daw = dai[0] * w[0] + dai[1] * w[1] + dai[2] * w[2];
+        const double dt = (i == 6 + k) ? 1.0 : 0.0;
+        for (int q = 0; q < 3; ++q)
+          dxi[3 * j + q] = dpi[q] + Rdw[q] + dt * (-sn * w[q] + cs * axw[q] + sn * a[q] * aw) +
+                           sn * daxw[q] + (1.0 - cs) * (dai[q] * aw + a[q] * daw);
+      }
+      double y[3];
+      rotate_v(R, w[0], w[1], w[2], y);
+      x[3 * j] = y[0] + px; x[3 * j + 1] = y[1] + py; x[3 * j + 2] = y[2] + pz;
+    }
+  }
+  if (nrb > 0) {
+    c[0] = c[1] = c[2] = 0.0;
+    for (int j = 0; j < n; ++j) { c[0] += x[3 * j]; c[1] += x[3 * j + 1]; c[2] += x[3 * j + 2]; }
+    c[0] /= n; c[1] /= n; c[2] /= n;
+  }
+  double rad = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double ex = c[0] - x[3 * j], ey = c[1] - x[3 * j + 1], ez = c[2] - x[3 * j + 2];
+    const double r = 0.01 + sqrt(ex * ex + ey * ey + ez * ez);
+    if (r > rad) rad = r;
+  }
+  /* r_xyz = Rz(g) Ry(b) Rx(a) and its three partial derivatives */
+  double X[9], dX[9], Y[9], dY[9], Z[9], dZ[9], ZY[9], M[9], D[3][9], tmp[9];
+  act_rx(dofs[3], X, dX); act_ry(dofs[4], Y, dY); act_rz(dofs[5], Z, dZ);
+  mat3_mult(Z, Y, ZY); mat3_mult(ZY, X, M);
+  mat3_mult(ZY, dX, D[0]);
+  mat3_mult(Z, dY, tmp); mat3_mult(tmp, X, D[1]);
+  mat3_mult(dZ, Y, tmp); mat3_mult(tmp, X, D[2]);
+  for (int j = 0; j < n; ++j) {
+    double o[3];
+    rotate_v(M, x[3 * j], x[3 * j + 1], x[3 * j + 2], o);
+    oxs[j] = o[0] + dofs[0]; oys[j] = o[1] + dofs[1]; ozs[j] = o[2] + dofs[2];
+    for (int i = 0; i < nd; ++i) {
+      double* ti = tan + ((size_t)i * n + j) * 3;
+      if (i < 3) {
+        ti[0] = ti[1] = ti[2] = 0.0;
+        ti[i] = 1.0;
+      } else if (i < 6) {
+        rotate_v(D[i - 3], x[3 * j], x[3 * j + 1], x[3 * j + 2], ti);
+      } else {
+        const double* dxi = dx + ((size_t)i * n + j) * 3;
+        rotate_v(M, dxi[0], dxi[1], dxi[2], ti);
+      }
+    }
+  }
+  free(x); free(dx); free(dp); free(da);
+  return rad;
+}
+
 /* ------------------------------------------------------------------------ */
 /* analytic gradients.  Pair energy with d = x_lig - x_prot, r = max(|d|, 0.01):
  *   E = w(r) [ke qi qj / r + D (p6^2 - 2 p6)],  p6 = (x/r)^6,

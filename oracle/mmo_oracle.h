@@ -112,6 +112,15 @@ double orc_apply_config(const double* cxs, const double* cys, const double* czs,
                         const double* dofs,
                         double* oxs, double* oys, double* ozs);
 
+/* Forward-mode derivative of orc_apply_config (this build's own, for the CPU checks of
+ * the DOF gradients): also writes tan[(i*n + j)*3 + c] = d out_j[c] / d dofs[i] for the
+ * 6 + nrb DOFs. */
+double orc_apply_config_jvp(const double* cxs, const double* cys, const double* czs, int n,
+                            const int32_t* rb_left, const int32_t* rb_right,
+                            const int32_t* rg_off, const int32_t* rg_atoms, int nrb,
+                            const double* dofs,
+                            double* oxs, double* oys, double* ozs, double* tan);
+
 /* analytic FP64 gradients (dE/dxyz of ligand atoms) -------------------------------
  * Derived from the energy formulas above (not present in the reference, which only
  * has central finite differences, src/lds.ml:1118-1136); checked by central FD of

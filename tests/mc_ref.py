@@ -48,12 +48,49 @@ def _decision_margin(cur, prev_e, best_e, u, beta, scale):
     return m
 
 
+def minimise_proposal(lig, conf2, c2, rot2, pos2, ntors, intra, inter_grad, mz):
+    """The library's Monte Carlo-plus-minimisation step on the CPU: base rows
+    b = rot2 (conf2 - c2) (the proposal centred at the origin), L-BFGS (tests/lbfgs_ref.py)
+    from [pos2, 0, 0, 0, 0..] over the FP64 energy and its exact DOF gradient
+    (oracle.dof_energy_grad).  Returns (E, E_inter, E_intra, conformer of the best point
+    before the rigid motion, M, T, evaluations)."""
+    from tests.lbfgs_ref import LBFGS
+    base = (conf2 - c2) @ rot2.T
+    n = 6 + ntors
+    opt = LBFGS(n, m=mz.get("m", 8), max_step=mz.get("max_step", 0.2), gtol=mz.get("gtol", 1e-6))
+    xt = np.zeros(n)
+    xt[:3] = pos2
+    parts = (0.0, 0.0)
+    for _ in range(mz["iters"]):
+        if opt.init and opt.status != 0:
+            break
+        ei, ea, g, rad = oracle.dof_energy_grad(base, lig, xt, inter_grad, intra=intra)
+        init0, acc0 = opt.init, opt.n_acc
+        xt = opt.update(xt, ei + ea, g, rad)
+        if not init0 or opt.n_acc != acc0:
+            parts = (ei, ea)
+    x = opt.x
+    d0 = np.concatenate([np.zeros(6), x[6:]])
+    r = len(x) - 6
+    rg_off = np.asarray(lig["rg_off"])[:r + 1]
+    tconf, _ = oracle.apply_config(base, np.asarray(lig["rb_left"])[:r], np.asarray(lig["rb_right"])[:r],
+                                   rg_off, np.asarray(lig["rg_atoms"])[:int(rg_off[-1])], d0)
+    M = oracle.rot_xyz(x[3], x[4], x[5])
+    return opt.f, parts[0], parts[1], tconf, M, x[:3].copy(), opt.n_eval, opt.f0
+
+
 def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True, hard_roi=None,
              metropolis_always=False, beta=1.0 / (0.0019872041 * 293.15), inter_fn=None,
-             trace=None, no_flip=False):
+             trace=None, no_flip=False, minimize=None, inter_grad=None):
     """Runs one chain for `nframes`; `trace`, when a list, receives one dict per frame.
     best_xyz is best_lig: the pose of the best proposal, or after a reset the registered
-    conformer at the start pose (src/lds.ml:645-646)."""
+    conformer at the start pose (src/lds.ml:645-646).
+
+    minimize = {"iters": N, "m": .., "max_step": .., "gtol": ..} runs the library's
+    Monte Carlo-plus-minimisation (not the reference's: its MC and its minimiser are
+    separate): every proposal is minimised first and the frame's decisions apply to the
+    minimised pose; inter_grad(x) -> (E_inter, dE/dx) is the energy form's gradient
+    (default: the shifted direct sum against `rec`)."""
     n = len(lig["xs"])
     conf = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1).astype(np.float64)
     center = np.zeros(3)
@@ -74,6 +111,11 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
         if inter_fn is not None:
             return inter_fn(x)
         return oracle.ene_inter(rec, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]), shifted=True)
+
+    if inter_grad is None:
+        def inter_grad(x):
+            return oracle.grad_inter(rec, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]), shifted=True)
+    min_evals = 0
 
     conf0 = conf.copy()
     start = conf @ rot.T + pos
@@ -114,15 +156,24 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
             rot2 = A @ rot
             pos2 = pos + (np.array(u[2:5]) * 2.0 - 1.0) * max_trans
         pose = (conf2 - c2) @ rot2.T + pos2
-        if not rigid and flex and intra:
-            prev_intra = e_intra(conf2)
-        prev_inter = e_inter(pose)
-        cur = prev_inter + prev_intra
+        cpos = pos2
+        e_start = None
+        if minimize is not None:
+            cur, prev_inter, prev_intra, tconf, mM, mT, ne, e_start = minimise_proposal(
+                lig, conf2, c2, rot2, pos2, nrb if flex else 0, intra, inter_grad, minimize)
+            min_evals += ne
+            pose = tconf @ mM.T + mT
+            cpos = mM @ tconf.mean(axis=0) + mT
+        else:
+            if not rigid and flex and intra:
+                prev_intra = e_intra(conf2)
+            prev_inter = e_inter(pose)
+            cur = prev_inter + prev_intra
         metropolis = False
         reset = False
         margin = math.inf
         if hard_roi is not None:
-            if np.linalg.norm(np.asarray(hard_roi[0]) - pos2) > hard_roi[1]:
+            if np.linalg.norm(np.asarray(hard_roi[0]) - cpos) > hard_roi[1]:
                 reset = True
                 ooroi += 1
             else:
@@ -153,8 +204,16 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
                 rot, pos, prev_e = rot2, pos2, cur
                 prev_e_inter = prev_inter
                 conf, center = conf2, c2
+                if minimize is not None:
+                    # the minimised pose in the chain's frame: conformer rot2^T c*, its
+                    # centre, rotation M* rot2, position the pose centre
+                    conf = tconf @ rot2
+                    center = tconf.mean(axis=0) @ rot2
+                    rot, pos = mM @ rot2, cpos
             if cur < best_e:
                 best_e, best_rot, best_pos = cur, rot2, pos2
+                if minimize is not None:
+                    best_rot, best_pos = mM @ rot2, cpos
                 best_inter = prev_inter
                 best_xyz = pose.copy()
             if rigid and rigid_step > 0 and rigid_step % 100 == 0:
@@ -185,9 +244,9 @@ def simulate(lig, rec, rot0, pos0, nframes, seed, chain, tweak=True, intra=True,
                           "conf": conf.copy(), "center": np.array(center), "pose": pose.copy(),
                           "bond": jr, "angle": ang, "max_rot": max_rot, "max_trans": max_trans,
                           "dr": list(dr), "best_xyz": best_xyz.copy(), "reset": reset,
-                          "margin": margin})
+                          "margin": margin, "e_start": e_start})
     return {"best_e": best_e, "e": prev_e, "e_inter": prev_inter, "e_intra": prev_intra,
             "rot": rot, "pos": pos, "conf": conf, "center": np.array(center), "accepted": accepted, "resets": resets,
             "ooroi": ooroi, "status": status, "max_rot": max_rot, "max_trans": max_trans,
             "best_rot": best_rot, "best_pos": best_pos, "best_xyz": best_xyz, "dr": dr,
-            "rigid_step": rigid_step, "conf_step": conf_step}
+            "rigid_step": rigid_step, "conf_step": conf_step, "min_evals": min_evals}

@@ -223,3 +223,37 @@ def test_reset_restores_the_start_pose_of_the_registered_conformer(config0):
         assert np.array_equal(t["best_xyz"], c0 @ np.eye(3).T + START)
         changed |= not np.allclose(t["conf"], c0)
     assert changed, "no reset happened with a turned conformer"
+
+
+# ------------------------------------------------------------- MC plus minimisation
+def test_mc_min_with_one_evaluation_is_plain_mc(config0):
+    """One L-BFGS evaluation per frame scores the proposal itself: the chain takes the
+    plain Monte Carlo's decisions and energies.  (Without E_intra: the plain chain keeps
+    a rejected torsion proposal's E_intra for the next frames, src/lds.ml:900-903, while
+    the minimising chain scores the pose it holds.)"""
+    lig, r0, tr0 = _trace(config0, 40, seed=31, chain=4, intra=False)
+    lig, r1, tr1 = _trace(config0, 40, seed=31, chain=4, intra=False, minimize={"iters": 1})
+    assert r1["accepted"] == r0["accepted"] > 0 and r1["resets"] == r0["resets"]
+    for a, b in zip(tr0, tr1):
+        assert b["e"] == pytest.approx(a["e"], rel=1e-10, abs=1e-9)
+        assert b["e_prop"] == pytest.approx(a["e_prop"], rel=1e-10, abs=1e-9)
+    # the minimising chain stores its conformer centred: equal up to that translation
+    assert np.allclose(r1["conf"] - r1["center"], r0["conf"] - r0["center"], atol=1e-10)
+    assert np.allclose(r1["rot"], r0["rot"], atol=1e-12) and np.allclose(r1["pos"], r0["pos"], atol=1e-12)
+    assert r1["min_evals"] == 40
+
+
+def test_mc_min_frames_lower_the_proposal_and_keep_a_consistent_state(config0):
+    """With 30 evaluations per frame: every minimised energy is at most the proposal's,
+    most frames go lower, and the chain's current energy is the oracle energy of its pose
+    (conformer, centre, rotation, position)."""
+    lig, r, tr = _trace(config0, 12, seed=32, chain=1, minimize={"iters": 30, "max_step": 0.2})
+    assert r["status"] == 0 and r["accepted"] > 0
+    lower = 0
+    for t in tr:
+        assert t["e_prop"] <= t["e_start"]
+        lower += t["e_prop"] < t["e_start"] - 1e-6
+    assert lower >= len(tr) // 2
+    _, rec = _small_system(config0)
+    assert r["e"] == pytest.approx(_energy_of_state(lig, rec, r), rel=1e-9)
+    assert 12 < r["min_evals"] <= 12 * 30

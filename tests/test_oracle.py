@@ -455,3 +455,56 @@ def test_scan_batch_equals_single_calls(config0):
     for k in range(4):
         x = xyz @ rots[pr[k]].T + trans[pt[k]]
         assert e[k] == pytest.approx(oracle.ene_inter(p, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])), rel=1e-13)
+
+
+def test_apply_config_tangents_match_central_differences(config0):
+    """The oracle's forward-mode derivative of Optim.apply_config (this build's exact DOF
+    gradient for the CPU checks) against central differences of apply_config itself:
+    every coordinate tangent of every DOF, at random DOF vectors of the 6-bond ligand."""
+    lig, _ = config0
+    base = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1)
+    rng = np.random.default_rng(12)
+    args = (lig["rb_left"], lig["rb_right"], lig["rg_off"], lig["rg_atoms"])
+    for _ in range(3):
+        d = np.concatenate([rng.uniform(-3, 3, 3), rng.uniform(-np.pi, np.pi, 3 + len(lig["rb_left"]))])
+        x, rad, tan = oracle.apply_config_jvp(base, *args, d)
+        x0, rad0 = oracle.apply_config(base, *args, d)
+        assert np.allclose(x, x0, atol=1e-12) and rad == pytest.approx(rad0, rel=1e-14)
+        h = 1e-6
+        for i in range(len(d)):
+            dp, dm = d.copy(), d.copy()
+            dp[i] += h
+            dm[i] -= h
+            fd = (oracle.apply_config(base, *args, dp)[0] - oracle.apply_config(base, *args, dm)[0]) / (2 * h)
+            assert np.allclose(tan[i], fd, rtol=0, atol=1e-7 * max(1.0, np.abs(fd).max())), i
+
+
+def test_dof_energy_gradient_matches_central_differences(config0):
+    """dE/dDOF of E_inter (shifted) + E_intra through the tangent pass against central
+    differences of the oracle energy (h = 1e-6 A / rad)."""
+    lig, rec = config0
+    sub = {k: np.asarray(rec[k])[:600] for k in ("xs", "ys", "zs", "q", "anum")}
+    base = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1)
+    rng = np.random.default_rng(13)
+    args = (lig["rb_left"], lig["rb_right"], lig["rg_off"], lig["rg_atoms"])
+
+    def inter_grad(x):
+        return oracle.grad_inter(sub, dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2]), shifted=True)
+
+    def energy(d):
+        x, _ = oracle.apply_config(base, *args, d)
+        m = dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+        return oracle.ene_inter(sub, m, shifted=True) + oracle.ene_intra(m)
+
+    for _ in range(3):
+        d = np.concatenate([[9.0, 0.0, 0.0] + rng.uniform(-1, 1, 3),
+                            rng.uniform(-0.5, 0.5, 3 + len(lig["rb_left"]))])
+        ei, ea, g, _ = oracle.dof_energy_grad(base, lig, d, inter_grad)
+        assert ei + ea == pytest.approx(energy(d), rel=1e-12)
+        h = 1e-6
+        fd = np.array([(energy(d + h * e) - energy(d - h * e)) / (2 * h) for e in np.eye(len(d))])
+        assert np.allclose(g, fd, rtol=0, atol=1e-6 * np.abs(fd).max()), np.abs(g - fd).max()
+    # rigid docking: 6 DOFs only, torsions untouched
+    d6 = np.array([9.5, 0.3, -0.2, 0.1, -0.2, 0.3])
+    _, _, g6, _ = oracle.dof_energy_grad(base, lig, d6, inter_grad)
+    assert g6.shape == (6,)
