@@ -9,7 +9,24 @@ namespace mmo {
 // --------------------------------------------------------------- pair arithmetic
 // NEAR = false: the caller knows the pair is at least 0.5 A apart (the residual check is
 // skipped, which keeps the loop branch-free)
-template <int MODE, bool NEAR = true>
+// shift t = 1 - r^2/144 in FP64 from the fixed-point differences and both atoms'
+// quantisation residuals.  The FP32 r^2 carries ~1e-7 relative error, so the FP32 t has
+// an absolute error of ~1e-7 and the weight w = t^2 a relative error of ~2e-7 / t:
+// negligible unless every pair of a sum lies within a few tenths of an A of the cutoff,
+// which is the case for a single probe atom on the far side of a grid.
+__device__ __forceinline__ float shift_t_exact(const RecPos& p, const LigPt& l, int i,
+                                               const PairK& pk) {
+  const RecRes rr = pk.rres[i];
+  const double s = pk.inv_s;
+  const double x = fma((double)(p.x - l.x), s, (double)rr.x - (double)l.rx);
+  const double y = fma((double)(p.y - l.y), s, (double)rr.y - (double)l.ry);
+  const double z = fma((double)(p.z - l.z), s, (double)rr.z - (double)l.rz);
+  const double t = fma(fma(x, x, fma(y, y, z * z)), -1.0 / 144.0, 1.0);
+  return (float)fmin(fmax(t, 0.0), 1.0);
+}
+
+// EXACT_T: the shift t from shift_t_exact (single-probe sums: the grid precompute)
+template <int MODE, bool NEAR = true, bool EXACT_T = false>
 __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, const LigPt& l,
                                             int i, const PairK& pk, float& sq, float& s12,
                                             float& s6) {
@@ -21,7 +38,7 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, co
   const float u3 = u * (u * u);
   if (MODE == MMO_SHIFTED) {
     // w u^3 serves both LJ terms: 9 multiplies and FMAs after u^3 instead of 10
-    const float t = shift_t(r2c, pk);
+    const float t = EXACT_T ? shift_t_exact(p, l, i, pk) : shift_t(r2c, pk);
     const float w = t * t;
     const float v = w * u3;
     sq = fmaf(p.q, w * ir, sq);
@@ -107,7 +124,7 @@ __device__ __forceinline__ int uniformi(int v) { return __builtin_amdgcn_readfir
 // 12 A + half-diagonal).  A row of cells holds its atoms sorted by x, so a row longer
 // than cg.clip_min atoms is clipped to that x range: one read per lane of the row's
 // first (and, past 64 atoms, last) 64 atoms and a ballot give the atoms below and above.
-template <int K, class Fold>
+template <int K, bool EXACT_T = false, class Fold>
 __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __restrict__ s_cell,
                                            const RecPos* __restrict__ s_pos,
                                            const RecAB* __restrict__ s_ab,
@@ -170,7 +187,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
           const RecAB ab = s_ab[i];
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            pair_energy<MMO_SHIFTED, true>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
+            pair_energy<MMO_SHIFTED, true, EXACT_T>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
         }
       } else {
 #pragma unroll 4
@@ -179,7 +196,7 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
           const RecAB ab = s_ab[i];
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            pair_energy<MMO_SHIFTED, false>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
+            pair_energy<MMO_SHIFTED, false, EXACT_T>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
         }
       }
     }

@@ -539,8 +539,8 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
   double Sq = 0.0, S12 = 0.0, S6 = 0.0;
   float qx, qy, qz, hx, hy, hz;
   if (wave_box<1>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) {
-    cull_sweep<1>(cg, s_cell, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, l,
-                  [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
+    cull_sweep<1, true>(cg, s_cell, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, l,
+                        [&](const float (&sq)[1], const float (&s12)[1], const float (&s6)[1]) {
                     Sq += (double)sq[0];
                     S12 += (double)s12[0];
                     S6 += (double)s6[0];

@@ -204,3 +204,64 @@ def test_dof_gradient_intra_routes(system):
         x = x_t.cpu().numpy()
         sums = [intra_abs_sum(lig, x[n * p:n * (p + 1)]) for p in range(2)]
         assert_energy(eg, e0, sums, what="adjoint intra, %d atoms" % n)
+
+
+def _grazing_poses(lig, rec, n, seed):
+    """Poses of `lig` pushed out of the receptor along random directions until their
+    nearest receptor atom lies at a chosen distance in [10.5, 11.99] A (so that only a
+    few pairs, all near the 12 A cutoff, contribute).  Returns [(R, T, x, d_min)]."""
+    from scipy.spatial import cKDTree
+    tree = cKDTree(np.stack([rec["xs"], rec["ys"], rec["zs"]], 1))
+    c = np.stack([lig["xs"], lig["ys"], lig["zs"]], 1)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        R = synth.random_rotations(rng, 1)[0]
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        target = rng.uniform(10.5, 11.99)
+        lo, hi = 0.0, 100.0
+        for _ in range(60):
+            m = 0.5 * (lo + hi)
+            if tree.query(c @ R.T + m * u)[0].min() < target:
+                lo = m
+            else:
+                hi = m
+        x = c @ R.T + hi * u
+        out.append((R, hi * u, x, float(tree.query(x)[0].min())))
+    return out
+
+
+def test_grazing_poses_carry_the_fp32_shift_weight_error(system):
+    """Poses whose every pair inside 12 A lies within ~1.5 A of the cutoff.  The FP32
+    kernels form the shift weight w = t^2, t = 1 - r^2/144, from an FP32 r^2 (~1e-7
+    relative), so t carries an absolute error of ~1.2e-7 and w a relative one of
+    ~2.4e-7 / t, which only matters when the pairs with t -> 0 dominate the sum.  Against
+    the FP64 oracle: within 1e-5 sum|w e| + 3e-7 sum t|e| for every pose (scan kernel
+    and batched pose kernel), and within 1e-5 (the parity bar) for every pose with a
+    pair inside 11.7 A (t >= 0.05).  Such poses score below 0.1 kcal/mol."""
+    from oracle import twin
+    ctx, lig, rec = system
+    lid = ctx.add_ligand(lig)
+    poses = _grazing_poses(lig, rec, 48, seed=12)
+    xyz = np.concatenate([p[2] for p in poses])
+    print("grazing poses: batch and scan", flush=True)
+    eb, _ = ctx.batch([lid] * len(poses)).score(xyz, ff.F_INTER_SHIFTED)
+    pxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    near_bar = 0
+    for k, (R, T, x, dmin) in enumerate(poses):
+        es = float(ctx.scan(lid, ff.SHIFTED, R[None], T[None]).cpu().numpy()[0])
+        m = dict(lig, xs=x[:, 0], ys=x[:, 1], zs=x[:, 2])
+        ref = oracle.ene_inter(rec, m, shifted=True)
+        r2, r, elec, vdw = twin._pair_terms(pxyz, np.asarray(rec["q"]), rec["anum"], x,
+                                            np.asarray(lig["q"]), lig["anum"])
+        t = np.where(r2 < 144.0, 1.0 - r2 / 144.0, 0.0)
+        e_abs = np.abs(twin.KE * elec + vdw)
+        s1, s2 = float(np.sum(t * t * e_abs)), float(np.sum(t * e_abs))
+        assert abs(ref) < 0.1
+        for name, e in (("batch", eb[k, 0]), ("scan", es)):
+            assert abs(e - ref) <= 1e-5 * s1 + 3e-7 * s2, (k, name, e, ref, dmin)
+            if dmin <= 11.7:
+                assert_energy(e, ref, s1, what="pose %d %s (nearest %.2f A)" % (k, name, dmin))
+        near_bar += dmin <= 11.7
+    assert 10 <= near_bar <= 40

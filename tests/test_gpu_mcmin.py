@@ -286,3 +286,43 @@ def test_mc_graph_recaptured_after_the_registry_grows(system):
     sa, sb = a.state(), b.state()
     for key in ("best_e", "e", "accepted", "resets", "frame"):
         assert np.array_equal(sa[key], sb[key]), key
+
+
+def test_receptor_in_lds_or_global_memory_bit_identical(system):
+    """The direct forms read the receptor and its cell table from LDS when they fit and
+    from global memory when they do not (large receptors): both reads give the same
+    numbers, so the minimiser, plain chains and minimising chains are bit-identical."""
+    import torch
+    from mmo_amd import mc as mcmod
+    from mmo_amd.minimize import Minimizer
+    s = system
+    ctx = s["ctx"]
+    rng = np.random.default_rng(9)
+    pose_lig, d0 = _start_dofs(s, rng)
+    b = ctx.batch(pose_lig)
+    cl = np.repeat(np.array(s["ids"][:4]), 2)
+    rot0 = synth.random_rotations(rng, len(cl))
+    pos0 = SURFACE + rng.uniform(-1.0, 1.0, (len(cl), 3))
+    flags = mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI
+    out = {}
+    for lds in (1, 0):
+        ctx.tune(min_rec_lds=lds, mc_rec_lds=lds, min_fused=1)
+        d = torch.tensor(np.concatenate(d0), device="cuda:0")
+        mz = Minimizer(b, flags=ff.F_INTER_SHIFTED | ff.F_INTRA, history=8, max_step=0.2)
+        print("receptor in LDS %d: minimiser" % lds, flush=True)
+        e = mz.run(d, 40).cpu().numpy()
+        plain = mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.SHIFTED, total_frames=60, flags=flags,
+                                 roi_center=SURFACE, roi_r=5.0, seed=31)
+        print("receptor in LDS %d: chains" % lds, flush=True)
+        plain.run(60)
+        mm = _mc_min(s, cl, rot0, pos0, ff.SHIFTED, 6, 20, seed=32)
+        print("receptor in LDS %d: minimising chains" % lds, flush=True)
+        mm.run(6)
+        out[lds] = (e, d.cpu().numpy(), plain.state(), plain.coords()[1], mm.state(), mm.coords()[1])
+    ctx.tune(min_rec_lds=1, mc_rec_lds=1)
+    a, g = out[1], out[0]
+    assert np.array_equal(a[0], g[0]) and np.array_equal(a[1], g[1])
+    for sa, sg, ba, bg in ((a[2], g[2], a[3], g[3]), (a[4], g[4], a[5], g[5])):
+        for key in ("best_e", "e", "accepted", "resets", "frame", "max_rot", "max_trans"):
+            assert np.array_equal(sa[key], sg[key]), key
+        assert all(np.array_equal(x, y) for x, y in zip(ba, bg))
