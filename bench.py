@@ -172,11 +172,12 @@ def measured_traffic(kernel):
     return counter_record(kernel).get("bytes_per_launch")
 
 
-def executed_tflops(kernel, kernel_ms):
-    """FP32 operations the kernel executed per launch (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F32
-    x 64 lanes, FMA = 2) over this run's kernel time."""
-    ops = counter_record(kernel).get("fp32_ops_per_launch")
-    return ops / (kernel_ms * 1e-3) / 1e12 if ops else None
+def executed_tflops(kernel, units, kernel_ms):
+    """FP32 operations the kernel executes per unit of work (pose, MC step, evaluation:
+    SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F32 x 64 lanes, FMA = 2, over the units of the
+    profiled run) x this run's units, over this run's kernel time."""
+    ops = counter_record(kernel).get("fp32_ops_per_unit")
+    return ops * units / (kernel_ms * 1e-3) / 1e12 if ops else None
 
 
 def timed_scan(ctx, lid, mode, rots, trans, out, steps, warmup, kernel):
@@ -225,7 +226,7 @@ def bench_interp(ctx, lig, rots, trans, steps, warmup):
             "roofline": gather_roofline(32.0 * lookups, avg, "k_interp (scan)",
                                         traffic=measured_traffic("k_interp scan"),
                                         algorithmic_TFLOPs=ops / (avg * 1e-3) / 1e12,
-                                        executed=executed_tflops("k_interp scan", avg)),
+                                        executed=executed_tflops("k_interp scan", npose, avg)),
             "note": "poses with an atom outside the grid return NaN and are counted"}
 
 
@@ -337,7 +338,7 @@ def bench_egrad(ctx, ligs, ids, rec, poses_per_lig=4, reps=20, cpu=True):
            "roofline": valu_roofline(flop, kernel_ms, "k_pose_cull<grad> (per-atom sweep)",
                                      flop_per_pose=flop / n,
                                      in_cutoff_pairs_per_pose=float(n_in.mean()),
-                                     executed=executed_tflops("k_pose_cull egrad", kernel_ms),
+                                     executed=executed_tflops("k_pose_cull egrad", n, kernel_ms),
                                      note="the pose kernel of the batched step; the step also "
                                           "runs the DOF chain and the FP64 intra kernels")}
     if cpu:
@@ -409,6 +410,7 @@ def bench_min(ctx, ligs, ids, rec, poses_per_lig=4, iters=200):
             r["roofline"] = valu_roofline(
                 flop_eval * evals, kms, "k_min_persist (one launch)",
                 evaluations=evals, flop_per_evaluation=flop_eval,
+                executed=executed_tflops("k_min_persist min", evals, kms),
                 note="evaluations actually run (converged poses stop early); in-cutoff pairs "
                      "counted at the final poses")
         out[name] = r
@@ -422,7 +424,7 @@ def mc_chain_poses(m):
     return best
 
 
-def mc_rooflines(m, rec, st, kernel_ms, mode, kernel, sample=2048, seed=9):
+def mc_rooflines(m, rec, st, kernel_ms, mode, kernel, sample=2048, seed=9, counter_key=None):
     """Roofline of one MC run: direct sum, 8 FLOP per ligand-receptor pair + 19 per pair
     inside 12 A for each step's E_inter (in-cutoff pairs counted at the best poses of a
     sample of chains); grid form, 32 B of corner reads per ligand atom and step."""
@@ -440,6 +442,8 @@ def mc_rooflines(m, rec, st, kernel_ms, mode, kernel, sample=2048, seed=9):
     per_step = 8.0 * L[pick] * len(rec["xs"]) + 19.0 * n_in
     flop = float(np.sum(frames[pick] * per_step)) * m.nchain / len(pick)
     return valu_roofline(flop, kernel_ms, kernel,
+                         executed=executed_tflops(counter_key, float(frames.sum()), kernel_ms)
+                         if counter_key else None,
                          flop_per_step=float(np.sum(frames[pick] * per_step) / np.sum(frames[pick])),
                          in_cutoff_pairs_per_step=float(n_in.mean()),
                          note="E_inter of every step (in-cutoff pairs at the best poses of %d "
@@ -500,7 +504,8 @@ def bench_mc(ctx, ligs, ids, tids, rec, chains_per_lig, frames, per_lig_sample, 
     st = m.state()
     done = int((st["frame"] == frames).sum())
     steps_b = float(st["frame"].sum())
-    roof_b = mc_rooflines(m, rec, st, kms_b, ff.SHIFTED, "k_mc direct (persistent, receptor in LDS)")
+    roof_b = mc_rooflines(m, rec, st, kms_b, ff.SHIFTED, "k_mc direct (persistent, receptor in LDS)",
+                          counter_key="k_mc direct")
     cpu_r = None
     if cpu:
         lig_of = {l: ligs[k] for k, l in enumerate(ids)}
@@ -524,7 +529,7 @@ def bench_mc(ctx, ligs, ids, tids, rec, chains_per_lig, frames, per_lig_sample, 
     sti = mi.state()
     steps_i = float(sti["frame"].sum())
     out = {"batched": steps_b / tb, "per_ligand": steps_p / tp,
-           "batched_interp": steps_i / ti,
+           "batched_interp": steps_i / ti, "steps": steps_b, "steps_interp": steps_i,
            "interp_grids": "10 x 100^3 FP32 at 0.375 A (%.0f MB)" % (10 * 100 ** 3 * 4 / 1e6),
            "speedup": (steps_b / tb) / (steps_p / tp), "unit": "MC steps/s",
            "chains": m.nchain, "frames": frames, "chains_finished": done,
@@ -570,6 +575,7 @@ def bench_mc_min(ctx, ligs, ids, tids, rec, chains_per_lig=4, frames=10, iters=2
         _, evals = m.min_evals()
         steps = float(st["frame"].sum())
         out[name] = {"steps_per_s": steps / dt, "evaluations_per_s": evals / dt,
+                     "steps": steps, "evaluations": evals,
                      "evaluations_per_step": evals / max(steps, 1.0), "s": dt,
                      "kernel_ms": ctx.last_kernel_ms("mc"),
                      "accept_rate": float(st["accepted"].sum() / max(steps, 1.0)),
@@ -751,7 +757,7 @@ def main():
         "roofline": {"bound": "fp32-valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(kernel + " scan"),
-                     "executed": executed_tflops(kernel + " scan", avg_ms),
+                     "executed": executed_tflops(kernel + " scan", npose, avg_ms),
                      "kernel": kernel + " (scan)", "kernel_ms": avg_ms,
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
         "tuning": {"key": scan_key, "applied": tuned, "cell_size": ctx.tuning("cell_size"),

@@ -25,6 +25,16 @@ HIPCC_FLAGS = [
     "-Wall", "-Wno-unused-result",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
 ]
+# The persistent chain and minimiser kernels run long per-frame control code around
+# their energy sweeps.  Machine-level loop-invariant code motion hoists constant
+# materialisations (the FP64 polynomial coefficients of exp and sincos, for instance) out
+# of the frame loop into VGPRs, which then spill to scratch and are reloaded, one
+# dependent scratch load after another, on every frame; without it these kernels fit
+# their register budgets (k_mc direct form: 220 -> 20 bytes of scratch per lane).
+PER_FILE_FLAGS = {
+    "mc_kernels.hip": ["-mllvm", "-disable-machine-licm"],
+    "min_kernels.hip": ["-mllvm", "-disable-machine-licm"],
+}
 
 
 def hipcc():
@@ -43,7 +53,7 @@ def build(verbose=False, jobs=None):
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + HIPCC_FLAGS + PER_FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

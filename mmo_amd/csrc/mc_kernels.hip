@@ -89,17 +89,25 @@ __device__ __forceinline__ void axis_angle(double x, double y, double z, double 
 }
 
 // --------------------------------------------------------------- energies of a pose
+// The pose being scored (rotation, conformer centre, position) and the frame's Metropolis
+// uniform, in the chain's LDS slot: the energy sweeps read it from there, so none of it
+// is held in registers (and spilled to scratch) while a sweep runs.
+struct McPose {
+  double m[9], c[3], p[3], u;
+};
+static_assert(sizeof(McPose) % 16 == 0, "McPose slots keep 16-byte alignment");
+
 // pose atom j: rot (conf_j - centre) + pos
 struct PoseView {
   const double* sx;
   const double* sy;
   const double* sz;
-  double m[9], cx, cy, cz, px, py, pz;
+  const McPose* q;
   __device__ void at(int j, double& x, double& y, double& z) const {
-    const double a = sx[j] - cx, b = sy[j] - cy, c = sz[j] - cz;
-    x = (m[0] * a + m[1] * b + m[2] * c) + px;
-    y = (m[3] * a + m[4] * b + m[5] * c) + py;
-    z = (m[6] * a + m[7] * b + m[8] * c) + pz;
+    const double a = sx[j] - q->c[0], b = sy[j] - q->c[1], c = sz[j] - q->c[2];
+    x = (q->m[0] * a + q->m[1] * b + q->m[2] * c) + q->p[0];
+    y = (q->m[3] * a + q->m[4] * b + q->m[5] * c) + q->p[1];
+    z = (q->m[6] * a + q->m[7] * b + q->m[8] * c) + q->p[2];
   }
 };
 
@@ -265,7 +273,7 @@ __device__ double mc_minimize(const McArgs& a, const LigInfo& li, int chain, boo
   for (int j = lane; j < L; j += kWave) {   // lane j later reads back the rows it wrote
     double x, y, z;
     v.at(j, x, y, z);
-    gbase[3 * j] = x - v.px; gbase[3 * j + 1] = y - v.py; gbase[3 * j + 2] = z - v.pz;
+    gbase[3 * j] = x - v.q->p[0]; gbase[3 * j + 1] = y - v.q->p[1]; gbase[3 * j + 2] = z - v.q->p[2];
   }
   DofEval ev;
   ev.li = li; ev.latoms = a.latoms; ev.atom_order = a.atom_order; ev.rb_left = a.rb_left;
@@ -280,7 +288,7 @@ __device__ double mc_minimize(const McArgs& a, const LigInfo& li, int chain, boo
   double* Y = a.min_Y + (size_t)chain * a.min_m * kWave + lane;
   double* rho = a.min_rho + (size_t)chain * a.min_m;
   MinPose s{};
-  double xt = lane == 0 ? v.px : lane == 1 ? v.py : lane == 2 ? v.pz : 0.0;
+  double xt = lane == 0 ? v.q->p[0] : lane == 1 ? v.q->p[1] : lane == 2 ? v.q->p[2] : 0.0;
   double x = 0.0, g = 0.0, d = 0.0, e_inter = 0.0, e_intra = 0.0;
   for (int it = 0; it < a.min_iters && __builtin_amdgcn_readfirstlane(s.status) == kMinRunning; ++it) {
     double gt, ei, ea, rad;
@@ -327,15 +335,17 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
   double* cx_ = reinterpret_cast<double*>(mc_lds + rec_bytes) + (size_t)wv * 3 * a.max_atoms;
   double* cy_ = cx_ + a.max_atoms;
   double* cz_ = cy_ + a.max_atoms;
+  const int nwv = blockDim.x / kWave;
   McState* s_st = reinterpret_cast<McState*>(
-      mc_lds + rec_bytes + (((size_t)(blockDim.x / kWave) * 3 * a.max_atoms * sizeof(double) + 15) &
-                            ~(size_t)15));
-  rv.wl = reinterpret_cast<int*>(s_st + blockDim.x / kWave) + wv * kAtomSweepInts;
+      mc_lds + rec_bytes + (((size_t)nwv * 3 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15));
+  McPose* pose = reinterpret_cast<McPose*>(s_st + nwv) + wv;   // this chain's pose slot
+  rv.wl = reinterpret_cast<int*>(reinterpret_cast<McPose*>(s_st + nwv) + nwv) + wv * kAtomSweepInts;
   MinLds ml{};
   if (MIN) {
     // after the scratch of every wavefront: 6 x max_atoms FP64 per wavefront
-    double* b = reinterpret_cast<double*>(reinterpret_cast<int*>(s_st + blockDim.x / kWave) +
-                                          (blockDim.x / kWave) * kAtomSweepInts) +
+    double* b = reinterpret_cast<double*>(
+                    reinterpret_cast<int*>(reinterpret_cast<McPose*>(s_st + nwv) + nwv) +
+                    nwv * kAtomSweepInts) +
                 (size_t)wv * 6 * a.max_atoms;
     ml = MinLds{b, b + a.max_atoms, b + 2 * a.max_atoms, b + 3 * a.max_atoms,
                 b + 4 * a.max_atoms, b + 5 * a.max_atoms};
@@ -387,8 +397,12 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
     if (!st.initialized) {
       // start pose: Mol.rotate_then_translate_copy centered_lig rot0 pos0, no
       // re-centring (src/lds.ml:759, src/mol.ml:669-672); E of the start pose
-      PoseView v{cx_, cy_, cz_, {}, 0.0, 0.0, 0.0, st.pos0[0], st.pos0[1], st.pos0[2]};
-      for (int k = 0; k < 9; ++k) v.m[k] = st.rot0[k];
+      if (lane == 0) {
+        for (int k = 0; k < 9; ++k) pose->m[k] = st.rot0[k];
+        for (int k = 0; k < 3; ++k) { pose->c[k] = 0.0; pose->p[k] = st.pos0[k]; }
+      }
+      wave_sync();
+      const PoseView v{cx_, cy_, cz_, pose};
       st.prev_e_intra = use_intra ? mc_intra(a, li, cx_, cy_, cz_) : 0.0;
       st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
       st.prev_e = st.prev_e_inter + st.prev_e_intra;
@@ -497,10 +511,21 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
         npos[2] = st.pos[2] + (u4 * 2.0 - 1.0) * st.max_trans;
       }
 
+      // the proposal's pose and the frame's Metropolis uniform go to the chain's LDS slot;
+      // from here on they are read from there (the memory clobber below keeps the
+      // compiler from carrying them in registers across the energy sweep)
+      if (lane == 0) {
+        for (int k = 0; k < 9; ++k) pose->m[k] = nrot[k];
+        pose->c[0] = ccx; pose->c[1] = ccy; pose->c[2] = ccz;
+        for (int k = 0; k < 3; ++k) pose->p[k] = npos[k];
+        pose->u = u5;
+      }
+      wave_sync();
+      asm volatile("" ::: "memory");
+
       // ---- energies (stored before the acceptance test, src/lds.ml:900-904)
-      PoseView v{sx, sy, sz, {}, ccx, ccy, ccz, npos[0], npos[1], npos[2]};
-      for (int k = 0; k < 9; ++k) v.m[k] = nrot[k];
-      double curr_e, cpos[3] = {npos[0], npos[1], npos[2]};
+      const PoseView v{sx, sy, sz, pose};
+      double curr_e, cpos[3];
       double mM[9], mT[3];   // MIN: rigid motion of the minimised pose
       if constexpr (MIN) {
         // the proposal, locally minimised; its centre is the pose centre of the ROI test
@@ -510,6 +535,10 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
         st.prev_e_inter = mc_inter<INTER>(a, li, v, rv);
         curr_e = st.prev_e_inter + st.prev_e_intra;
       }
+      asm volatile("" ::: "memory");
+      const McPose& P = *pose;
+      if constexpr (!MIN)
+        for (int k = 0; k < 3; ++k) cpos[k] = P.p[k];
       const double dxr = a.roi_x - cpos[0], dyr = a.roi_y - cpos[1], dzr = a.roi_z - cpos[2];
       const double dist_roi = sqrt(dxr * dxr + dyr * dyr + dzr * dzr);
       const bool hard = (a.flags & kMcHardRoi) != 0;
@@ -548,8 +577,9 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
         // u5 is a multiple of 2^-53, so once positive it exceeds exp(-x) for every x >= 40
         // (exp(-40) < 2^-53): those proposals are rejected without evaluating exp
         const double x = (curr_e - st.prev_e) * a.beta;
+        const double u5p = P.u;
         const bool accept =
-            (curr_e <= st.prev_e) || (!(x >= 40.0 && u5 > 0.0) && u5 < exp(-x));
+            (curr_e <= st.prev_e) || (!(x >= 40.0 && u5p > 0.0) && u5p < exp(-x));
         if (rigid) sw_process(st.sw, accept);
         else if (just_rotated > -1) sw_process(rsw[just_rotated], accept);
         if (accept) {
@@ -561,26 +591,26 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
             double cm[3] = {0.0, 0.0, 0.0};
             for (int j = lane; j < L; j += kWave) {
               const double x0 = ml.tx[j], y0 = ml.ty[j], z0 = ml.tz[j];
-              cx_[j] = nrot[0] * x0 + nrot[3] * y0 + nrot[6] * z0;
-              cy_[j] = nrot[1] * x0 + nrot[4] * y0 + nrot[7] * z0;
-              cz_[j] = nrot[2] * x0 + nrot[5] * y0 + nrot[8] * z0;
+              cx_[j] = P.m[0] * x0 + P.m[3] * y0 + P.m[6] * z0;
+              cy_[j] = P.m[1] * x0 + P.m[4] * y0 + P.m[7] * z0;
+              cz_[j] = P.m[2] * x0 + P.m[5] * y0 + P.m[8] * z0;
               cm[0] += x0; cm[1] += y0; cm[2] += z0;
             }
             for (int k = 0; k < 3; ++k) cm[k] = wave_sum(cm[k]) / L;
             for (int k = 0; k < 3; ++k)
-              st.center[k] = nrot[k] * cm[0] + nrot[3 + k] * cm[1] + nrot[6 + k] * cm[2];
+              st.center[k] = P.m[k] * cm[0] + P.m[3 + k] * cm[1] + P.m[6 + k] * cm[2];
             for (int r = 0; r < 3; ++r)
               for (int cc = 0; cc < 3; ++cc)
-                st.rot[3 * r + cc] = mM[3 * r] * nrot[cc] + mM[3 * r + 1] * nrot[3 + cc] +
-                                     mM[3 * r + 2] * nrot[6 + cc];
+                st.rot[3 * r + cc] = mM[3 * r] * P.m[cc] + mM[3 * r + 1] * P.m[3 + cc] +
+                                     mM[3 * r + 2] * P.m[6 + cc];
             for (int k = 0; k < 3; ++k) st.pos[k] = cpos[k];
             mg0 = mg1 = 0;   // conf holds the minimised conformer: nothing to restore
             wave_sync();
           } else {
-            for (int k = 0; k < 9; ++k) st.rot[k] = nrot[k];
-            for (int k = 0; k < 3; ++k) st.pos[k] = npos[k];
+            for (int k = 0; k < 9; ++k) st.rot[k] = P.m[k];
+            for (int k = 0; k < 3; ++k) st.pos[k] = P.p[k];
             if (mg1 > mg0) {   // conf := conf' (already in place): keep it
-              st.center[0] = ccx; st.center[1] = ccy; st.center[2] = ccz;
+              for (int k = 0; k < 3; ++k) st.center[k] = P.c[k];
               mg0 = mg1 = 0;
             }
           }
@@ -590,8 +620,8 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
           if constexpr (MIN) {
             for (int r = 0; r < 3; ++r)
               for (int cc = 0; cc < 3; ++cc)
-                st.best_rot[3 * r + cc] = mM[3 * r] * nrot[cc] + mM[3 * r + 1] * nrot[3 + cc] +
-                                          mM[3 * r + 2] * nrot[6 + cc];
+                st.best_rot[3 * r + cc] = mM[3 * r] * P.m[cc] + mM[3 * r + 1] * P.m[3 + cc] +
+                                          mM[3 * r + 2] * P.m[6 + cc];
             for (int k = 0; k < 3; ++k) st.best_pos[k] = cpos[k];
             for (int j = lane; j < L; j += kWave) {
               const double x0 = ml.tx[j], y0 = ml.ty[j], z0 = ml.tz[j];
@@ -600,8 +630,8 @@ __global__ void __launch_bounds__(mc_form_waves(INTER, MIN) * 64) k_mc(McArgs a,
               gbest[3 * j + 2] = (mM[6] * x0 + mM[7] * y0 + mM[8] * z0) + mT[2];
             }
           } else {
-            for (int k = 0; k < 9; ++k) st.best_rot[k] = nrot[k];
-            for (int k = 0; k < 3; ++k) st.best_pos[k] = npos[k];
+            for (int k = 0; k < 9; ++k) st.best_rot[k] = P.m[k];
+            for (int k = 0; k < 3; ++k) st.best_pos[k] = P.p[k];
             for (int j = lane; j < L; j += kWave) {
               double x, y, z;
               v.at(j, x, y, z);
@@ -655,7 +685,8 @@ size_t mc_lds_bytes(const McArgs& a, int waves, bool rec_lds) {
            (size_t)(a.cg.nx * a.cg.ny * a.cg.nz + 1) * sizeof(int) + 15) & ~(size_t)15;
   const size_t conf = ((size_t)waves * 3 * a.max_atoms * sizeof(double) + 15) & ~(size_t)15;
   const size_t mins = a.min_iters > 0 ? (size_t)waves * 6 * a.max_atoms * sizeof(double) : 0;
-  return rec + conf + (size_t)waves * (sizeof(McState) + kAtomSweepInts * sizeof(int)) + mins;
+  return rec + conf + (size_t)waves * (sizeof(McState) + sizeof(McPose) + kAtomSweepInts * sizeof(int)) +
+         mins;
 }
 
 template <bool R, int I>
