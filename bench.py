@@ -256,8 +256,23 @@ def bench_select(ctx, lig, rec, lid, rots, trans, out, k=100):
     topk_s = time.perf_counter() - t0
     kept = int(torch.isfinite(out).sum().item())
     best = int(idx[0].item())
+    # filter before scoring (the reference's order): filters, compaction, then only the
+    # kept poses scored; energies and top-k are those of the path above
+    from mmo_amd import ff
+    e2 = torch.empty_like(out)
+    ctx.scan_filtered(lid, ff.SHIFTED, rots, trans, roi=roi, lig_radii=radii, out=e2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e2, nk = ctx.scan_filtered(lid, ff.SHIFTED, rots, trans, roi=roi, lig_radii=radii, out=e2)
+    v2, i2 = ctx.topk(e2, k)
+    torch.cuda.synchronize()
+    fs_s = time.perf_counter() - t0
+    same = bool(torch.equal(i2, idx)) and nk == kept
     return {"poses": out.numel(), "kept_after_filters": kept, "filter_s": filt_s,
             "topk_s": topk_s, "k": k, "best_e": float(vals[0].item()),
+            "filtered_scan_s": fs_s, "filtered_scan_same_topk": same,
+            "filtered_scan_note": "filters first, kept poses compacted and scored, then top-k "
+                                  "(wall clock, rotations and translations from the host)",
             "best_rotation": best // len(trans), "best_translation": best % len(trans),
             "mask": "%d x %d x %d at %.3f A, built in %.2f s on the host" % (dims + (step, mask_s)),
             "filters": "ROI sphere r = 4.75 A, centre clash (AND), atom clash (OR)"}

@@ -223,6 +223,19 @@ enum {
 int mmo_scan_filter_dev(mmo_ctx* ctx, int lig_id, const double* rots, int nrot, const double* trans,
                         int ntrans, const double* roi_center, double roi_r,
                         const double* lig_radii, int flags, double* e_inout, void* stream);
+/* Filter before scoring: the exhaustive scan of mmo_scan_dev restricted to the poses the
+ * clash filters of mmo_scan_filter_dev keep, as Lds.exhaustive_rigid_ligand_docking
+ * scores only the poses it does not skip (src/lds.ml:1082-1092).  The filters run first
+ * (per translation: ROI sphere and centre clash; per pose: atom clashes), the kept poses
+ * are compacted in the scan's pose order and only they are scored (mode MMO_SHIFTED, the
+ * culled kernel, or MMO_INTERP); e_out (device, nrot * ntrans) holds their energies and
+ * +inf for every skipped pose, bit for bit what mmo_scan_dev followed by
+ * mmo_scan_filter_dev gives, so the top-k is the same.  n_kept (host, nullable) receives
+ * the number of scored poses.  Returns when e_out is complete. */
+int mmo_scan_filtered_dev(mmo_ctx* ctx, int lig_id, int mode, const double* rots, int nrot,
+                          const double* trans, int ntrans, const double* roi_center, double roi_r,
+                          const double* lig_radii, int flags, double* e_out, int64_t* n_kept,
+                          void* stream);
 /* The k lowest energies of e (device, n) with their indices, ascending, ties by index
  * (the TopKeeper of src/lds.ml:1055-1064, :1110-1113); NaN sorts last; k <= 2048
  * (k > n gives n).  e_out, idx_out: device.  Enqueued on stream. */

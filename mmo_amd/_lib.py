@@ -53,6 +53,7 @@ SIGNATURES = {
     "mmo_set_vdw_mask": (_i, [_p, _p, _d, _i, _i, _i, _p]),
     "mmo_get_vdw_mask": (_i, [_p, _p]),
     "mmo_scan_filter_dev": (_i, [_p, _i, _p, _i, _p, _i, _p, _d, _p, _i, _p, _p]),
+    "mmo_scan_filtered_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _d, _p, _i, _p, _p, _p]),
     "mmo_topk_dev": (_i, [_p, _p, ct.c_int64, _i, _p, _p, _p]),
     "mmo_topk_dev_noalloc": (_i, [_p, _p, ct.c_int64, _i, _p, _p, _p]),
     "mmo_mc_create": (_i, [_p, _p, _i, _p, _p, _i, _i, _p, _d, _d, ct.c_uint64, _i, _pp]),

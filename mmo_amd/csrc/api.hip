@@ -169,10 +169,15 @@ struct mmo_ctx {
   double mask_step = 0.0, mask_o[3] = {0.0, 0.0, 0.0};
   int mask_n[3] = {0, 0, 0};
   std::vector<uint32_t> mask_host;
+  std::vector<uint8_t> skip_host;   // the last scan filter's per-translation skips
   DevBuf<uint32_t> d_mask;
   DevBuf<uint8_t> s_skip;
   DevBuf<uint64_t> s_tk_st;
   DevBuf<unsigned> s_tk_u;
+  // filtered scan: keep flags and compacted slots of the scan's pose order, the kept
+  // count, the compaction's scratch
+  DevBuf<uint8_t> s_keep, s_select;
+  DevBuf<int> s_list, s_nkept;
   // tuning
   int inter_k = 4, inter_waves = 4, inter_rtile = 2048;
   std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -507,6 +512,58 @@ int plain_setup(mmo_ctx* c, const double* xs, const double* ys, const double* zs
   return MMO_OK;
 }
 
+// the scan kernel's configuration.  scan_cull 1: receptor staged in LDS when it fits;
+// 2: read through the scalar cache; 3: read through the scalar cache, rows cut along x by
+// the pose sweeps' cell table; 4: positions in LDS, LJ factors through the scalar cache;
+// 0: the unculled kernel
+InterLaunch scan_launch(mmo_ctx* c, int mode, int scan_cull) {
+  InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
+  L.cull = 0;
+  if (mode == MMO_SHIFTED && scan_cull) {
+    L.cull = 2;
+    if (scan_cull == 1 && c->cull_ok) L.cull = 1;
+    if (scan_cull == 4 &&
+        (size_t)c->nrec * sizeof(RecPos) + (size_t)(c->cg.nx * c->cg.ny * c->cg.nz + 1) * sizeof(int) <=
+            150 * 1024)
+      L.cull = 3;
+  }
+  if (L.cull && scan_cull == 3) {
+    L.cg = c->cgp;
+    L.cg.clip_min = 1 << 30;
+  }
+  return L;
+}
+
+// translations in Morton order of their per-axis ranks: 64 consecutive slots form a
+// compact brick (a 4x4x4 block of a lattice)
+std::vector<int> morton_translations(const double* trans, int ntrans) {
+  std::vector<int> tperm(ntrans);
+  std::vector<uint32_t> rank[3];
+  for (int a = 0; a < 3; ++a) {
+    std::vector<double> v(ntrans);
+    for (int t = 0; t < ntrans; ++t) v[t] = trans[3 * t + a];
+    std::vector<double> u = v;
+    std::sort(u.begin(), u.end());
+    std::vector<double> uq;
+    for (double x : u)
+      if (uq.empty() || x - uq.back() > 1e-9) uq.push_back(x);
+    rank[a].resize(ntrans);
+    for (int t = 0; t < ntrans; ++t)
+      rank[a][t] = (uint32_t)(std::lower_bound(uq.begin(), uq.end(), v[t] - 1e-9) - uq.begin());
+  }
+  auto spread = [](uint32_t x) {
+    uint64_t r = 0;
+    for (int b = 0; b < 21; ++b) r |= (uint64_t)((x >> b) & 1u) << (3 * b);
+    return r;
+  };
+  std::vector<uint64_t> key(ntrans);
+  for (int t = 0; t < ntrans; ++t)
+    key[t] = spread(rank[0][t]) | (spread(rank[1][t]) << 1) | (spread(rank[2][t]) << 2);
+  for (int t = 0; t < ntrans; ++t) tperm[t] = t;
+  std::stable_sort(tperm.begin(), tperm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  return tperm;
+}
+
 }  // namespace
 
 namespace mmo {
@@ -569,7 +626,9 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     c->s_xyz.release(); c->s_e.release(); c->s_g.release(); c->s_part.release();
     c->s_lig.release(); c->s_latom.release(); c->s_pij.release(); c->s_ppar.release();
     c->s_int.release(); c->s_tile.release(); c->s_scan.release(); c->s_tperm.release();
-    c->d_cell.release();
+    c->d_cell.release(); c->d_cellp.release(); c->d_mask.release(); c->s_skip.release();
+    c->s_tk_st.release(); c->s_tk_u.release(); c->s_keep.release(); c->s_select.release();
+    c->s_list.release(); c->s_nkept.release();
     for (auto& kv : c->ev) {
       (void)hipEventDestroy(kv.second.first);
       (void)hipEventDestroy(kv.second.second);
@@ -1309,34 +1368,7 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     for (int t : c->ligs[lig_id].type)
       if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
   hipStream_t s = (hipStream_t)stream;
-  // translations in Morton order of their per-axis ranks: 64 consecutive slots form a
-  // compact brick (a 4x4x4 block of a lattice)
-  std::vector<int> tperm(ntrans);
-  {
-    std::vector<uint32_t> rank[3];
-    for (int a = 0; a < 3; ++a) {
-      std::vector<double> v(ntrans);
-      for (int t = 0; t < ntrans; ++t) v[t] = trans[3 * t + a];
-      std::vector<double> u = v;
-      std::sort(u.begin(), u.end());
-      std::vector<double> uq;
-      for (double x : u)
-        if (uq.empty() || x - uq.back() > 1e-9) uq.push_back(x);
-      rank[a].resize(ntrans);
-      for (int t = 0; t < ntrans; ++t)
-        rank[a][t] = (uint32_t)(std::lower_bound(uq.begin(), uq.end(), v[t] - 1e-9) - uq.begin());
-    }
-    auto spread = [](uint32_t x) {
-      uint64_t r = 0;
-      for (int b = 0; b < 21; ++b) r |= (uint64_t)((x >> b) & 1u) << (3 * b);
-      return r;
-    };
-    std::vector<uint64_t> key(ntrans);
-    for (int t = 0; t < ntrans; ++t)
-      key[t] = spread(rank[0][t]) | (spread(rank[1][t]) << 1) | (spread(rank[2][t]) << 2);
-    for (int t = 0; t < ntrans; ++t) tperm[t] = t;
-    std::stable_sort(tperm.begin(), tperm.end(), [&](int a, int b) { return key[a] < key[b]; });
-  }
+  const std::vector<int> tperm = morton_translations(trans, ntrans);
   if ((r = upload(c, c->s_tperm, tperm.data(), tperm.size(), s))) return r;
   HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
   HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double),
@@ -1351,23 +1383,7 @@ int mmo_scan_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
     HIPCHK(c, launch_interp_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "interp", s);
   } else {
-    InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
-    // scan_cull 1: receptor staged in LDS when it fits; 2: read through the scalar cache;
-    // 3: read through the scalar cache, rows cut along x by the pose sweeps' cell table;
-    // 4: positions in LDS, LJ factors through the scalar cache
-    L.cull = 0;
-    if (mode == MMO_SHIFTED && c->scan_cull) {
-      L.cull = 2;
-      if (c->scan_cull == 1 && c->cull_ok) L.cull = 1;
-      if (c->scan_cull == 4 &&
-          (size_t)c->nrec * sizeof(RecPos) + (size_t)(c->cg.nx * c->cg.ny * c->cg.nz + 1) * sizeof(int) <=
-              150 * 1024)
-        L.cull = 3;
-    }
-    if (L.cull && c->scan_cull == 3) {
-      L.cg = c->cgp;
-      L.cg.clip_min = 1 << 30;
-    }
+    InterLaunch L = scan_launch(c, mode, c->scan_cull);
     timer_begin(c, "inter", s);
     HIPCHK(c, launch_inter_scan(L, lig_id, d_rot, nrot, d_trans, c->s_tperm.p, ntrans, e_out, s));
     timer_end(c, "inter", s);
@@ -1817,6 +1833,62 @@ bool mask_get(const mmo_ctx* c, long i, long j, long k) {
   const size_t v = (size_t)i + (size_t)j * c->mask_n[0] + (size_t)k * c->mask_n[0] * c->mask_n[1];
   return (c->mask_host[v >> 5] >> (v & 31)) & 1u;
 }
+
+// The scan filters' inputs on the device: the per-translation skips (ROI sphere, centre
+// clash; computed here on the host), the rotations and translations, the conformer and
+// the vdW mask; `a.e` is left to the caller.  Enqueued on s from host arrays that the
+// caller waits for before returning.
+int scan_filter_setup(mmo_ctx* c, int lig_id, const double* rots, int nrot, const double* trans,
+                      int ntrans, const double* roi_center, double roi_r, const double* lig_radii,
+                      int flags, hipStream_t s, ScanFilterArgs& a) {
+  const LigHost& h = c->ligs[lig_id];
+  // Mol.is_ligand_center_vdW_occuppied (src/mol.ml:1208-1218)
+  bool centre_occupied = false;
+  if (flags & MMO_SCAN_CENTER_CLASH) {
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int i = 0; i < h.n; ++i) { cx += h.xyz[3 * i]; cy += h.xyz[3 * i + 1]; cz += h.xyz[3 * i + 2]; }
+    cx /= h.n; cy /= h.n; cz /= h.n;
+    for (int i = 0; i < h.n && !centre_occupied; ++i) {
+      const double dx = cx - h.xyz[3 * i], dy = cy - h.xyz[3 * i + 1], dz = cz - h.xyz[3 * i + 2];
+      centre_occupied = std::sqrt(dx * dx + dy * dy + dz * dz) < lig_radii[i];
+    }
+  }
+  std::vector<uint8_t>& skip = c->skip_host;
+  skip.assign(ntrans, 0);
+  for (int t = 0; t < ntrans; ++t) {
+    const double* p = trans + 3 * (size_t)t;
+    if (flags & MMO_SCAN_ROI) {   // ROI.is_inside: dist2 < out_r^2 (src/ROI.ml:65-66)
+      const double dx = roi_center[0] - p[0], dy = roi_center[1] - p[1], dz = roi_center[2] - p[2];
+      if (!(dx * dx + dy * dy + dz * dz < roi_r * roi_r)) { skip[t] = 1; continue; }
+    }
+    if (centre_occupied) {   // G3D.vdW_clash_AND: all 8 corners in the protein volume
+      const double inv = 1.0 / c->mask_step;
+      const long i0 = (long)((p[0] - c->mask_o[0]) * inv), j0 = (long)((p[1] - c->mask_o[1]) * inv),
+                 k0 = (long)((p[2] - c->mask_o[2]) * inv);
+      bool all = true;
+      for (int q = 0; q < 8 && all; ++q)
+        all = mask_get(c, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2));
+      if (all) skip[t] = 1;
+    }
+  }
+  HIPCHK(c, c->s_skip.ensure(ntrans));
+  HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
+  HIPCHK(c, hipMemcpyAsync(c->s_skip.p, skip.data(), ntrans, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(c->s_scan.p + 9 * (size_t)nrot, trans, 3 * (size_t)ntrans * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+  a = ScanFilterArgs{};
+  a.rots = c->s_scan.p; a.trans = c->s_scan.p + 9 * (size_t)nrot; a.nrot = nrot; a.ntrans = ntrans;
+  size_t base = 0;
+  for (int l = 0; l < lig_id; ++l) base += (size_t)c->ligs[l].n;   // LigInfo.atom_base
+  a.lig = c->d_lig_xyz.p + 3 * base; a.n_atoms = h.n;
+  a.mask = c->has_mask ? c->d_mask.p : nullptr;
+  a.ox = c->mask_o[0]; a.oy = c->mask_o[1]; a.oz = c->mask_o[2];
+  a.inv_step = c->has_mask ? 1.0 / c->mask_step : 0.0;
+  a.nx = c->mask_n[0]; a.ny = c->mask_n[1]; a.nz = c->mask_n[2];
+  a.skip = c->s_skip.p; a.atom_clash = (flags & MMO_SCAN_ATOM_CLASH) ? 1 : 0; a.e = nullptr;
+  return MMO_OK;
+}
 }  // namespace
 
 int mmo_set_vdw_mask(mmo_ctx* c, const double* radii, double step, int nx, int ny, int nz,
@@ -1892,54 +1964,84 @@ int mmo_scan_filter_dev(mmo_ctx* c, int lig_id, const double* rots, int nrot, co
   if ((flags & MMO_SCAN_CENTER_CLASH) && !lig_radii)
     return fail(c, MMO_ERR_BAD_ARG, "the centre-clash filter needs the ligand's radii");
   if ((r = sync_ligands(c))) return r;
-  const LigHost& h = c->ligs[lig_id];
-  // Mol.is_ligand_center_vdW_occuppied (src/mol.ml:1208-1218)
-  bool centre_occupied = false;
-  if (flags & MMO_SCAN_CENTER_CLASH) {
-    double cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int i = 0; i < h.n; ++i) { cx += h.xyz[3 * i]; cy += h.xyz[3 * i + 1]; cz += h.xyz[3 * i + 2]; }
-    cx /= h.n; cy /= h.n; cz /= h.n;
-    for (int i = 0; i < h.n && !centre_occupied; ++i) {
-      const double dx = cx - h.xyz[3 * i], dy = cy - h.xyz[3 * i + 1], dz = cz - h.xyz[3 * i + 2];
-      centre_occupied = std::sqrt(dx * dx + dy * dy + dz * dz) < lig_radii[i];
-    }
-  }
-  std::vector<uint8_t> skip(ntrans, 0);
-  for (int t = 0; t < ntrans; ++t) {
-    const double* p = trans + 3 * (size_t)t;
-    if (flags & MMO_SCAN_ROI) {   // ROI.is_inside: dist2 < out_r^2 (src/ROI.ml:65-66)
-      const double dx = roi_center[0] - p[0], dy = roi_center[1] - p[1], dz = roi_center[2] - p[2];
-      if (!(dx * dx + dy * dy + dz * dz < roi_r * roi_r)) { skip[t] = 1; continue; }
-    }
-    if (centre_occupied) {   // G3D.vdW_clash_AND: all 8 corners in the protein volume
-      const double inv = 1.0 / c->mask_step;
-      const long i0 = (long)((p[0] - c->mask_o[0]) * inv), j0 = (long)((p[1] - c->mask_o[1]) * inv),
-                 k0 = (long)((p[2] - c->mask_o[2]) * inv);
-      bool all = true;
-      for (int q = 0; q < 8 && all; ++q)
-        all = mask_get(c, i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2));
-      if (all) skip[t] = 1;
-    }
-  }
   hipStream_t s = (hipStream_t)stream;
-  HIPCHK(c, c->s_skip.ensure(ntrans));
-  HIPCHK(c, c->s_scan.ensure(9 * (size_t)nrot + 3 * (size_t)ntrans));
-  HIPCHK(c, hipMemcpyAsync(c->s_skip.p, skip.data(), ntrans, hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->s_scan.p, rots, 9 * (size_t)nrot * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(c->s_scan.p + 9 * (size_t)nrot, trans, 3 * (size_t)ntrans * sizeof(double),
-                           hipMemcpyHostToDevice, s));
   ScanFilterArgs a{};
-  a.rots = c->s_scan.p; a.trans = c->s_scan.p + 9 * (size_t)nrot; a.nrot = nrot; a.ntrans = ntrans;
-  size_t base = 0;
-  for (int l = 0; l < lig_id; ++l) base += (size_t)c->ligs[l].n;   // LigInfo.atom_base
-  a.lig = c->d_lig_xyz.p + 3 * base; a.n_atoms = h.n;
-  a.mask = c->has_mask ? c->d_mask.p : nullptr;
-  a.ox = c->mask_o[0]; a.oy = c->mask_o[1]; a.oz = c->mask_o[2];
-  a.inv_step = c->has_mask ? 1.0 / c->mask_step : 0.0;
-  a.nx = c->mask_n[0]; a.ny = c->mask_n[1]; a.nz = c->mask_n[2];
-  a.skip = c->s_skip.p; a.atom_clash = (flags & MMO_SCAN_ATOM_CLASH) ? 1 : 0; a.e = e_inout;
+  if ((r = scan_filter_setup(c, lig_id, rots, nrot, trans, ntrans, roi_center, roi_r, lig_radii,
+                             flags, s, a)))
+    return r;
+  a.e = e_inout;
   HIPCHK(c, launch_scan_filter(a, s));
   // the host arrays are read by the copies above: wait before returning
+  HIPCHK(c, hipStreamSynchronize(s));
+  return MMO_OK;
+}
+
+int mmo_scan_filtered_dev(mmo_ctx* c, int lig_id, int mode, const double* rots, int nrot,
+                          const double* trans, int ntrans, const double* roi_center, double roi_r,
+                          const double* lig_radii, int flags, double* e_out, int64_t* n_kept,
+                          void* stream) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (lig_id < 0 || lig_id >= (int)c->ligs.size())
+    return fail(c, MMO_ERR_BAD_ARG, "unknown ligand %d", lig_id);
+  if (!rots || !trans || nrot <= 0 || ntrans <= 0 || !e_out)
+    return fail(c, MMO_ERR_BAD_ARG, "bad filtered scan arguments");
+  if ((long long)nrot * ntrans > 2147483647LL)
+    return fail(c, MMO_ERR_BAD_ARG, "too many poses (%d x %d)", nrot, ntrans);
+  if (mode != MMO_SHIFTED && mode != MMO_INTERP)
+    return fail(c, MMO_ERR_BAD_ARG, "the filtered scan scores with MMO_SHIFTED or MMO_INTERP");
+  if (mode == MMO_SHIFTED && !c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (mode == MMO_INTERP && !c->has_grid) return fail(c, MMO_ERR_NO_GRID, "no grids set");
+  if ((flags & MMO_SCAN_ROI) && (!roi_center || !(roi_r > 0.0)))
+    return fail(c, MMO_ERR_BAD_ARG, "ROI filter without an ROI");
+  if ((flags & (MMO_SCAN_CENTER_CLASH | MMO_SCAN_ATOM_CLASH)) && !c->has_mask)
+    return fail(c, MMO_ERR_BAD_ARG, "clash filters need mmo_set_vdw_mask first");
+  if ((flags & MMO_SCAN_CENTER_CLASH) && !lig_radii)
+    return fail(c, MMO_ERR_BAD_ARG, "the centre-clash filter needs the ligand's radii");
+  if ((r = sync_ligands(c))) return r;
+  if (mode == MMO_INTERP)
+    for (int t : c->ligs[lig_id].type)
+      if (t >= c->ntypes) return fail(c, MMO_ERR_BAD_ARG, "ligand grid type %d >= %d", t, c->ntypes);
+  hipStream_t s = (hipStream_t)stream;
+  ScanFilterArgs a{};
+  if ((r = scan_filter_setup(c, lig_id, rots, nrot, trans, ntrans, roi_center, roi_r, lig_radii,
+                             flags, s, a)))
+    return r;
+  const std::vector<int> tperm = morton_translations(trans, ntrans);
+  if ((r = upload(c, c->s_tperm, tperm.data(), tperm.size(), s))) return r;
+  const int n = nrot * ntrans;
+  HIPCHK(c, c->s_keep.ensure(n));
+  HIPCHK(c, c->s_list.ensure(n));
+  HIPCHK(c, c->s_nkept.ensure(1));
+  // keep flags in the scan's pose order, then the kept slots compacted in that order (so
+  // that consecutive kept poses stay in the scan's bricks)
+  HIPCHK(c, launch_scan_keep(a, c->s_tperm.p, c->s_keep.p, s));
+  size_t tb = 0;
+  HIPCHK(c, select_flagged_slots(c->s_keep.p, n, c->s_list.p, c->s_nkept.p, nullptr, &tb, s));
+  HIPCHK(c, c->s_select.ensure(tb));
+  HIPCHK(c, select_flagged_slots(c->s_keep.p, n, c->s_list.p, c->s_nkept.p, c->s_select.p, &tb, s));
+  HIPCHK(c, launch_fill(e_out, (size_t)n, __builtin_inf(), s));
+  int nk = 0;
+  HIPCHK(c, hipMemcpyAsync(&nk, c->s_nkept.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (nk > 0) {
+    if (mode == MMO_INTERP) {
+      InterpLaunch L{c->g, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p};
+      timer_begin(c, "interp", s);
+      HIPCHK(c, launch_interp_scan_list(L, lig_id, a.rots, a.trans, c->s_tperm.p, ntrans, c->s_list.p,
+                                        nk, e_out, s));
+      timer_end(c, "interp", s);
+    } else {
+      // the scan's own culled kernel (the list form has no unculled one): the kept poses'
+      // energies are those of mmo_scan_dev, bit for bit
+      InterLaunch L = scan_launch(c, MMO_SHIFTED, c->scan_cull ? c->scan_cull : 2);
+      timer_begin(c, "inter", s);
+      HIPCHK(c, launch_inter_scan_list(L, lig_id, a.rots, a.trans, c->s_tperm.p, ntrans, c->s_list.p,
+                                       nk, e_out, s));
+      timer_end(c, "inter", s);
+    }
+  }
+  if (n_kept) *n_kept = nk;
   HIPCHK(c, hipStreamSynchronize(s));
   return MMO_OK;
 }

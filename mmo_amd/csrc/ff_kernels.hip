@@ -74,6 +74,23 @@ struct ScanSrc {
   __device__ int row(int, int) const { return 0; }
 };
 
+// the kept poses of a filtered scan: entry i of `list` is a slot of the scan's pose order
+// (rotation * ntrans + Morton slot of the translation), so consecutive entries keep the
+// scan's compact bricks; tiles hold 64 consecutive entries
+struct ScanListSrc : ScanSrc {
+  const int* list;
+  int nlist;
+  __device__ Tile tile(int t) const {
+    Tile o;
+    o.lig = lig;
+    o.pose0 = t * kWave;
+    o.npose = min(kWave, nlist - t * kWave);
+    o.split = 1;
+    return o;
+  }
+  __device__ int pose_id(int i) const { return ScanSrc::pose_id(list[i]); }
+};
+
 template <class Src>
 struct InterArgs {
   Src src;
@@ -829,6 +846,20 @@ hipError_t launch_inter_scan(const InterLaunch& L, int lig, const double* rots, 
   return launch_inter_any(L.mode, L.k, false, a, L.waves, s);
 }
 
+hipError_t launch_inter_scan_list(const InterLaunch& L, int lig, const double* rots,
+                                  const double* trans, const int* tperm, int ntrans,
+                                  const int* list, int nlist, double* e_out, hipStream_t s) {
+  if (!L.cull) return hipErrorInvalidValue;   // the list form has only the culled kernel
+  InterArgs<ScanListSrc> a;
+  a.src.rots = rots; a.src.trans = trans; a.src.tperm = tperm; a.src.ntrans = ntrans;
+  a.src.tiles_per_rot = 0; a.src.lig = lig; a.src.list = list; a.src.nlist = nlist;
+  a.ntiles = (nlist + kWave - 1) / kWave; a.fr = L.fr; a.rpos = L.rpos; a.rab = L.rab;
+  a.rres = L.rres; a.nrec = L.nrec; a.rtile = L.rtile; a.ligs = L.ligs; a.latoms = L.latoms;
+  a.lig_xyz = L.lig_xyz; a.e_out = e_out; a.e_stride = 1; a.e_comp = nullptr; a.grad = nullptr;
+  a.write_e = 1; a.rslice = L.nrec; a.npose_total = nlist;
+  return launch_cull(L, a, s);
+}
+
 hipError_t launch_intra(const IntraArgs& a, bool grad, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
   // coordinates, then one gradient copy per wavefront
@@ -876,6 +907,22 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL((k_interp<false, ScanSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_scan_list(const InterpLaunch& L, int lig, const double* rots,
+                                   const double* trans, const int* tperm, int ntrans,
+                                   const int* list, int nlist, double* e_out, hipStream_t s) {
+  InterpArgs<ScanListSrc> a;
+  a.src.rots = rots; a.src.trans = trans; a.src.tperm = tperm; a.src.ntrans = ntrans;
+  a.src.tiles_per_rot = 0; a.src.lig = lig; a.src.list = list; a.src.nlist = nlist;
+  a.ntiles = (nlist + kWave - 1) / kWave; a.g = L.g; a.ligs = L.ligs; a.latoms = L.latoms;
+  a.lig_xyz = L.lig_xyz; a.type_override = nullptr; a.e_out = e_out; a.e_stride = 1;
+  a.grad = nullptr; a.grad_accumulate = 0;
+  const int waves = 4;
+  const int blocks = (a.ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_interp<false, ScanListSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
   return hipGetLastError();
 }
 

@@ -116,6 +116,13 @@ hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const 
 hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots, int nrot,
                               const double* trans, const int* tperm, int ntrans, double* e_out,
                               hipStream_t s);
+// filtered scan: score only the listed slots of the scan's pose order (the culled kernel)
+hipError_t launch_inter_scan_list(const InterLaunch& L, int lig, const double* rots,
+                                  const double* trans, const int* tperm, int ntrans,
+                                  const int* list, int nlist, double* e_out, hipStream_t s);
+hipError_t launch_interp_scan_list(const InterpLaunch& L, int lig, const double* rots,
+                                   const double* trans, const int* tperm, int ntrans,
+                                   const int* list, int nlist, double* e_out, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_fill_int(int* p, size_t n, int v, hipStream_t s);
 hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s);
@@ -168,6 +175,13 @@ struct ScanFilterArgs {
   double* e;                  // scan energies, set to +inf for skipped poses
 };
 hipError_t launch_scan_filter(const ScanFilterArgs& a, hipStream_t s);
+// keep[slot] = 1 for the poses the filters keep, over the scan's pose order (rotation r,
+// translation tperm[slot - r * ntrans])
+hipError_t launch_scan_keep(const ScanFilterArgs& a, const int* tperm, uint8_t* keep, hipStream_t s);
+// out[0 .. *n_out) = the slots whose flag is set, in order (n_out: device); temp: device
+// scratch of *temp_bytes (with temp == nullptr only *temp_bytes is set)
+hipError_t select_flagged_slots(const uint8_t* flags, int n, int* out, int* n_out, void* temp,
+                                size_t* temp_bytes, hipStream_t s);
 
 constexpr int kTopkMax = 2048;
 struct TopkScratch {

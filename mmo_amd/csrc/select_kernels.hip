@@ -14,24 +14,17 @@
 // one workgroup sorts the k (energy, index) pairs.  NaN sorts as the largest value.
 #include <cstdint>
 
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 #include "kernels.h"
 
 namespace mmo {
 
 // ------------------------------------------------------------------ clash filter
-__global__ void __launch_bounds__(256) k_scan_filter(ScanFilterArgs a) {
-  extern __shared__ double s_lig[];
-  for (int i = threadIdx.x; i < 3 * a.n_atoms; i += blockDim.x) s_lig[i] = a.lig[i];
-  __syncthreads();
-  const size_t pose = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pose >= (size_t)a.nrot * a.ntrans) return;
-  const int r = (int)(pose / a.ntrans), t = (int)(pose - (size_t)r * a.ntrans);
-  if (a.skip[t]) {
-    a.e[pose] = __builtin_inf();
-    return;
-  }
-  if (!a.atom_clash) return;
+// whether an atom of pose (rotation r, translation t) has a protein-volume voxel among its
+// 8 corners (Mol.protein_ligand_clash); s_lig: the centred conformer rows
+__device__ bool pose_atom_clash(const ScanFilterArgs& a, const double* s_lig, int r, int t) {
   const double* R = a.rots + 9 * (size_t)r;
   const double* T = a.trans + 3 * (size_t)t;
   const size_t nxy = (size_t)a.nx * a.ny;
@@ -55,11 +48,46 @@ __global__ void __launch_bounds__(256) k_scan_filter(ScanFilterArgs a) {
                        (size_t)(k0 + (cidx >> 2)) * nxy;
       hit |= ((a.mask[v >> 5] >> (v & 31)) & 1u) != 0u;
     }
-    if (hit) {
-      a.e[pose] = __builtin_inf();
-      return;
-    }
+    if (hit) return true;
   }
+  return false;
+}
+
+__global__ void __launch_bounds__(256) k_scan_filter(ScanFilterArgs a) {
+  extern __shared__ double s_lig[];
+  for (int i = threadIdx.x; i < 3 * a.n_atoms; i += blockDim.x) s_lig[i] = a.lig[i];
+  __syncthreads();
+  const size_t pose = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pose >= (size_t)a.nrot * a.ntrans) return;
+  const int r = (int)(pose / a.ntrans), t = (int)(pose - (size_t)r * a.ntrans);
+  if (a.skip[t] || (a.atom_clash && pose_atom_clash(a, s_lig, r, t))) a.e[pose] = __builtin_inf();
+}
+
+// filter before scoring: the same tests, as keep flags over the scan's pose order
+__global__ void __launch_bounds__(256) k_scan_keep(ScanFilterArgs a, const int* tperm, uint8_t* keep) {
+  extern __shared__ double s_lig[];
+  for (int i = threadIdx.x; i < 3 * a.n_atoms; i += blockDim.x) s_lig[i] = a.lig[i];
+  __syncthreads();
+  const size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= (size_t)a.nrot * a.ntrans) return;
+  const int r = (int)(slot / a.ntrans);
+  const int t = tperm[slot - (size_t)r * a.ntrans];
+  keep[slot] = (a.skip[t] || (a.atom_clash && pose_atom_clash(a, s_lig, r, t))) ? 0 : 1;
+}
+
+hipError_t launch_scan_keep(const ScanFilterArgs& a, const int* tperm, uint8_t* keep, hipStream_t s) {
+  const size_t n = (size_t)a.nrot * a.ntrans;
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(k_scan_keep, dim3((unsigned)blocks), dim3(256),
+                     (size_t)3 * a.n_atoms * sizeof(double), s, a, tperm, keep);
+  return hipGetLastError();
+}
+
+hipError_t select_flagged_slots(const uint8_t* flags, int n, int* out, int* n_out, void* temp,
+                                size_t* temp_bytes, hipStream_t s) {
+  return hipcub::DeviceSelect::Flagged(temp, *temp_bytes, hipcub::CountingInputIterator<int>(0),
+                                       flags, out, n_out, n, s);
 }
 
 hipError_t launch_scan_filter(const ScanFilterArgs& a, hipStream_t s) {

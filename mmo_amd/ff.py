@@ -270,6 +270,29 @@ class Context:
                                             _stream_ptr(stream)))
         return e
 
+    def scan_filtered(self, lig_id, mode, rots, trans, roi=None, lig_radii=None, flags=None,
+                      out=None, stream=None):
+        """Filter before scoring (Lds.exhaustive_rigid_ligand_docking, src/lds.ml:1040-1114):
+        the clash filters of scan_filter first, then only the kept poses scored; returns
+        (energies, number of poses scored) with +inf for every skipped pose, the same as
+        scan() followed by scan_filter()."""
+        import torch
+        r = _f64(np.asarray(rots).reshape(-1, 9))
+        t = _f64(np.asarray(trans).reshape(-1, 3))
+        if flags is None:
+            flags = (_lib.SCAN_ROI if roi is not None else 0) | _lib.SCAN_ATOM_CLASH | \
+                (_lib.SCAN_CENTER_CLASH if lig_radii is not None else 0)
+        if out is None:
+            out = torch.empty(len(r) * len(t), dtype=torch.float64, device="cuda:%d" % self.device)
+        c = _f64(roi[0]) if roi is not None else None
+        rad = _f64(lig_radii) if lig_radii is not None else None
+        nk = ct.c_int64()
+        self._check(lib.mmo_scan_filtered_dev(self.h, int(lig_id), int(mode), _ptr(r), len(r), _ptr(t),
+                                              len(t), _ptr(c), float(roi[1]) if roi is not None else 0.0,
+                                              _ptr(rad), int(flags), ct.c_void_p(out.data_ptr()),
+                                              ct.byref(nk), _stream_ptr(stream)))
+        return out, nk.value
+
     def topk(self, e, k, stream=None):
         """The k lowest values of the CUDA FP64 tensor e and their indices (ascending)."""
         import torch

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from mmo_amd import ff, synth
+from mmo_amd import _lib, ff, synth
 from mmo_amd._lib import MMOError
 from tests.parity import assert_energy, assert_grad, inter_abs_sum, intra_abs_sum
 
@@ -1218,6 +1218,53 @@ def test_vdw_mask_and_scan_filters(sys0):
             want = scan_ref.pose_skipped(ref, step, origin, c, radii, rots[r], trans[t], roi=roi)
             assert got[s] == want, (len(L["xs"]), r, t)
         assert 0 < got.sum() < len(got)
+    torch.cuda.synchronize()
+
+
+def test_filtered_scan_equals_scan_then_filter(sys0):
+    """Filter before scoring, as the reference's exhaustive docking does: the filters
+    first, then only the kept poses scored.  At the headline scan's full size (4,608
+    rotations x 20^3 translations at 0.5 A), against the full scan followed by the
+    filters, in the direct and the interpolated form, with an ROI that keeps a few poses,
+    one that keeps a share of them, and the ROI alone around the whole lattice (every pose
+    kept, so every slot goes through the list form): every energy bit for bit (+inf for
+    skipped poses) and the same top-100."""
+    import torch
+    ctx, lig, rec = sys0
+    rxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
+    step = 0.375
+    origin = rxyz.min(axis=0) - 2.0
+    dims = tuple(int(v) for v in np.ceil((rxyz.max(axis=0) + 2.0 - origin) / step) + 1)
+    ctx.set_vdw_mask(step, dims, origin)
+    probe_a, probe_q, types = synth.grid_types([lig], 10)
+    gorigin, gdims, gstep = synth.grid_spec((0.0, 0.0, 0.0), 64, 0.375)
+    ctx.compute_grids(probe_a, probe_q, gstep, gdims, gorigin, copy_out=False)
+    lid = ctx.add_ligand(lig, types=types[0])
+    rots = synth.so3_rotations(4608)
+    trans = synth.lattice((0.0, 0.0, 0.0), 20, 0.5)
+    radii = _bondi(lig["anum"])
+    cases = [("roi 4.75 + clashes", (np.zeros(3), 4.75 + 1e-9), radii, None),
+             ("roi 6 + clashes", (np.array([0.5, -0.5, 0.25]), 6.0), radii, None),
+             ("roi only, all kept", (np.zeros(3), 20.0), None, _lib.SCAN_ROI)]
+    for mode in (ff.SHIFTED, ff.INTERP):
+        full = ctx.scan(lid, mode, rots, trans)
+        for name, roi, rad, flags in cases:
+            print("filtered scan", mode, name, flush=True)
+            e_ref = full.clone()
+            ctx.scan_filter(lid, rots, trans, e_ref, roi=roi, lig_radii=rad, flags=flags)
+            e, nk = ctx.scan_filtered(lid, mode, rots, trans, roi=roi, lig_radii=rad, flags=flags)
+            a, b = e.cpu().numpy(), e_ref.cpu().numpy()
+            assert nk == int((~np.isinf(b)).sum()), (name, nk)
+            assert np.array_equal(a, b, equal_nan=True), (mode, name)
+            if flags == _lib.SCAN_ROI:
+                assert nk == len(a)
+            else:
+                assert 0 < nk < len(a)
+            v1, i1 = ctx.topk(e, 100)
+            v2, i2 = ctx.topk(e_ref, 100)
+            assert torch.equal(i1, i2) and np.array_equal(v1.cpu().numpy(), v2.cpu().numpy(),
+                                                          equal_nan=True)
+        del full
     torch.cuda.synchronize()
 
 
