@@ -12,10 +12,16 @@ Reference layout (src/G3D.ml:14-64, src/lds.ml:471-555, src/mol.ml:280-304):
   "%d (%d, %g)" lines (src/lds.ml:1600-1606); --less-charges first rounds charges to
   0.01 (Utls.reduce_precision, src/utls.ml:129-132).
 
+- unless run with --no-compress, the reference keeps each grid file compressed with the
+  zstd tool as "<file>.ba1.zst" (the raw file removed, its .dims file kept) and reads
+  that form when present (src/lds.ml:471-555, src/utls.ml:22-45); `write_ba1(...,
+  compress=True)` writes the same zstd frame and `read_ba1` / `load_grids` read either.
+
 Grids written here load in the reference and the other way round; `load_grids` returns
 the (ntypes, z, y, x) FP32 array that `Context.set_grids` takes.
 """
 import math
+import os
 
 import numpy as np
 
@@ -83,38 +89,57 @@ def read_dims(fn):
     return vals["step"], (vals["x_dim"], vals["y_dim"], vals["z_dim"])
 
 
-def write_ba1(fn, grid, step):
-    """grid: (z, y, x) array; written as FP32 with its .dims file."""
+def write_ba1(fn, grid, step, compress=False):
+    """grid: (z, y, x) array; written as FP32 with its .dims file.  compress: replace the
+    raw file by "<fn>.zst" (as `zstd --rm` does); returns the data file's name."""
     g = np.ascontiguousarray(grid, dtype=np.float32)
     nz, ny, nx = g.shape
-    g.tofile(fn)
     write_dims(fn + ".dims", step, (nx, ny, nz))
+    if not compress:
+        g.tofile(fn)
+        return fn
+    from . import zstd
+    with open(fn + ".zst", "wb") as f:
+        f.write(zstd.compress(g.tobytes()))
+    if os.path.exists(fn):
+        os.remove(fn)
+    return fn + ".zst"
 
 
 def read_ba1(fn, dims=None):
-    """Returns (grid (z, y, x) FP32, step); dims from the .dims file unless given."""
+    """Returns (grid (z, y, x) FP32, step) of "<x>.ba1" or "<x>.ba1.zst"; dims from the
+    raw file's .dims file unless given."""
+    raw = fn[:-4] if fn.endswith(".zst") else fn
     step = None
     if dims is None:
-        step, dims = read_dims(fn + ".dims")
+        step, dims = read_dims(raw + ".dims")
     nx, ny, nz = dims
-    a = np.fromfile(fn, dtype=np.float32)
+    if fn.endswith(".zst"):
+        from . import zstd
+        with open(fn, "rb") as f:
+            a = np.frombuffer(zstd.decompress(f.read(), size=4 * nx * ny * nz), dtype=np.float32)
+    else:
+        a = np.fromfile(fn, dtype=np.float32)
     if a.size != nx * ny * nz:
         raise ValueError("%s holds %d floats, the dims give %d" % (fn, a.size, nx * ny * nz))
     return a.reshape(nz, ny, nx), step
 
 
-def save_grids(lig_fn, grids, step):
-    """grids: (ntypes, z, y, x); writes <lig_fn>.t<t>.ba1 (+ .dims) for every type."""
+def save_grids(lig_fn, grids, step, compress=False):
+    """grids: (ntypes, z, y, x); writes <lig_fn>.t<t>.ba1 (+ .dims) for every type, or
+    <lig_fn>.t<t>.ba1.zst with compress."""
     for t in range(len(grids)):
-        write_ba1(grid_file(lig_fn, t), grids[t], step)
+        write_ba1(grid_file(lig_fn, t), grids[t], step, compress=compress)
 
 
 def load_grids(lig_fn, ntypes):
-    """Reads the ntypes grids of <lig_fn>; all must share step and dimensions."""
+    """Reads the ntypes grids of <lig_fn> (each from its .ba1 file, or from .ba1.zst when
+    only that exists); all must share step and dimensions."""
     out, step0, dims0 = [], None, None
     for t in range(ntypes):
-        fn = grid_file(lig_fn, t)
-        step, dims = read_dims(fn + ".dims")
+        raw = grid_file(lig_fn, t)
+        fn = raw if os.path.exists(raw) or not os.path.exists(raw + ".zst") else raw + ".zst"
+        step, dims = read_dims(raw + ".dims")
         if step0 is None:
             step0, dims0 = step, dims
         elif step != step0 or dims != dims0:

@@ -49,6 +49,58 @@ def test_ba1_dims_round_trip(tmp_path):
     assert np.array_equal(back, grids)
 
 
+def test_compressed_grid_files(tmp_path):
+    """The reference's compressed cache (<file>.ba1.zst, raw file removed, .dims kept):
+    a standard zstd frame whose content is the raw .ba1 bytes; load_grids reads it."""
+    from mmo_amd import zstd
+    rng = np.random.default_rng(1)
+    grids = rng.normal(0, 100, (2, 4, 5, 6)).astype(np.float32)
+    grids[:, :2] = 0.0                                   # the sparse parts compress
+    base = str(tmp_path / "ligs.mol2")
+    gridio.save_grids(base, grids, 0.5, compress=True)
+    import os
+    assert not os.path.exists(base + ".t0.ba1") and os.path.exists(base + ".t0.ba1.zst")
+    assert open(base + ".t0.ba1.dims").read() == "step: 0.5\nx_dim: 6\ny_dim: 5\nz_dim: 4\n"
+    blob = open(base + ".t1.ba1.zst", "rb").read()
+    assert blob[:4] == b"\x28\xb5\x2f\xfd"               # zstd frame magic 0xFD2FB528
+    assert zstd.decompress(blob) == grids[1].tobytes()   # the raw file's bytes
+    back, step = gridio.load_grids(base, 2)
+    assert step == 0.5 and np.array_equal(back, grids)
+    # a raw file next to the compressed one is preferred (the reference's --no-compress)
+    grids[0, 3, 4, 5] = 7.0
+    gridio.write_ba1(base + ".t0.ba1", grids[0], 0.5)
+    back, _ = gridio.load_grids(base, 2)
+    assert back[0, 3, 4, 5] == 7.0
+    with pytest.raises(ValueError):
+        zstd.decompress(blob, size=len(grids[1].tobytes()) + 4)
+    with pytest.raises(ValueError):
+        zstd.decompress(b"not a frame")
+
+
+def test_roi_bild_file(tmp_path):
+    """ROI.from_bild on the reference's data/ROI.bild (a copy in tests/golden), the
+    writer's three lines, and the one-sphere rule."""
+    import os
+    from mmo_amd import roi
+    here = os.path.dirname(os.path.abspath(__file__))
+    c, r = roi.from_bild(os.path.join(here, "golden", "ROI.bild"))
+    assert np.array_equal(c, [11.47, -3.64, -32.13]) and r == 10.0
+    fn = str(tmp_path / "roi.bild")
+    roi.to_bild(fn, [1.5, -2.25, 1e-3], 6.5)
+    assert open(fn).read() == ".transparency 0.8\n.color blue\n.sphere 1.5 -2.25 0.001 6.5\n"
+    c2, r2 = roi.from_bild(fn)
+    assert np.array_equal(c2, [1.5, -2.25, 1e-3]) and r2 == 6.5
+    assert roi.is_inside(c2, r2, c2 + [6.4, 0, 0]) and not roi.is_inside(c2, r2, c2 + [6.5, 0, 0])
+    with open(fn, "a") as f:
+        f.write(".sphere 0 0 0 1\n")
+    with pytest.raises(ValueError):
+        roi.from_bild(fn)
+    with open(fn, "w") as f:
+        f.write(".color blue\n")
+    with pytest.raises(ValueError):
+        roi.from_bild(fn)
+
+
 def test_ba1_size_mismatch_is_an_error(tmp_path):
     fn = str(tmp_path / "x.t0.ba1")
     np.zeros(10, dtype=np.float32).tofile(fn)
