@@ -251,6 +251,34 @@ int mmo_topk_dev(mmo_ctx* ctx, const double* e, int64_t n, int k, double* e_out,
 int mmo_topk_dev_noalloc(mmo_ctx* ctx, const double* e, int64_t n, int k, double* e_out,
                          int64_t* idx_out, void* stream);
 
+/* ------------------------------------------------------------- desolvation */
+/* Protein desolvation (Lds.first_solvent_shell and Lds.protein_desolv, src/lds.ml:171-234;
+ * formula (2) of Majeux, Scarsi and Caflisch, Proteins 2001): on the grid origin + (i, j,
+ * k) * step (origin NULL = (0, 0, 0), the reference's grid corner), the receptor's first
+ * solvent shell (nodes within r_vdW + 1.4 A of an atom and not within r_vdW of any;
+ * radii nullable: Ptable.vdW_radii by element), and per shell node strictly inside the
+ * ROI sphere (roi_center NULL: every shell node) the contribution
+ * Const.desolvation (src/const.ml:31) x step^3 x sum over receptor atoms closer than
+ * 12 A of (q / d^2)^2; 0 at every other node.  Needs the receptor; replaced by the
+ * next receptor.  Returns the number of nodes given a contribution (>= 0) or an error. */
+int mmo_desolv_setup(mmo_ctx* ctx, const double* radii, double step, int nx, int ny, int nz,
+                     const double* origin, const double* roi_center, double roi_r);
+/* The receptor's first solvent shell of the last mmo_desolv_setup as 32-bit words (bit v
+ * of word v/32 = node v, v = i + j*nx + k*nx*ny) into words (nullable); returns the
+ * number of words. */
+int mmo_get_solvent_shell(const mmo_ctx* ctx, uint32_t* words);
+/* The per-node protein contributions of the last mmo_desolv_setup into out (host,
+ * nullable, nx*ny*nz doubles); returns the number of nodes. */
+int mmo_get_desolv_contribs(mmo_ctx* ctx, double* out);
+/* Lds.desolvation_penalty (src/lds.ml:236-267) of one ligand pose: over the nodes in both
+ * the receptor's and the ligand's first solvent shells, *prot = the sum of the protein
+ * contributions and *lig = Const.desolvation x step^3 x the sum of the ligand atoms'
+ * (q / d^2)^2 closer than 12 A.  xyz: host (n x 3 rows, the receptor's frame); q, radii:
+ * host (n).  Sums in FP64 (the node order of the sums differs from the reference's, so
+ * results agree to rounding).  Needs mmo_desolv_setup. */
+int mmo_desolv_penalty(mmo_ctx* ctx, const double* xyz, const double* q, const double* radii,
+                       int n, double* prot, double* lig);
+
 /* -------------------------------------------------------------- Monte Carlo */
 /* Batched MC chains (Lds.simulate_lig, src/lds.ml:741-1000) on the GPU: chain k
  * docks ligand chain_lig[k] from the start pose (rot0[9k..], pos0[3k..]) with rigid

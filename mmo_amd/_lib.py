@@ -52,6 +52,10 @@ SIGNATURES = {
     "mmo_scan_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _p]),
     "mmo_set_vdw_mask": (_i, [_p, _p, _d, _i, _i, _i, _p]),
     "mmo_get_vdw_mask": (_i, [_p, _p]),
+    "mmo_desolv_setup": (_i, [_p, _p, _d, _i, _i, _i, _p, _p, _d]),
+    "mmo_get_solvent_shell": (_i, [_p, _p]),
+    "mmo_get_desolv_contribs": (_i, [_p, _p]),
+    "mmo_desolv_penalty": (_i, [_p, _p, _p, _p, _i, _p, _p]),
     "mmo_scan_filter_dev": (_i, [_p, _i, _p, _i, _p, _i, _p, _d, _p, _i, _p, _p]),
     "mmo_scan_filtered_dev": (_i, [_p, _i, _i, _p, _i, _p, _i, _p, _d, _p, _i, _p, _p, _p]),
     "mmo_topk_dev": (_i, [_p, _p, ct.c_int64, _i, _p, _p, _p]),

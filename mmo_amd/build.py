@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 SOURCES = ["api.hip", "ff_kernels.hip", "pose_kernels.hip", "mc_kernels.hip", "min_kernels.hip",
-           "select_kernels.hip", "molio.cpp"]
+           "select_kernels.hip", "desolv_kernels.hip", "molio.cpp"]
 LIB = os.path.join(HERE, "libmmo_ff.so")
 ARCH = os.environ.get("MMO_OFFLOAD_ARCH", "gfx950")
 

@@ -164,12 +164,23 @@ struct mmo_ctx {
   // receptor on the host (rows x, y, z; atomic numbers) for host-side set-ups
   std::vector<double> rec_host;
   std::vector<int> rec_anum;
+  std::vector<double> rec_q;
   // protein vdW volume bitmask (Lds.vdW_volume) for the rigid scan's clash filters
   bool has_mask = false;
   double mask_step = 0.0, mask_o[3] = {0.0, 0.0, 0.0};
   int mask_n[3] = {0, 0, 0};
   std::vector<uint32_t> mask_host;
   std::vector<uint8_t> skip_host;   // the last scan filter's per-translation skips
+  // desolvation: the receptor's first solvent shell on a grid, its nodes' contributions
+  bool has_desolv = false;
+  double desolv_step = 0.0, desolv_o[3] = {0.0, 0.0, 0.0};
+  int desolv_n[3] = {0, 0, 0};
+  std::vector<uint32_t> desolv_shell;
+  DevBuf<double> d_desolv;      // per grid node
+  DevBuf<double4> d_rec64;      // receptor x, y, z, q (FP64)
+  DevBuf<int> s_dvox;
+  DevBuf<double4> s_datoms;
+  DevBuf<double> s_dpart;
   DevBuf<uint32_t> d_mask;
   DevBuf<uint8_t> s_skip;
   DevBuf<uint64_t> s_tk_st;
@@ -628,7 +639,8 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     c->s_int.release(); c->s_tile.release(); c->s_scan.release(); c->s_tperm.release();
     c->d_cell.release(); c->d_cellp.release(); c->d_mask.release(); c->s_skip.release();
     c->s_tk_st.release(); c->s_tk_u.release(); c->s_keep.release(); c->s_select.release();
-    c->s_list.release(); c->s_nkept.release();
+    c->s_list.release(); c->s_nkept.release(); c->d_desolv.release(); c->d_rec64.release();
+    c->s_dvox.release(); c->s_datoms.release(); c->s_dpart.release();
     for (auto& kv : c->ev) {
       (void)hipEventDestroy(kv.second.first);
       (void)hipEventDestroy(kv.second.second);
@@ -781,6 +793,8 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   c->has_rec = true;
   c->rec_host.resize(3 * (size_t)n);
   c->rec_anum.assign(anum, anum + n);
+  c->rec_q.assign(q, q + n);
+  c->has_desolv = false;   // desolvation data belong to the receptor they came from
   for (int i = 0; i < n; ++i) {
     c->rec_host[3 * i] = xs[i]; c->rec_host[3 * i + 1] = ys[i]; c->rec_host[3 * i + 2] = zs[i];
   }
@@ -1834,6 +1848,49 @@ bool mask_get(const mmo_ctx* c, long i, long j, long k) {
   return (c->mask_host[v >> 5] >> (v & 31)) & 1u;
 }
 
+// Lds.atom_bitmask_set (src/lds.ml:145-169) on the grid o + (i, j, k) step: the nodes of
+// the cube of half-side ceil(r / step) cells around Grid.coord_of_point (truncation of
+// (x - x0) / step, src/grid.ml:87-90) closer than r to p get bit b; nodes outside the grid
+// are skipped (the reference reads past its node arrays there)
+void bitmask_atom(std::vector<uint32_t>& mask, const int n[3], const double o[3], double step,
+                  const double* p, double r, bool b) {
+  const long ci = (long)((p[0] - o[0]) / step), cj = (long)((p[1] - o[1]) / step),
+             ck = (long)((p[2] - o[2]) / step);
+  const long rs = (long)std::ceil(r / step);
+  const double r2 = r * r;
+  for (long i = ci - rs; i <= ci + rs; ++i) {
+    if (i < 0 || i >= n[0]) continue;
+    const double dx = p[0] - (o[0] + i * step);
+    for (long j = cj - rs; j <= cj + rs; ++j) {
+      if (j < 0 || j >= n[1]) continue;
+      const double dy = p[1] - (o[1] + j * step);
+      for (long k = ck - rs; k <= ck + rs; ++k) {
+        if (k < 0 || k >= n[2]) continue;
+        const double dz = p[2] - (o[2] + k * step);
+        if (dx * dx + dy * dy + dz * dz < r2) {
+          const size_t v = (size_t)i + (size_t)j * n[0] + (size_t)k * n[0] * n[1];
+          if (b) mask[v >> 5] |= 1u << (v & 31);
+          else mask[v >> 5] &= ~(1u << (v & 31));
+        }
+      }
+    }
+  }
+}
+
+// Lds.first_solvent_shell (src/lds.ml:171-184): set within r_vdW + r_H2O (1.4 A,
+// src/const.ml:10) of every atom, then unset within r_vdW of every atom
+void solvent_shell(std::vector<uint32_t>& mask, const int n[3], const double o[3], double step,
+                   const double* xyz, const double* radii, int natoms) {
+  mask.assign(((size_t)n[0] * n[1] * n[2] + 31) / 32, 0u);
+  for (int a = 0; a < natoms; ++a) bitmask_atom(mask, n, o, step, xyz + 3 * (size_t)a, radii[a] + 1.4, true);
+  for (int a = 0; a < natoms; ++a) bitmask_atom(mask, n, o, step, xyz + 3 * (size_t)a, radii[a], false);
+}
+
+// Const.desolvation (src/const.ml:31): (1/eps_prot - 1/eps_H2O) / (8 pi), eps 4 and 78.5
+double desolvation_const() {
+  return (1.0 / 4.0 - 1.0 / 78.5) / (8.0 * (4.0 * std::atan(1.0)));
+}
+
 // The scan filters' inputs on the device: the per-translation skips (ROI sphere, centre
 // clash; computed here on the host), the rotations and translations, the conformer and
 // the vdW mask; `a.e` is left to the caller.  Enqueued on s from host arrays that the
@@ -1901,34 +1958,14 @@ int mmo_set_vdw_mask(mmo_ctx* c, const double* radii, double step, int nx, int n
   const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
   const size_t nvox = (size_t)nx * ny * nz;
   std::vector<uint32_t> mask((nvox + 31) / 32, 0u);
+  const int dims[3] = {nx, ny, nz};
   // atom_bitmask_set grid mask xyz r_vdW true (src/lds.ml:164-184) for every atom
   for (int a = 0; a < c->nrec; ++a) {
     const double rad = radii ? radii[a] : bondi_radius(c->rec_anum[a]);
     if (!(rad > 0.0) || !std::isfinite(rad))
       return fail(c, MMO_ERR_BAD_ARG, "receptor atom %d: no vdW radius (element %d)", a,
                   c->rec_anum[a]);
-    const double* p = &c->rec_host[3 * (size_t)a];
-    // Grid.coord_of_point (src/grid.ml:87-90): truncation of (x - x0) / step
-    const long ci = (long)((p[0] - o[0]) / step), cj = (long)((p[1] - o[1]) / step),
-               ck = (long)((p[2] - o[2]) / step);
-    const long rs = (long)std::ceil(rad / step);
-    const double r2 = rad * rad;
-    for (long i = ci - rs; i <= ci + rs; ++i) {
-      if (i < 0 || i >= nx) continue;
-      const double dx = p[0] - (o[0] + i * step);
-      for (long j = cj - rs; j <= cj + rs; ++j) {
-        if (j < 0 || j >= ny) continue;
-        const double dy = p[1] - (o[1] + j * step);
-        for (long k = ck - rs; k <= ck + rs; ++k) {
-          if (k < 0 || k >= nz) continue;
-          const double dz = p[2] - (o[2] + k * step);
-          if (dx * dx + dy * dy + dz * dz < r2) {
-            const size_t v = (size_t)i + (size_t)j * nx + (size_t)k * nx * ny;
-            mask[v >> 5] |= 1u << (v & 31);
-          }
-        }
-      }
-    }
+    bitmask_atom(mask, dims, o, step, &c->rec_host[3 * (size_t)a], rad, true);
   }
   if ((r = upload(c, c->d_mask, mask.data(), mask.size(), c->stream))) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1944,6 +1981,142 @@ int mmo_get_vdw_mask(const mmo_ctx* c, uint32_t* words) {
   if (!c || !c->has_mask) return MMO_ERR_BAD_ARG;
   if (words) std::memcpy(words, c->mask_host.data(), c->mask_host.size() * sizeof(uint32_t));
   return (int)c->mask_host.size();
+}
+
+// Desolvation (src/lds.ml:171-267, formula (2) of Majeux, Scarsi and Caflisch, Proteins
+// 2001): the receptor's first solvent shell on the grid origin + (i, j, k) step, and per
+// shell node inside the ROI sphere the protein contribution desolvation x voxel volume x
+// the sum over receptor atoms within 12 A of (q / d^2)^2 (k_desolv_nodes, FP64).
+int mmo_desolv_setup(mmo_ctx* c, const double* radii, double step, int nx, int ny, int nz,
+                     const double* origin, const double* roi_center, double roi_r) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_rec) return fail(c, MMO_ERR_NO_RECEPTOR, "no receptor set");
+  if (!(step > 0.0) || nx < 2 || ny < 2 || nz < 2 || (double)nx * ny * nz > 2147483647.0)
+    return fail(c, MMO_ERR_BAD_ARG, "bad desolvation grid (step %g, %d x %d x %d)", step, nx,
+                ny, nz);
+  if (roi_center && !(roi_r > 0.0)) return fail(c, MMO_ERR_BAD_ARG, "ROI radius %g", roi_r);
+  const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+  const int dims[3] = {nx, ny, nz};
+  const size_t nvox = (size_t)nx * ny * nz;
+  std::vector<double> rad(c->nrec);
+  for (int a = 0; a < c->nrec; ++a) {
+    rad[a] = radii ? radii[a] : bondi_radius(c->rec_anum[a]);
+    if (!(rad[a] > 0.0) || !std::isfinite(rad[a]))
+      return fail(c, MMO_ERR_BAD_ARG, "receptor atom %d: no vdW radius (element %d)", a,
+                  c->rec_anum[a]);
+  }
+  std::vector<uint32_t> shell;
+  solvent_shell(shell, dims, o, step, c->rec_host.data(), rad.data(), c->nrec);
+  // the shell nodes inside the ROI (ROI.is_inside: squared distance strictly below r^2)
+  std::vector<int> vox;
+  for (size_t w = 0; w < shell.size(); ++w)
+    for (uint32_t bits = shell[w]; bits; bits &= bits - 1) {
+      const size_t v = 32 * w + __builtin_ctz(bits);
+      if (roi_center) {
+        const size_t k = v / ((size_t)nx * ny), j = (v / nx) % ny, i = v % nx;
+        const double dx = roi_center[0] - (o[0] + i * step), dy = roi_center[1] - (o[1] + j * step),
+                     dz = roi_center[2] - (o[2] + k * step);
+        if (!(dx * dx + dy * dy + dz * dz < roi_r * roi_r)) continue;
+      }
+      vox.push_back((int)v);
+    }
+  std::vector<double4> at(c->nrec);
+  for (int a = 0; a < c->nrec; ++a)
+    at[a] = make_double4(c->rec_host[3 * (size_t)a], c->rec_host[3 * (size_t)a + 1],
+                         c->rec_host[3 * (size_t)a + 2], c->rec_q[a]);
+  hipStream_t s = c->stream;
+  HIPCHK(c, c->d_desolv.ensure(nvox));
+  HIPCHK(c, launch_fill(c->d_desolv.p, nvox, 0.0, s));
+  if ((r = upload(c, c->d_rec64, at.data(), at.size(), s))) return r;
+  if ((r = upload(c, c->s_dvox, vox.data(), vox.size(), s))) return r;
+  DesolvArgs a{};
+  a.vox = c->s_dvox.p; a.nvox = (int)vox.size();
+  a.atoms = c->d_rec64.p; a.natoms = c->nrec;
+  a.ox = o[0]; a.oy = o[1]; a.oz = o[2]; a.step = step; a.nx = nx; a.ny = ny;
+  a.scale = desolvation_const() * (step * step * step);
+  a.out = c->d_desolv.p;
+  HIPCHK(c, launch_desolv(a, false, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  c->desolv_shell.swap(shell);
+  c->desolv_step = step;
+  c->desolv_o[0] = o[0]; c->desolv_o[1] = o[1]; c->desolv_o[2] = o[2];
+  c->desolv_n[0] = nx; c->desolv_n[1] = ny; c->desolv_n[2] = nz;
+  c->has_desolv = true;
+  return (int)vox.size();
+}
+
+int mmo_get_solvent_shell(const mmo_ctx* c, uint32_t* words) {
+  if (!c || !c->has_desolv) return MMO_ERR_BAD_ARG;
+  if (words) std::memcpy(words, c->desolv_shell.data(), c->desolv_shell.size() * sizeof(uint32_t));
+  return (int)c->desolv_shell.size();
+}
+
+int mmo_get_desolv_contribs(mmo_ctx* c, double* out) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_desolv) return fail(c, MMO_ERR_BAD_ARG, "no desolvation data (mmo_desolv_setup)");
+  const size_t nvox = (size_t)c->desolv_n[0] * c->desolv_n[1] * c->desolv_n[2];
+  if (out) {
+    HIPCHK(c, hipMemcpyAsync(out, c->d_desolv.p, nvox * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return (int)nvox;
+}
+
+// Lds.desolvation_penalty (src/lds.ml:236-267) of one ligand pose: over the nodes of both
+// the receptor's and the ligand's first solvent shells, the sum of the protein
+// contributions (*prot) and desolvation x voxel volume x the ligand atoms' (q / d^2)^2
+// within 12 A (*lig); k_desolv_pose sums the nodes, the host adds the workgroups' partials.
+int mmo_desolv_penalty(mmo_ctx* c, const double* xyz, const double* q, const double* radii,
+                       int n, double* prot, double* lig) {
+  int r = check_ctx(c);
+  if (r) return r;
+  if (!c->has_desolv) return fail(c, MMO_ERR_BAD_ARG, "no desolvation data (mmo_desolv_setup)");
+  if (!xyz || !q || !radii || n <= 0 || !prot || !lig)
+    return fail(c, MMO_ERR_BAD_ARG, "bad desolvation penalty arguments (n=%d)", n);
+  for (int a = 0; a < n; ++a) {
+    if (!std::isfinite(xyz[3 * (size_t)a]) || !std::isfinite(xyz[3 * (size_t)a + 1]) ||
+        !std::isfinite(xyz[3 * (size_t)a + 2]) || !std::isfinite(q[a]))
+      return fail(c, MMO_ERR_BAD_ARG, "ligand atom %d not finite", a);
+    if (!(radii[a] > 0.0) || !std::isfinite(radii[a]))
+      return fail(c, MMO_ERR_BAD_ARG, "ligand atom %d: radius %g", a, radii[a]);
+  }
+  std::vector<uint32_t> ls;
+  solvent_shell(ls, c->desolv_n, c->desolv_o, c->desolv_step, xyz, radii, n);
+  std::vector<int> vox;
+  for (size_t w = 0; w < ls.size(); ++w)
+    for (uint32_t bits = ls[w] & c->desolv_shell[w]; bits; bits &= bits - 1)
+      vox.push_back((int)(32 * w + __builtin_ctz(bits)));
+  *prot = 0.0;
+  *lig = 0.0;
+  if (vox.empty()) return MMO_OK;
+  std::vector<double4> at(n);
+  for (int a = 0; a < n; ++a)
+    at[a] = make_double4(xyz[3 * (size_t)a], xyz[3 * (size_t)a + 1], xyz[3 * (size_t)a + 2], q[a]);
+  const int blocks = ((int)vox.size() + kDesolvNodes - 1) / kDesolvNodes;
+  hipStream_t s = c->stream;
+  if ((r = upload(c, c->s_datoms, at.data(), at.size(), s))) return r;
+  if ((r = upload(c, c->s_dvox, vox.data(), vox.size(), s))) return r;
+  HIPCHK(c, c->s_dpart.ensure(2 * (size_t)blocks));
+  DesolvArgs a{};
+  a.vox = c->s_dvox.p; a.nvox = (int)vox.size();
+  a.atoms = c->s_datoms.p; a.natoms = n;
+  a.ox = c->desolv_o[0]; a.oy = c->desolv_o[1]; a.oz = c->desolv_o[2];
+  a.step = c->desolv_step; a.nx = c->desolv_n[0]; a.ny = c->desolv_n[1];
+  a.prot = c->d_desolv.p;
+  a.part = c->s_dpart.p;
+  HIPCHK(c, launch_desolv(a, true, s));
+  std::vector<double> part(2 * (size_t)blocks);
+  HIPCHK(c, hipMemcpyAsync(part.data(), c->s_dpart.p, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  double sl = 0.0, sp = 0.0;
+  for (int b = 0; b < blocks; ++b) { sl += part[2 * b]; sp += part[2 * b + 1]; }
+  *prot = sp;
+  *lig = desolvation_const() * (c->desolv_step * c->desolv_step * c->desolv_step * sl);
+  return MMO_OK;
 }
 
 int mmo_scan_filter_dev(mmo_ctx* c, int lig_id, const double* rots, int nrot, const double* trans,

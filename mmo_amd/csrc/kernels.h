@@ -123,6 +123,23 @@ hipError_t launch_inter_scan_list(const InterLaunch& L, int lig, const double* r
 hipError_t launch_interp_scan_list(const InterpLaunch& L, int lig, const double* rots,
                                    const double* trans, const int* tperm, int ntrans,
                                    const int* list, int nlist, double* e_out, hipStream_t s);
+// desolvation sums (src/lds.ml:201-267): per listed node v (index i + j nx + k nx ny of
+// the grid origin + (i, j, k) step), sum over atoms (x, y, z, q) within 12 A of (q / d^2)^2
+struct DesolvArgs {
+  const int* vox;             // nodes
+  int nvox;
+  const double4* atoms;       // x, y, z, q
+  int natoms;
+  double ox, oy, oz, step;
+  int nx, ny;
+  double scale;               // nodes: out[v] = scale * sum
+  double* out;                // nodes: per grid node
+  const double* prot;         // pose: the protein contributions per grid node
+  double* part;               // pose: per workgroup (sum, sum of prot[v])
+};
+// k_desolv_pose writes part[2b], part[2b+1] for workgroup b of kDesolvNodes nodes
+constexpr int kDesolvNodes = 256;
+hipError_t launch_desolv(const DesolvArgs& a, bool pose, hipStream_t s);
 hipError_t launch_fill(double* p, size_t n, double v, hipStream_t s);
 hipError_t launch_fill_int(int* p, size_t n, int v, hipStream_t s);
 hipError_t launch_grid(const GridArgs& a, const CellGrid& cg, bool in_lds, hipStream_t s);

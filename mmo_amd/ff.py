@@ -253,6 +253,54 @@ class Context:
         lib.mmo_get_vdw_mask(self.h, _ptr(w))
         return w
 
+    def desolv_setup(self, step, dims, origin=None, roi=None, radii=None):
+        """The receptor's first solvent shell and per-node protein desolvation contributions
+        on the grid origin + (i, j, k) * step (Lds.first_solvent_shell and
+        Lds.protein_desolv, src/lds.ml:171-234); roi = (center, radius) limits the
+        contributions to the shell nodes inside the sphere.  Returns the number of nodes
+        given a contribution."""
+        r = _f64(radii) if radii is not None else None
+        o = _f64(origin) if origin is not None else None
+        c = _f64(roi[0]) if roi is not None else None
+        nx, ny, nz = (int(v) for v in dims)
+        rc = lib.mmo_desolv_setup(self.h, _ptr(r), float(step), nx, ny, nz, _ptr(o), _ptr(c),
+                                  float(roi[1]) if roi is not None else 0.0)
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def solvent_shell(self):
+        """The receptor's first solvent shell as 32-bit words (bit v of word v // 32 is node
+        v = i + j*nx + k*nx*ny)."""
+        n = lib.mmo_get_solvent_shell(self.h, None)
+        if n < 0:
+            raise MMOError(n, "no desolvation data")
+        w = np.zeros(n, dtype=np.uint32)
+        lib.mmo_get_solvent_shell(self.h, _ptr(w))
+        return w
+
+    def desolv_contribs(self):
+        """The per-node protein desolvation contributions (flat, node i + j*nx + k*nx*ny)."""
+        n = lib.mmo_get_desolv_contribs(self.h, None)
+        if n < 0:
+            self._check(n)
+        out = np.zeros(n, dtype=np.float64)
+        self._check(min(0, lib.mmo_get_desolv_contribs(self.h, _ptr(out))))
+        return out
+
+    def desolvation_penalty(self, xyz, q, radii):
+        """Lds.desolvation_penalty (src/lds.ml:236-267) of one ligand pose: (prot, lig)."""
+        x = _f64(np.asarray(xyz).reshape(-1, 3))
+        qq = _f64(q)
+        rr = _f64(radii)
+        if len(qq) != len(x) or len(rr) != len(x):
+            raise ValueError("xyz, q and radii disagree on the number of atoms")
+        prot = ct.c_double()
+        lig = ct.c_double()
+        self._check(lib.mmo_desolv_penalty(self.h, _ptr(x), _ptr(qq), _ptr(rr), len(x),
+                                           ct.byref(prot), ct.byref(lig)))
+        return prot.value, lig.value
+
     def scan_filter(self, lig_id, rots, trans, e, roi=None, lig_radii=None, flags=None,
                     stream=None):
         """Clash filters of the exhaustive scan on the energies `e` of ctx.scan(lig_id, ...,

@@ -204,6 +204,51 @@ def bitmask_roi(grid, center, out_r):
     return m
 
 
+_sig("orc_desolvation_const", _d, [])
+_sig("orc_first_solvent_shell", None, [_p, _p, _p, _p, _i, _p, _p])
+_sig("orc_protein_desolv", None, [_p, _p, _p, _p, _i, _p, _p, _d, _d, _d, _d, _p])
+_sig("orc_desolvation_penalty", None, [_p, _p, _p, _p, _p, _p, _p, _p, _i, _p, _p])
+
+
+def desolvation_const():
+    return lib.orc_desolvation_const()
+
+
+def first_solvent_shell(grid, xyz, radii):
+    """Lds.first_solvent_shell on the grid with lowest node (0, 0, 0): uint8 per node."""
+    k = _Keep()
+    x = _np.asarray(xyz, dtype=_np.float64)
+    m = _np.zeros(grid.x_dim * grid.y_dim * grid.z_dim, dtype=_np.uint8)
+    lib.orc_first_solvent_shell(k.f(x[:, 0]), k.f(x[:, 1]), k.f(x[:, 2]), k.f(radii), len(x),
+                                _ct.byref(grid), _ptr(m))
+    return m
+
+
+def protein_desolv(grid, prot, shell, roi_center, roi_r):
+    """Lds.protein_desolv: per node (FP64, 0 outside the shell and the ROI)."""
+    k = _Keep()
+    out = _np.zeros(grid.x_dim * grid.y_dim * grid.z_dim)
+    sh = _np.ascontiguousarray(shell, dtype=_np.uint8)
+    k.keep.append(sh)
+    lib.orc_protein_desolv(k.f(prot["xs"]), k.f(prot["ys"]), k.f(prot["zs"]), k.f(prot["q"]),
+                           len(prot["xs"]), _ct.byref(grid), _ptr(sh), float(roi_center[0]),
+                           float(roi_center[1]), float(roi_center[2]), float(roi_r), _ptr(out))
+    return out
+
+
+def desolvation_penalty(grid, prot_contribs, prot_shell, xyz, q, radii):
+    """Lds.desolvation_penalty: (prot, lig) of one ligand pose."""
+    k = _Keep()
+    x = _np.asarray(xyz, dtype=_np.float64)
+    pc = _np.ascontiguousarray(prot_contribs, dtype=_np.float64)
+    sh = _np.ascontiguousarray(prot_shell, dtype=_np.uint8)
+    k.keep += [pc, sh]
+    p, l = _ct.c_double(), _ct.c_double()
+    lib.orc_desolvation_penalty(_ct.byref(grid), _ptr(pc), _ptr(sh), k.f(x[:, 0]), k.f(x[:, 1]),
+                                k.f(x[:, 2]), k.f(q), k.f(radii), len(x), _ct.byref(p), _ct.byref(l))
+    return p.value, l.value
+
+
 def _mat(fn, *args):
     m = _np.zeros(9)
     fn(*args, _ptr(m))

@@ -711,3 +711,99 @@ void orc_scan_interp(const orc_grid_t* g, const float* const* grids,
     free(buf);
   }
 }
+
+/* ------------------------------------------------------------------ desolvation */
+/* Const.desolvation, src/const.ml:31 (eps_prot 4, eps_H2O 78.5, Const.r_H2O 1.4) */
+double orc_desolvation_const(void) {
+  const double pi = 4.0 * atan(1.0);
+  return (1.0 / 4.0 - 1.0 / 78.5) / (8.0 * pi);
+}
+
+/* Lds.atom_bitmask_set, src/lds.ml:145-169: the nodes of the cube of half-side
+ * ceil(r / step) cells around Grid.coord_of_point (truncation, src/grid.ml:87-90) whose
+ * distance to p is below r; indices outside the grid are skipped (the reference reads
+ * past its node arrays there) */
+static void atom_bitmask_set(const orc_grid_t* g, uint8_t* mask, double px, double py, double pz,
+                             double r, uint8_t b) {
+  const long nx = g->x_dim, ny = g->y_dim, nz = g->z_dim;
+  const long ci = (long)(px / g->step), cj = (long)(py / g->step), ck = (long)(pz / g->step);
+  const long rs = (long)ceil(r / g->step);
+  const double r2 = r * r;
+  for (long i = ci - rs; i <= ci + rs; ++i) {
+    if (i < 0 || i >= nx) continue;
+    for (long j = cj - rs; j <= cj + rs; ++j) {
+      if (j < 0 || j >= ny) continue;
+      for (long k = ck - rs; k <= ck + rs; ++k) {
+        if (k < 0 || k >= nz) continue;
+        const double dx = px - i * g->step, dy = py - j * g->step, dz = pz - k * g->step;
+        if (dx * dx + dy * dy + dz * dz < r2) mask[i + j * nx + k * nx * ny] = b;
+      }
+    }
+  }
+}
+
+/* Lds.first_solvent_shell, src/lds.ml:171-184: set within r_vdW + r_H2O of every atom,
+ * then unset within r_vdW of every atom */
+void orc_first_solvent_shell(const double* xs, const double* ys, const double* zs,
+                             const double* radii, int n, const orc_grid_t* g, uint8_t* mask) {
+  memset(mask, 0, (size_t)g->x_dim * g->y_dim * g->z_dim);
+  for (int a = 0; a < n; ++a) atom_bitmask_set(g, mask, xs[a], ys[a], zs[a], radii[a] + 1.4, 1);
+  for (int a = 0; a < n; ++a) atom_bitmask_set(g, mask, xs[a], ys[a], zs[a], radii[a], 0);
+}
+
+/* Lds.protein_desolv, src/lds.ml:201-234: for every protein solvent-shell node inside the
+ * ROI sphere, desolvation x voxel volume x sum over protein atoms within 12 A (BST
+ * neighbours) of (q_j / d^2)^2; 0 elsewhere */
+void orc_protein_desolv(const double* pxs, const double* pys, const double* pzs,
+                        const double* pq, int np, const orc_grid_t* g, const uint8_t* shell,
+                        double cx, double cy, double cz, double roi_r, double* out) {
+  const long nx = g->x_dim, ny = g->y_dim, nz = g->z_dim;
+  const double vol = g->step * g->step * g->step, kd = orc_desolvation_const();
+  for (long idx = 0; idx < nx * ny * nz; ++idx) {
+    out[idx] = 0.0;
+    if (!shell[idx]) continue;
+    const long k = idx / (nx * ny), j = (idx - k * nx * ny) / nx, i = idx - (k * nx * ny + j * nx);
+    const double x = i * g->step, y = j * g->step, z = k * g->step;
+    const double rx = cx - x, ry = cy - y, rz = cz - z;
+    if (!(rx * rx + ry * ry + rz * rz < roi_r * roi_r)) continue;   /* ROI.is_inside */
+    double s = 0.0;
+    for (int a = 0; a < np; ++a) {
+      const double dx = x - pxs[a], dy = y - pys[a], dz = z - pzs[a];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (!(d2 < CUTOFF2)) continue;
+      const double v = pq[a] / d2;
+      s += v * v;
+    }
+    out[idx] = kd * (vol * s);
+  }
+}
+
+/* Lds.desolvation_penalty, src/lds.ml:236-267: over the nodes in both the protein's and
+ * the ligand's first solvent shells, the protein contributions (prot) and desolvation x
+ * voxel volume x the ligand atoms' (q_j / d^2)^2 within 12 A (lig) */
+void orc_desolvation_penalty(const orc_grid_t* g, const double* prot_contribs,
+                             const uint8_t* prot_shell, const double* xs, const double* ys,
+                             const double* zs, const double* q, const double* radii, int n,
+                             double* prot, double* lig) {
+  const long nx = g->x_dim, ny = g->y_dim, nz = g->z_dim, nv = nx * ny * nz;
+  uint8_t* ls = (uint8_t*)malloc((size_t)nv);
+  orc_first_solvent_shell(xs, ys, zs, radii, n, g, ls);
+  double sp = 0.0, sl = 0.0;
+  for (long idx = 0; idx < nv; ++idx) {
+    if (!(ls[idx] && prot_shell[idx])) continue;
+    const long k = idx / (nx * ny), j = (idx - k * nx * ny) / nx, i = idx - (k * nx * ny + j * nx);
+    const double x = i * g->step, y = j * g->step, z = k * g->step;
+    for (int a = 0; a < n; ++a) {
+      const double dx = x - xs[a], dy = y - ys[a], dz = z - zs[a];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < CUTOFF2) {
+        const double v = q[a] / d2;
+        sl += v * v;
+      }
+    }
+    sp += prot_contribs[idx];
+  }
+  free(ls);
+  *prot = sp;
+  *lig = orc_desolvation_const() * (g->step * g->step * g->step * sl);
+}

@@ -158,4 +158,17 @@ void orc_scan_interp(const orc_grid_t* g, const float* const* grids,
 #ifdef __cplusplus
 }
 #endif
+/* desolvation (dead code in the reference, src/lds.ml:145-267, src/const.ml:31);
+ * coordinates relative to the grid's lowest node */
+double orc_desolvation_const(void);
+void orc_first_solvent_shell(const double* xs, const double* ys, const double* zs,
+                             const double* radii, int n, const orc_grid_t* g, uint8_t* mask);
+void orc_protein_desolv(const double* pxs, const double* pys, const double* pzs,
+                        const double* pq, int np, const orc_grid_t* g, const uint8_t* shell,
+                        double cx, double cy, double cz, double roi_r, double* out);
+void orc_desolvation_penalty(const orc_grid_t* g, const double* prot_contribs,
+                             const uint8_t* prot_shell, const double* xs, const double* ys,
+                             const double* zs, const double* q, const double* radii, int n,
+                             double* prot, double* lig);
+
 #endif

@@ -508,3 +508,102 @@ def test_dof_energy_gradient_matches_central_differences(config0):
     d6 = np.array([9.5, 0.3, -0.2, 0.1, -0.2, 0.3])
     _, _, g6, _ = oracle.dof_energy_grad(base, lig, d6, inter_grad)
     assert g6.shape == (6,)
+
+
+# ------------------------------------------------------------------ desolvation
+KD = float((mpmath.mpf(1) / 4 - mpmath.mpf(1) / mpmath.mpf("78.5")) / (8 * mpmath.pi))
+
+
+def _nodes(g):
+    k, j, i = np.meshgrid(np.arange(g.z_dim), np.arange(g.y_dim), np.arange(g.x_dim),
+                          indexing="ij")
+    return np.stack([i.ravel(), j.ravel(), k.ravel()], 1) * g.step   # node v = i + j nx + k nx ny
+
+
+def _shell_brute(g, xyz, radii):
+    """Every node tested against every atom (no cube walk): within r + 1.4 of some atom
+    and not within r of any."""
+    p = _nodes(g)
+    d2 = ((p[:, None, :] - np.asarray(xyz)[None, :, :]) ** 2).sum(-1)
+    r = np.asarray(radii)
+    return ((d2 < (r + 1.4) ** 2).any(1) & ~(d2 < r ** 2).any(1)).astype(np.uint8)
+
+
+def test_desolvation_constant():
+    """Const.desolvation = (1/eps_protein - 1/eps_water) / (8 pi), eps 4 and 78.5
+    (src/const.ml:31), at 50 digits."""
+    assert oracle.desolvation_const() == pytest.approx(KD, rel=1e-15)
+
+
+def test_first_solvent_shell_against_brute_force():
+    """Set within r + r_H2O then unset within r of every atom: with two overlapping atoms
+    the second atom's vdW sphere removes the first one's shell nodes; the cube walk around
+    the truncated node coordinate covers every node of each sphere."""
+    g = oracle.Grid.make(0.5, (24, 22, 20))
+    xyz = np.array([[5.13, 4.71, 5.02], [6.62, 5.33, 4.48]])
+    radii = np.array([1.7, 1.52])
+    got = oracle.first_solvent_shell(g, xyz, radii)
+    want = _shell_brute(g, xyz, radii)
+    assert np.array_equal(got, want)
+    one = oracle.first_solvent_shell(g, xyz[:1], radii[:1])
+    assert (one & ~got).any()   # nodes of atom 0's shell inside atom 1's sphere are gone
+    # a shell near the grid's corner: nodes outside the grid are skipped
+    edge = oracle.first_solvent_shell(g, [[0.4, 0.3, 0.2]], [1.2])
+    assert np.array_equal(edge, _shell_brute(g, [[0.4, 0.3, 0.2]], [1.2]))
+
+
+def test_protein_desolv_closed_form():
+    """One charge q at p: a shell node at distance d inside the ROI holds
+    Const.desolvation x step^3 x (q / d^2)^2; nodes outside the ROI, inside the atom, or
+    off the shell hold 0.  A second charge 12.5 A away adds nothing (12 A cutoff) to the
+    nodes around the first one; at 11 A it adds its own (q / d^2)^2."""
+    step = 0.5
+    g = oracle.Grid.make(step, (60, 24, 24))
+    p = np.array([6.0, 6.0, 6.0])
+    shell = oracle.first_solvent_shell(g, [p], [1.7])
+    prot = mol([p], [0.8], [6])
+    out = oracle.protein_desolv(g, prot, shell, p, 4.0)
+    v = lambda i, j, k: i + j * g.x_dim + k * g.x_dim * g.y_dim
+    # d = 2 (along x) and d = 3 (along y): both inside 1.7 + 1.4
+    assert out[v(16, 12, 12)] == pytest.approx(KD * step ** 3 * (0.8 / 4.0) ** 2, rel=1e-14)
+    assert out[v(12, 18, 12)] == pytest.approx(KD * step ** 3 * (0.8 / 9.0) ** 2, rel=1e-14)
+    assert out[v(12, 12, 12)] == 0.0                     # the atom's own node: no shell
+    assert out[v(20, 12, 12)] == 0.0                     # d = 4: off the shell
+    roi_small = oracle.protein_desolv(g, prot, shell, p + [0.0, 2.0, 0.0], 1.5)
+    assert roi_small[v(16, 12, 12)] == 0.0               # outside the ROI sphere
+    assert roi_small[v(12, 16, 12)] > 0.0                # d = 2, at the ROI's centre
+    assert np.count_nonzero(out) == np.count_nonzero(shell)   # the ROI holds the shell
+    for dx, adds in ((12.5, False), (11.0, True)):
+        two = mol([p, p + [dx + 2.0, 0.0, 0.0]], [0.8, -0.6], [6, 6])
+        o2 = oracle.protein_desolv(g, two, shell, p, 4.0)
+        extra = KD * step ** 3 * (0.6 / dx ** 2) ** 2 if adds else 0.0
+        assert o2[v(16, 12, 12)] == pytest.approx(
+            KD * step ** 3 * (0.8 / 4.0) ** 2 + extra, rel=1e-14)
+
+
+def test_desolvation_penalty_brute_force_and_invariants():
+    """Over the nodes of both shells: prot sums the protein contributions there, lig is
+    Const.desolvation x step^3 x the ligand atoms' (q / d^2)^2 within 12 A, summed over
+    those nodes; lig scales with q^2 and prot does not see the ligand's charges; a ligand
+    whose shell misses the protein's gives (0, 0)."""
+    step = 0.5
+    g = oracle.Grid.make(step, (30, 26, 24))
+    pxyz = np.array([[6.1, 6.3, 5.9], [7.4, 5.2, 6.6], [5.2, 7.0, 6.8]])
+    prad = np.array([1.7, 1.55, 1.52])
+    prot = mol(pxyz, [0.4, -0.5, 0.3], [6, 7, 8])
+    shell = oracle.first_solvent_shell(g, pxyz, prad)
+    contribs = oracle.protein_desolv(g, prot, shell, pxyz.mean(0), 30.0)
+    lxyz = np.array([[9.3, 6.4, 6.2], [10.1, 7.5, 6.0]])
+    lq, lrad = np.array([0.35, -0.25]), np.array([1.7, 1.2])
+    pp, ll = oracle.desolvation_penalty(g, contribs, shell, lxyz, lq, lrad)
+    both = (_shell_brute(g, lxyz, lrad) & _shell_brute(g, pxyz, prad)).astype(bool)
+    assert both.sum() > 20
+    nodes = _nodes(g)[both]
+    d2 = ((nodes[:, None, :] - lxyz[None]) ** 2).sum(-1)
+    lig_sum = np.where(d2 < 144.0, (lq[None, :] / d2) ** 2, 0.0).sum()
+    assert ll == pytest.approx(KD * step ** 3 * lig_sum, rel=1e-13)
+    assert pp == pytest.approx(contribs[both].sum(), rel=1e-13)
+    p2, l2 = oracle.desolvation_penalty(g, contribs, shell, lxyz, 2.0 * lq, lrad)
+    assert p2 == pp and l2 == pytest.approx(4.0 * ll, rel=1e-14)
+    far = oracle.desolvation_penalty(g, contribs, shell, lxyz + [4.5, 4.0, 0.0], lq, lrad)
+    assert far == (0.0, 0.0)
