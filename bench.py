@@ -750,8 +750,9 @@ def main():
     flops_pose, mean_in = algorithmic_flops_per_pose(lig, rec, rots, trans)
     avg_ms = float(np.mean(kernel_ms))
     achieved = flops_pose * npose / (avg_ms * 1e-3) / 1e12
-    kernel = ("k_inter_cull" if ctx.tuning("scan_cull") > 0 and ctx.tuning("cull_ok") > 0
-              else "k_inter")
+    sc = ctx.tuning("scan_cull")
+    kernel = ("k_scan_rows" if sc >= 5 else
+              "k_inter_cull" if sc > 0 and ctx.tuning("cull_ok") > 0 else "k_inter")
     line = {
         "metric": "poses/sec (energy, energy+grad) on 1 MI355X at <=1e-5 rel err vs FP64; MC steps/s",
         "value": npose * args.steps / elapsed,
@@ -777,7 +778,9 @@ def main():
                      "flop_per_pose": flops_pose, "in_cutoff_pairs_per_pose": mean_in},
         "tuning": {"key": scan_key, "applied": tuned, "cell_size": ctx.tuning("cell_size"),
                    "scan_cull": ctx.tuning("scan_cull"),
-                   "cull_k": ctx.tuning("cull_k"), "cull_waves": ctx.tuning("cull_waves")},
+                   "cull_k": ctx.tuning("cull_k"), "cull_waves": ctx.tuning("cull_waves"),
+                   "scan_cell": ctx.tuning("scan_cell"), "scan_xsub": ctx.tuning("scan_xsub"),
+                   "scan_chunk": ctx.tuning("scan_chunk")},
     }
     want = set(x for x in args.extras.split(",") if x) if not args.only_scan else set()
     if want:

@@ -118,7 +118,7 @@ struct mmo_ctx {
   int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
   int mc_sweep = 1;           // the same choice for the Monte Carlo direct sum
   bool cull_ok = false;
-  int scan_cull = 1;
+  int scan_cull = 5;
   int cull_k = 1;
   int mc_waves_per_cu = 16;
   int mc_wg_waves = 16;       // Monte Carlo chains (wavefronts) per workgroup
@@ -127,12 +127,22 @@ struct mmo_ctx {
   int min_waves = 16;         // fused minimiser: wavefronts per workgroup
   int min_rec_lds = 1;        // fused minimiser: receptor in LDS when >= 8 wavefronts fit
   int mc_rec_lds = 1;         // Monte Carlo direct form: receptor in LDS when it fits
-  int cull_waves = 16;         // rigid scan: wavefronts per workgroup
+  int cull_waves = 8;          // rigid scan: wavefronts per workgroup
   int pose_waves = 16;         // batches (one wavefront per pose): wavefronts per workgroup
   CellGrid cg{};              // scan and grid kernels: cubic cells
   DevBuf<int> d_cell;
   CellGrid cgp{};             // pose and Monte Carlo sweeps: cells cut along x, same atom order
   DevBuf<int> d_cellp;
+  // rigid scan row lists (scan_cull 5, 6): cells of scan_cell A in y and z, cut scan_xsub
+  // times along x, with a copy of the receptor sorted by that grid
+  float scan_cell = 3.5f;
+  int scan_xsub = 8;
+  int scan_chunk = 4;
+  CellGrid cgs{};
+  DevBuf<int> d_cells;
+  DevBuf<RecPos> rpos_s;
+  DevBuf<RecAB> rab_s;
+  DevBuf<RecRes> rres_s;
   bool cullp_ok = false;
   // grids
   bool has_grid = false;
@@ -439,6 +449,7 @@ InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAto
   L.cull = 0;
   L.cull_waves = c->cull_waves;
   L.cull_k = c->cull_k;
+  L.row_chunk = c->scan_chunk;
   L.cg = c->cg;
   L.fr = c->fr;
   L.rpos = c->rpos.p;
@@ -526,10 +537,19 @@ int plain_setup(mmo_ctx* c, const double* xs, const double* ys, const double* zs
 // the scan kernel's configuration.  scan_cull 1: receptor staged in LDS when it fits;
 // 2: read through the scalar cache; 3: read through the scalar cache, rows cut along x by
 // the pose sweeps' cell table; 4: positions in LDS, LJ factors through the scalar cache;
-// 0: the unculled kernel
+// 5: row lists over the scan grid (k_scan_rows), receptor through the scalar cache;
+// 6: row lists, receptor in LDS when it fits; 0: the unculled kernel
 InterLaunch scan_launch(mmo_ctx* c, int mode, int scan_cull) {
   InterLaunch L = inter_launch(c, mode, c->d_ligs.p, c->d_latoms.p, c->d_lig_xyz.p);
   L.cull = 0;
+  if (mode == MMO_SHIFTED && scan_cull >= 5) {
+    L.cull = (scan_cull == 6 && (size_t)(c->nrec + kRowChunk) * (sizeof(RecPos) + sizeof(RecAB)) <=
+                                    150 * 1024) ? 5 : 4;
+    L.cg = c->cgs;
+    L.rpos = c->rpos_s.p; L.rab = c->rab_s.p; L.rres = c->rres_s.p;
+    L.row_chunk = c->scan_chunk;
+    return L;
+  }
   if (mode == MMO_SHIFTED && scan_cull) {
     L.cull = 2;
     if (scan_cull == 1 && c->cull_ok) L.cull = 1;
@@ -631,6 +651,7 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->rpos.release(); c->rab.release(); c->rres.release(); c->grid.release();
+    c->rpos_s.release(); c->rab_s.release(); c->rres_s.release(); c->d_cells.release();
     c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
     c->d_pair_ij.release(); c->d_pair_par.release(); c->d_sweep.release();
     c->d_rb_left.release(); c->d_rb_right.release(); c->d_rg_off.release(); c->d_rg_atoms.release();
@@ -769,6 +790,42 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   if ((r = upload(c, c->rres, res.data(), res.size(), c->stream))) return r;
   if ((r = upload(c, c->d_cell, cstart.data(), cstart.size(), c->stream))) return r;
   if ((r = upload(c, c->d_cellp, cstartp.data(), cstartp.size(), c->stream))) return r;
+  // the rigid scan's row-list grid: cells of scan_cell A in y and z and scan_cell /
+  // scan_xsub A along x, atoms sorted by cell and by x within a cell (so every row of
+  // cells is sorted by x), with its own copy of the receptor arrays in that order
+  const double css = c->scan_cell, cssx = css / c->scan_xsub;
+  const int snx = (int)std::floor((hi[0] - fr.ox - gx0) / cssx) + 1;
+  const int sny = (int)std::floor((hi[1] - fr.oy - gy0) / css) + 1;
+  const int snz = (int)std::floor((hi[2] - fr.oz - gz0) / css) + 1;
+  if ((double)snx * sny * snz > 6.4e7)
+    return fail(c, MMO_ERR_BAD_ARG, "scan cell grid too large (%d x %d x %d)", snx, sny, snz);
+  std::vector<int> rank(n), cells(n), orders(n);
+  for (int k2 = 0; k2 < n; ++k2) rank[order[k2]] = k2;
+  for (int i = 0; i < n; ++i) {
+    const int cx = std::min(snx - 1, std::max(0, (int)std::floor((xs[i] - fr.ox - gx0) / cssx)));
+    const int cy = std::min(sny - 1, std::max(0, (int)std::floor((ys[i] - fr.oy - gy0) / css)));
+    const int cz = std::min(snz - 1, std::max(0, (int)std::floor((zs[i] - fr.oz - gz0) / css)));
+    cells[i] = (cz * sny + cy) * snx + cx;
+    orders[i] = i;
+  }
+  std::stable_sort(orders.begin(), orders.end(), [&](int a, int b) {
+    return cells[a] < cells[b] || (cells[a] == cells[b] && xs[a] < xs[b]);
+  });
+  std::vector<int> cstarts((size_t)snx * sny * snz + 1, 0);
+  for (int i = 0; i < n; ++i) cstarts[cells[i] + 1]++;
+  for (size_t k2 = 0; k2 + 1 < cstarts.size(); ++k2) cstarts[k2 + 1] += cstarts[k2];
+  // kRowChunk - 1 inert atoms of padding: row_sweep fetches whole chunks
+  std::vector<RecPos> pos_s(n + kRowChunk);
+  std::vector<RecAB> ab_s(n + kRowChunk);
+  std::vector<RecRes> res_s(n + kRowChunk);
+  for (int k2 = 0; k2 < n; ++k2) {
+    const int from = rank[orders[k2]];
+    pos_s[k2] = pos[from]; ab_s[k2] = ab[from]; res_s[k2] = res[from];
+  }
+  if ((r = upload(c, c->rpos_s, pos_s.data(), pos_s.size(), c->stream))) return r;
+  if ((r = upload(c, c->rab_s, ab_s.data(), ab_s.size(), c->stream))) return r;
+  if ((r = upload(c, c->rres_s, res_s.data(), res_s.size(), c->stream))) return r;
+  if ((r = upload(c, c->d_cells, cstarts.data(), cstarts.size(), c->stream))) return r;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   CellGrid cg{};
   cg.cell_start = c->d_cell.p;
@@ -783,6 +840,12 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   cgp.cx = (float)csx; cgp.inv_cx = (float)(1.0 / csx);
   cgp.nx = gnxp;
   c->cgp = cgp;
+  CellGrid cgs = cg;
+  cgs.cell_start = c->d_cells.p;
+  cgs.c = (float)css; cgs.inv_c = (float)(1.0 / css);
+  cgs.cx = (float)cssx; cgs.inv_cx = (float)(1.0 / cssx);
+  cgs.nx = snx; cgs.ny = sny; cgs.nz = snz;
+  c->cgs = cgs;
   // the culled kernel keeps the whole receptor and the cell table in LDS (160 KiB/CU)
   const size_t lds = (size_t)n * (sizeof(RecPos) + sizeof(RecAB)) + (ncell + 1) * sizeof(int);
   c->cull_ok = lds <= 150 * 1024;
@@ -2482,8 +2545,11 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "cull_k" && pow2_in(iv, 1, 4)) c->cull_k = iv;
   else if (k == "cull_waves" && iv >= 1 && iv <= 16) c->cull_waves = iv;
   else if (k == "pose_waves" && iv >= 1 && iv <= 16) c->pose_waves = iv;
-  else if (k == "scan_cull" && iv >= 0 && iv <= 4) c->scan_cull = iv;
+  else if (k == "scan_cull" && iv >= 0 && iv <= 6) c->scan_cull = iv;
   else if (k == "cell_size" && v >= 1.0 && v <= 24.0) c->cell_size = (float)v;
+  else if (k == "scan_cell" && v >= 1.0 && v <= 24.0) c->scan_cell = (float)v;
+  else if (k == "scan_xsub" && iv >= 1 && iv <= 16) c->scan_xsub = iv;
+  else if (k == "scan_chunk" && (iv == 4 || iv == 8)) c->scan_chunk = iv;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "mc_wg_waves" && iv >= 1 && iv <= 16) c->mc_wg_waves = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
@@ -2508,6 +2574,9 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "inter_rtile") return c->inter_rtile;
   if (k == "cull_k") return c->cull_k;
   if (k == "cull_waves") return c->cull_waves;
+  if (k == "scan_cell") return c->scan_cell;
+  if (k == "scan_xsub") return c->scan_xsub;
+  if (k == "scan_chunk") return c->scan_chunk;
   if (k == "pose_waves") return c->pose_waves;
   if (k == "scan_cull") return c->scan_cull;
   if (k == "cell_size") return c->cell_size;

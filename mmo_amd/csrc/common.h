@@ -55,6 +55,10 @@ struct Frame {
 // Receptor cell list for culling: atoms sorted by cell (z, y, x order, x fastest);
 // cell_start[c]..cell_start[c+1] are the atoms of cell c.  Coordinates in A relative
 // to the fixed-point frame origin.
+// receptor atoms fetched at once by the rigid scan's row sweep; its receptor arrays carry
+// this many inert atoms of padding so that a fetch may run past the last atom
+constexpr int kRowChunk = 8;
+
 struct CellGrid {
   const int* cell_start;   // ncell + 1
   float x0, y0, z0;        // corner of cell (0,0,0)

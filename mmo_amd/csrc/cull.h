@@ -204,6 +204,116 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
   }
 }
 
+// Rigid-scan sweep over a list of cell rows, one ligand atom per lane.  The (y, z) rows
+// of the grid cg that come within 12 A of the box [q - h, q + h] are enumerated 64 at a
+// time across the lanes: each lane takes one candidate row, computes the x chord of the
+// box grown by 12 A at that row and the atom range of the cells the chord covers (two
+// reads of the cell table, all lanes at once).  The non-empty rows are then swept in
+// (z, y) order by the whole wavefront, the lowest set bit of a ballot first, each row's
+// range read back with readlane.  FP32 sums are folded once per z slab, after the slab's
+// last row: pairs beyond 12 A add exact zeros, so a lane's energy depends only on its
+// own atom, not on the box it was swept with.  fold(sq, s12, s6) as in cull_sweep.
+// SMEM: the receptor arrays are read with wavefront-uniform addresses from global memory
+// (scalar loads into SGPRs); else from LDS into VGPRs.
+template <bool SMEM, int CHUNK, class Fold>
+__device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __restrict__ s_pos,
+                                          const RecAB* __restrict__ s_ab,
+                                          const RecRes* __restrict__ rres, float inv_s,
+                                          float inv_s2, float qx, float qy, float qz, float hx,
+                                          float hy, float hz, const LigPt& l, Fold&& fold) {
+  const PairK pk = pair_k(inv_s2, inv_s, rres);
+  constexpr float R = 12.05f;   // cutoff + FP32 rounding margin
+  constexpr float R2 = R * R;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int cz0 = max(0, (int)floorf((qz - hz - R - cg.z0) * cg.inv_c));
+  const int cz1 = min(cg.nz - 1, (int)floorf((qz + hz + R - cg.z0) * cg.inv_c));
+  const int cy0 = max(0, (int)floorf((qy - hy - R - cg.y0) * cg.inv_c));
+  const int cy1 = min(cg.ny - 1, (int)floorf((qy + hy + R - cg.y0) * cg.inv_c));
+  if (cz0 > cz1 || cy0 > cy1) return;
+  const int ncy = cy1 - cy0 + 1;
+  const int nc = (cz1 - cz0 + 1) * ncy;
+  // candidate c -> (c / ncy, c % ncy) from the float quotient of (c + 1/2) / ncy, which
+  // lies at least 1/(2 ncy) from an integer (exact for c < 2^16; corrected by one beyond)
+  const float inv_ncy = 1.0f / (float)ncy;
+  float sq = 0.0f, s12 = 0.0f, s6 = 0.0f;
+  int slab = -1;   // z slab of the running FP32 sums
+  for (int base = 0; base < nc; base += kWave) {
+    const int cidx = base + lane;
+    int a0 = 0, a1 = 0, tag = 0;
+    if (cidx < nc) {
+      int dzi = (int)(((float)cidx + 0.5f) * inv_ncy);
+      dzi -= dzi * ncy > cidx ? 1 : 0;
+      dzi += (dzi + 1) * ncy <= cidx ? 1 : 0;
+      const int cz = cz0 + dzi, cy = cy0 + (cidx - dzi * ncy);
+      const float zlo = cg.z0 + cz * cg.c, ylo = cg.y0 + cy * cg.c;
+      const float dz = fmaxf(0.0f, fmaxf(zlo - (qz + hz), (qz - hz) - (zlo + cg.c)));
+      const float dy = fmaxf(0.0f, fmaxf(ylo - (qy + hy), (qy - hy) - (ylo + cg.c)));
+      const float rx2 = R2 - dz * dz - dy * dy;
+      if (rx2 >= 0.0f) {
+        const float rx = hx + bound_sqrt(rx2);
+        const int cx0 = max(0, (int)floorf((qx - rx - cg.x0) * cg.inv_cx));
+        const int cx1 = min(cg.nx - 1, (int)floorf((qx + rx - cg.x0) * cg.inv_cx));
+        if (cx0 <= cx1) {
+          const int rowb = (cz * cg.ny + cy) * cg.nx;
+          a0 = cg.cell_start[rowb + cx0];
+          a1 = cg.cell_start[rowb + cx1 + 1];
+        }
+      }
+      // a row 0.5 A or more from the box in y or z holds no atom closer than 0.5 A to
+      // any lane's atom: its pairs skip the near-pair residual check
+      tag = 2 * cz + ((dz < 0.55f && dy < 0.55f) ? 1 : 0);
+    }
+    uint64_t rows = __ballot(a1 > a0);
+    while (rows) {
+      const int r = __builtin_ctzll(rows);
+      rows &= rows - 1;
+      const int ra0 = __builtin_amdgcn_readlane(a0, r);
+      const int ra1 = __builtin_amdgcn_readlane(a1, r);
+      const int rtag = __builtin_amdgcn_readlane(tag, r);
+      if ((rtag >> 1) != slab) {
+        if (slab >= 0) fold(sq, s12, s6);
+        sq = s12 = s6 = 0.0f;
+        slab = rtag >> 1;
+      }
+      if (rtag & 1) {
+        for (int i = ra0; i < ra1; ++i) {
+          const RecPos p = s_pos[i];
+          const RecAB ab = s_ab[i];
+          pair_energy<MMO_SHIFTED, true>(p, ab, l, i, pk, sq, s12, s6);
+        }
+      } else {
+        // CHUNK (<= kRowChunk) atoms fetched together (one memory latency per chunk of a
+        // short row; the arrays are padded so that a chunk may run past the last atom)
+        for (int c0 = ra0; c0 < ra1; c0 += CHUNK) {
+          RecPos pp[CHUNK];
+          RecAB aa[CHUNK];
+#pragma unroll
+          for (int k = 0; k < CHUNK; ++k) {
+            pp[k] = s_pos[c0 + k];
+            aa[k] = s_ab[c0 + k];
+          }
+          // every fetch of the chunk completes here, before the first pair: without this
+          // the compiler sinks each atom's fetch (and a wait) into that atom's branch
+#pragma unroll
+          for (int k = 0; k < CHUNK; ++k) {
+            if constexpr (SMEM)
+              asm volatile("" ::"s"(pp[k].x), "s"(pp[k].y), "s"(pp[k].z), "s"(pp[k].q), "s"(aa[k].a),
+                           "s"(aa[k].b));
+            else
+              asm volatile("" ::"v"(pp[k].x), "v"(pp[k].y), "v"(pp[k].z), "v"(pp[k].q), "v"(aa[k].a),
+                           "v"(aa[k].b));
+          }
+          const int cnt = ra1 - c0;
+#pragma unroll
+          for (int k = 0; k < CHUNK; ++k)
+            if (k < cnt) pair_energy<MMO_SHIFTED, false>(pp[k], aa[k], l, c0 + k, pk, sq, s12, s6);
+        }
+      }
+    }
+  }
+  if (slab >= 0) fold(sq, s12, s6);
+}
+
 // FP64 variant of the shifted sweep (Monte Carlo parity mode): exact fixed-point
 // differences, FP64 pair arithmetic (sqrt, division) and FP64 sums; returns the sums
 // already weighted by ligand parameters cq (ke q_j), ca (A_j), cb (B_j).

@@ -422,6 +422,72 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
   }
 }
 
+// Rigid scan over the row lists of a finer cell grid (row_sweep): one pose per lane, one
+// ligand atom at a time.  RMODE 0: receptor read in place (uniform addresses, scalar
+// cache); 1: receptor positions and LJ factors staged in LDS.  The cell table stays in
+// global memory (it is read per lane, once per row).
+template <int RMODE, int CHUNK, class Src>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+k_scan_rows(InterArgs<Src> a, CellGrid cg) {
+  extern __shared__ __align__(16) char smem[];
+  const RecPos* s_pos = a.rpos;
+  const RecAB* s_ab = a.rab;
+  if constexpr (RMODE == 1) {
+    // with the padding atoms (the row sweep's chunks may read past the last atom)
+    const int nr = a.nrec + kRowChunk;
+    RecPos* p = reinterpret_cast<RecPos*>(smem);
+    RecAB* b = reinterpret_cast<RecAB*>(smem + (size_t)nr * sizeof(RecPos));
+    for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+      p[i] = a.rpos[i];
+      b[i] = a.rab[i];
+    }
+    __syncthreads();
+    s_pos = p;
+    s_ab = b;
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  const int tidx = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+  if (tidx >= a.ntiles) return;
+  const Tile t = a.src.tile(tidx);
+  const bool pose_ok = lane < t.npose;
+  const int pose = pose_ok ? a.src.pose_id(t.pose0 + lane) : 0;
+  const LigInfo li = a.ligs[t.lig];
+  const Frame fr = a.fr;
+  double e_el = 0.0, e_vd = 0.0;
+  int sat = 0;
+  for (int j = 0; j < li.n_atoms; ++j) {
+    LigPt l[1];
+    bool valid[1] = {pose_ok};
+    double cq = 0.0, ca = 0.0, cb = 0.0;
+    if (pose_ok) {
+      double x, y, z;
+      a.src.pos(pose, lane, j, li, a.latoms, a.lig_xyz, x, y, z);
+      l[0] = lig_pt(x, y, z, fr, &sat);
+      const LigAtom la = a.latoms[li.atom_base + j];
+      cq = la.keq; ca = la.a; cb = la.b;
+    } else {
+      l[0] = lig_pt_zero();
+    }
+    float qx, qy, qz, hx, hy, hz;
+    if (!wave_box<1>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) continue;
+    row_sweep<RMODE == 0, CHUNK>(cg, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx, hy, hz, l[0],
+              [&](float sq, float s12, float s6) {
+                e_el = fma((double)sq, cq, e_el);
+                e_vd = fma((double)s12, ca, e_vd);
+                e_vd = fma(-2.0 * (double)s6, cb, e_vd);
+              });
+  }
+  (void)sat;   // saturated atoms stay beyond the cutoff of every receptor atom
+  if (pose_ok) {
+    const size_t o = (size_t)pose * a.e_stride;
+    a.e_out[o] = e_el + e_vd;
+    if (a.e_comp) {
+      a.e_comp[o + 2] = e_el;
+      a.e_comp[o + 3] = e_vd;
+    }
+  }
+}
+
 // ----------------------------------------------------- inter kernel, pose-culled
 // ligand atoms per pass of the per-atom sweep
 constexpr int kPoseSweepAtoms = 4;
@@ -794,7 +860,33 @@ hipError_t launch_inter_coords(const InterLaunch& L, const Tile* tiles, const in
 }
 
 template <class Src>
+static hipError_t launch_rows(const InterLaunch& L, const InterArgs<Src>& a, hipStream_t s) {
+  const int waves = L.cull_waves;
+  const int blocks = (a.ntiles + waves - 1) / waves;
+  if (blocks == 0) return hipSuccess;
+  const size_t lds = L.cull == 5 ? (size_t)(a.nrec + kRowChunk) * (sizeof(RecPos) + sizeof(RecAB)) : 0;
+#define MMO_ROWS(RM, CH)                                                                       \
+  {                                                                                            \
+    if (lds > 0) {                                                                             \
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_rows<RM, CH, Src>), \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+      if (e != hipSuccess) return e;                                                           \
+    }                                                                                          \
+    hipLaunchKernelGGL((k_scan_rows<RM, CH, Src>), dim3(blocks), dim3(waves * kWave), lds, s, a, \
+                       L.cg);                                                                  \
+  }
+  if (L.cull == 5) {
+    if (L.row_chunk == 4) MMO_ROWS(1, 4) else MMO_ROWS(1, 8)
+  } else {
+    if (L.row_chunk == 4) MMO_ROWS(0, 4) else MMO_ROWS(0, 8)
+  }
+#undef MMO_ROWS
+  return hipGetLastError();
+}
+
+template <class Src>
 static hipError_t launch_cull(const InterLaunch& L, const InterArgs<Src>& a, hipStream_t s) {
+  if (L.cull >= 4) return launch_rows(L, a, s);
   const int waves = L.cull_waves;
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;

@@ -12,9 +12,12 @@ struct InterLaunch {
   int waves;           // wavefronts per block
   int rtile;           // receptor atoms per LDS tile
   int rslice;          // receptor atoms per block row (gridDim.y > 1 splits the receptor)
-  int cull;            // culled shifted form: 1 receptor cell list in LDS, 2 in global memory
+  int cull;            // culled shifted form: 1 receptor cell list in LDS, 2 in global memory,
+                       // 3 positions in LDS; 4 row lists (k_scan_rows), receptor in place,
+                       // 5 row lists with the receptor in LDS
   int cull_waves;      // wavefronts per block of the culled kernel
   int cull_k;          // ligand atoms per lane in the culled kernel (1, 2, 4)
+  int row_chunk;       // k_scan_rows: receptor atoms fetched at once (4 or 8)
   CellGrid cg;
   Frame fr;
   const RecPos* rpos;

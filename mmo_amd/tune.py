@@ -6,8 +6,13 @@ The library exposes its launch shapes as context settings (`Context.tune`):
                           (wavefronts per workgroup, 1-16), cell_size (receptor cell
                           edge in A; takes effect at the next set_receptor), scan_cull
                           (1 receptor in LDS, 2 read through the scalar cache, 3 the
-                          same on the cells cut along x), cell_xsub (x cut of the pose
-                          and Monte Carlo sweeps' cells; next set_receptor)
+                          same on the cells cut along x, 4 positions in LDS, 5 row
+                          lists of the scan grid, 6 row lists with the receptor in
+                          LDS), scan_cell / scan_xsub (the row lists' cell edge in y
+                          and z and its x cut; next set_receptor), scan_chunk
+                          (receptor atoms fetched at once by the row lists: 4 or 8),
+                          cell_xsub (x cut of the pose and Monte Carlo sweeps' cells;
+                          next set_receptor)
   plain batches           inter_k (1, 2, 4, 8), inter_waves (1-4), inter_rtile (receptor
                           atoms per LDS tile)
   batches (pose culled)   pose_waves (wavefronts per workgroup, 1-16)
@@ -83,11 +88,15 @@ def _time(fn, reps):
 
 
 def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0), ks=(1,),
-              waves=(4, 8, 16), scan_culls=(1, 2, 3), reps=2, store=True):
+              waves=(4, 8, 16), scan_culls=(1, 2, 3),
+              row_grids=((3.0, 8), (3.5, 8), (4.0, 8), (3.5, 4)), row_chunks=(4, 8),
+              row_waves=(4, 8), reps=2, store=True):
     """Culled rigid scan: sweeps cell size x receptor path (LDS or scalar cache) x atoms
-    per lane x wavefronts per workgroup on a slice of the caller's scan (rots, trans) and
-    leaves the best setting applied.  The pose sweeps' x cut (cell_xsub) stays as set:
-    it is the Monte Carlo's choice."""
+    per lane x wavefronts per workgroup of the cell-sweep kernel, and row-list grid (cell
+    edge in y and z, x cut) x receptor atoms fetched at once x wavefronts per workgroup of
+    the row-list kernel (scan_cull 5), on a slice of the caller's scan (rots, trans), and
+    leaves the best setting applied.  The pose sweeps' x cut (cell_xsub) stays as set: it
+    is the Monte Carlo's choice."""
     from . import ff
     import torch
     n = len(rots) * len(trans)
@@ -95,26 +104,42 @@ def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0), ks=(1,),
     ctx.scan(lig_id, ff.SHIFTED, rots, trans, out=out)
     ref = out.clone()
     results = []
+
+    def measure(setting):
+        def run():
+            ctx.scan(lig_id, ff.SHIFTED, rots, trans, out=out)
+            return ctx.last_kernel_ms("inter")
+        ms = _time(run, reps)
+        ok = bool(torch.allclose(out, ref, rtol=2e-6, atol=1e-6))
+        results.append(dict(setting, ms=ms, ok=ok))
+
+    keep_cell, keep_scell, keep_xsub = (ctx.tuning("cell_size"), ctx.tuning("scan_cell"),
+                                        ctx.tuning("scan_xsub"))
     for cs in cell_sizes:
         ctx.tune(cell_size=cs)
         ctx.set_receptor(rec)
         for sc, k, w in itertools.product(scan_culls, ks, waves):
             ctx.tune(scan_cull=sc, cull_k=k, cull_waves=w)
-
-            def run():
-                ctx.scan(lig_id, ff.SHIFTED, rots, trans, out=out)
-                return ctx.last_kernel_ms("inter")
-            ms = _time(run, reps)
-            ok = bool(torch.allclose(out, ref, rtol=2e-6, atol=1e-6))
-            results.append({"cell_size": cs, "scan_cull": sc, "cull_k": k, "cull_waves": w,
-                            "ms": ms, "ok": ok})
+            measure({"cell_size": cs, "scan_cull": sc, "cull_k": k, "cull_waves": w})
+    ctx.tune(cell_size=keep_cell)
+    for (cell, xsub) in row_grids:
+        ctx.tune(scan_cell=cell, scan_xsub=xsub)
+        ctx.set_receptor(rec)
+        for ch, w in itertools.product(row_chunks, row_waves):
+            ctx.tune(scan_cull=5, scan_chunk=ch, cull_waves=w, cull_k=1)
+            measure({"cell_size": keep_cell, "scan_cull": 5, "cull_k": 1, "cull_waves": w,
+                     "scan_cell": cell, "scan_xsub": xsub, "scan_chunk": ch})
     good = [r for r in results if r["ok"]]
     best = min(good, key=lambda r: r["ms"])
-    ctx.tune(cell_size=best["cell_size"])
+    chosen = {k: v for k, v in best.items() if k not in ("ms", "ok")}
+    chosen.setdefault("scan_cell", keep_scell)
+    chosen.setdefault("scan_xsub", keep_xsub)
+    ctx.tune(cell_size=chosen["cell_size"], scan_cell=chosen["scan_cell"],
+             scan_xsub=chosen["scan_xsub"])
     ctx.set_receptor(rec)
-    ctx.tune(scan_cull=best["scan_cull"], cull_k=best["cull_k"], cull_waves=best["cull_waves"])
-    entry = {"best": {k: best[k] for k in ("cell_size", "scan_cull", "cull_k", "cull_waves")},
-             "ms": best["ms"], "poses": n, "sweep": results}
+    ctx.tune(**{k: v for k, v in chosen.items()
+                if k not in ("cell_size", "scan_cell", "scan_xsub")})
+    entry = {"best": chosen, "ms": best["ms"], "poses": n, "sweep": results}
     if store:
         _store(key_for("scan", lig_atoms=ctx.ligand_natoms(lig_id), rec_atoms=len(rec["xs"])), entry)
     return entry

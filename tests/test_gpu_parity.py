@@ -289,8 +289,9 @@ def test_rigid_scan_subset_matches_oracle(sys0):
 def test_rigid_scan_full_size_sample_and_invariants(sys0):
     """The scan at its full size (4,608 rotations x 20^3 translations = 36,864,000 poses):
     a random sample of 1,024 poses against the oracle, under the library's tuned shape
-    and every receptor path (LDS, scalar cache, x-cut cells, positions in LDS), which
-    must agree to the last bit with one another where they visit the same pairs."""
+    and every receptor path (LDS, scalar cache, x-cut cells, positions in LDS, row lists
+    through the scalar cache or LDS), which agree with one another to FP32 rounding (the
+    paths fold their FP32 partial sums over different groups of receptor atoms)."""
     import torch
     ctx, lig, rec = sys0
     lid = ctx.add_ligand(lig)
@@ -310,7 +311,7 @@ def test_rigid_scan_full_size_sample_and_invariants(sys0):
     assert_energy(e[sample], ref, sums, what="full scan sample")
     keep = ctx.tuning("scan_cull")
     try:
-        for sc in (1, 2, 4):
+        for sc in (1, 2, 4, 5, 6):
             ctx.tune(scan_cull=sc)
             print("full scan, receptor path", sc, flush=True)
             ctx.scan(lid, ff.SHIFTED, rots, trans, out=out)
