@@ -450,6 +450,7 @@ InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAto
   L.cull_waves = c->cull_waves;
   L.cull_k = c->cull_k;
   L.row_chunk = c->scan_chunk;
+  L.atom_order = nullptr;
   L.cg = c->cg;
   L.fr = c->fr;
   L.rpos = c->rpos.p;
@@ -548,6 +549,7 @@ InterLaunch scan_launch(mmo_ctx* c, int mode, int scan_cull) {
     L.cg = c->cgs;
     L.rpos = c->rpos_s.p; L.rab = c->rab_s.p; L.rres = c->rres_s.p;
     L.row_chunk = c->scan_chunk;
+    L.atom_order = c->d_sweep.p;
     return L;
   }
   if (mode == MMO_SHIFTED && scan_cull) {

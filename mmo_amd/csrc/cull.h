@@ -204,23 +204,24 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
   }
 }
 
-// Rigid-scan sweep over a list of cell rows, one ligand atom per lane.  The (y, z) rows
+// Rigid-scan sweep over a list of cell rows, K ligand atoms per lane.  The (y, z) rows
 // of the grid cg that come within 12 A of the box [q - h, q + h] are enumerated 64 at a
 // time across the lanes: each lane takes one candidate row, computes the x chord of the
 // box grown by 12 A at that row and the atom range of the cells the chord covers (two
 // reads of the cell table, all lanes at once).  The non-empty rows are then swept in
 // (z, y) order by the whole wavefront, the lowest set bit of a ballot first, each row's
-// range read back with readlane.  FP32 sums are folded once per z slab, after the slab's
-// last row: pairs beyond 12 A add exact zeros, so a lane's energy depends only on its
-// own atom, not on the box it was swept with.  fold(sq, s12, s6) as in cull_sweep.
+// range read back with readlane; every receptor atom fetched is paired with the lane's
+// K atoms.  FP32 sums are folded once per z slab, after the slab's last row: pairs
+// beyond 12 A add exact zeros, so a lane's energy depends only on its own atoms, not on
+// the box it was swept with.  fold(sq, s12, s6) receives the K slab sums.
 // SMEM: the receptor arrays are read with wavefront-uniform addresses from global memory
 // (scalar loads into SGPRs); else from LDS into VGPRs.
-template <bool SMEM, int CHUNK, class Fold>
+template <bool SMEM, int CHUNK, int K, class Fold>
 __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __restrict__ s_pos,
                                           const RecAB* __restrict__ s_ab,
                                           const RecRes* __restrict__ rres, float inv_s,
                                           float inv_s2, float qx, float qy, float qz, float hx,
-                                          float hy, float hz, const LigPt& l, Fold&& fold) {
+                                          float hy, float hz, const LigPt (&l)[K], Fold&& fold) {
   const PairK pk = pair_k(inv_s2, inv_s, rres);
   constexpr float R = 12.05f;   // cutoff + FP32 rounding margin
   constexpr float R2 = R * R;
@@ -235,7 +236,9 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
   // candidate c -> (c / ncy, c % ncy) from the float quotient of (c + 1/2) / ncy, which
   // lies at least 1/(2 ncy) from an integer (exact for c < 2^16; corrected by one beyond)
   const float inv_ncy = 1.0f / (float)ncy;
-  float sq = 0.0f, s12 = 0.0f, s6 = 0.0f;
+  float sq[K], s12[K], s6[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
   int slab = -1;   // z slab of the running FP32 sums
   for (int base = 0; base < nc; base += kWave) {
     const int cidx = base + lane;
@@ -272,14 +275,17 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
       const int rtag = __builtin_amdgcn_readlane(tag, r);
       if ((rtag >> 1) != slab) {
         if (slab >= 0) fold(sq, s12, s6);
-        sq = s12 = s6 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; ++k) sq[k] = s12[k] = s6[k] = 0.0f;
         slab = rtag >> 1;
       }
       if (rtag & 1) {
         for (int i = ra0; i < ra1; ++i) {
           const RecPos p = s_pos[i];
           const RecAB ab = s_ab[i];
-          pair_energy<MMO_SHIFTED, true>(p, ab, l, i, pk, sq, s12, s6);
+#pragma unroll
+          for (int k = 0; k < K; ++k)
+            pair_energy<MMO_SHIFTED, true>(p, ab, l[k], i, pk, sq[k], s12[k], s6[k]);
         }
       } else {
         // CHUNK (<= kRowChunk) atoms fetched together (one memory latency per chunk of a
@@ -306,7 +312,11 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
           const int cnt = ra1 - c0;
 #pragma unroll
           for (int k = 0; k < CHUNK; ++k)
-            if (k < cnt) pair_energy<MMO_SHIFTED, false>(pp[k], aa[k], l, c0 + k, pk, sq, s12, s6);
+            if (k < cnt) {
+#pragma unroll
+              for (int m = 0; m < K; ++m)
+                pair_energy<MMO_SHIFTED, false>(pp[k], aa[k], l[m], c0 + k, pk, sq[m], s12[m], s6[m]);
+            }
         }
       }
     }
