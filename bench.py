@@ -36,6 +36,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3        # MI355X FP32 vector peak (AMD spec)
 PEAK_HBM_GBS = 8000.0
+# L2 read bandwidth: 16 channels of 128 B per clock in each XCD's L2, 8 XCDs, 2.4 GHz
+# (the L2 structure AMD documents for CDNA 3, kept on CDNA 4; no MI355X figure is published)
+PEAK_L2_GBS = 8 * 16 * 128 * 2.4
 N_ROT, N_TRANS_SIDE, TRANS_STEP = 4608, 20, 0.5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
@@ -87,12 +90,19 @@ def valu_roofline(flop, kernel_ms, kernel, **more):
     return r
 
 
-def gather_roofline(nbytes, kernel_ms, kernel, **more):
+def gather_roofline(nbytes, kernel_ms, kernel, traffic=None, **more):
+    """Trilinear gathers: the grids stay in L2 and the Infinity Cache, so the corner reads
+    are measured against the L2 read bandwidth; `traffic` (bytes per launch beyond L2,
+    from the FETCH_SIZE / WRITE_SIZE counters) is also given as a rate against HBM."""
     gbs = nbytes / (kernel_ms * 1e-3) / 1e9
-    r = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-         "frac": gbs / PEAK_HBM_GBS, "kernel": kernel, "kernel_ms": kernel_ms,
-         "note": "grid corner reads (32 B per ligand atom) over the kernel time; the grids "
-                 "stay in L2 / Infinity Cache, so the reads are served from there"}
+    r = {"bound": "l2", "achieved": gbs, "peak": PEAK_L2_GBS, "unit": "GB/s",
+         "frac": gbs / PEAK_L2_GBS, "kernel": kernel, "kernel_ms": kernel_ms,
+         "traffic": traffic,
+         "note": "grid corner reads (32 B per ligand atom) over the kernel time, served by "
+                 "L2 and the Infinity Cache"}
+    if traffic:
+        r["beyond_l2_GBs"] = traffic / (kernel_ms * 1e-3) / 1e9
+        r["beyond_l2_frac_of_hbm"] = r["beyond_l2_GBs"] / PEAK_HBM_GBS
     r.update(more)
     return r
 
