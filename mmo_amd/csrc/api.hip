@@ -1594,6 +1594,19 @@ int mmo_mc_create(mmo_ctx* c, const int32_t* chain_lig, int nchain, const double
     s0.max_trans = 0.15;         // Params.max_trans, src/params.ml:13-14
     s0.status = kMcRunning;
   }
+  {
+    // one chain per workgroup must fit: its conformer, undo rows and chain state in LDS
+    McArgs a{};
+    a.max_atoms = m->max_atoms;
+    const size_t need = mc_lds_bytes(a, 1, false);
+    if (need > kMaxLds) {
+      const int na = m->max_atoms;
+      delete m;
+      return fail(c, MMO_ERR_BAD_ARG,
+                  "ligands of %d atoms need %zu bytes of LDS per chain (at most %zu)", na, need,
+                  (size_t)kMaxLds);
+    }
+  }
   std::vector<double> dr(std::max(m->nrb, 1), 5.0 * deg);   // Params.max_rbond_rot
   std::vector<SwState> sw(std::max(m->nrb, 1));
   std::memset(sw.data(), 0, sw.size() * sizeof(SwState));

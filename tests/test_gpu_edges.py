@@ -149,6 +149,16 @@ def test_size_limits(system):
     d = torch.zeros(42, dtype=torch.float64, device="cuda:0")
     e = torch.zeros((1, 4), dtype=torch.float64, device="cuda:0")
     b.score_dofs_dev(d, e, flags=ff.F_INTER_SHIFTED | ff.F_INTRA)
+    # a ligand whose chain state does not fit in one workgroup's LDS (7,000 atoms: 24 B of
+    # conformer per atom): rejected when the chains are created, before any launch
+    n = 7000
+    dists = np.full((n, n), 2, dtype=np.int32)
+    np.fill_diagonal(dists, 0)
+    big = {"xs": np.linspace(-5.0, 5.0, n), "ys": np.zeros(n), "zs": np.zeros(n),
+           "q": np.zeros(n), "anum": np.full(n, 6, dtype=np.int32), "dists": dists}
+    bid = ctx.add_ligand(big)
+    assert _code(lambda: mcmod.MonteCarlo(ctx, [bid], np.eye(3)[None], np.zeros((1, 3)),
+                                          ff.SHIFTED, total_frames=1)) == ERR_BAD_ARG
     # top-k bounds
     with pytest.raises(MMOError):
         ctx.topk(torch.zeros(10, dtype=torch.float64, device="cuda:0"), 5000)

@@ -180,12 +180,23 @@ def lib_sys():
     return ctx, ligs, ids, rec, (grids, step, origin)
 
 
-def test_batch_gradient_rows_fully_written(lib_sys):
+@pytest.mark.parametrize("pose_sweep", [1, 0])
+def test_batch_gradient_rows_fully_written(lib_sys, pose_sweep):
     """The pose kernel writes every atom's gradient row (the buffer is not cleared
     first): a device gradient buffer filled with NaN comes back finite, and a second call
-    on the same batch gives bit for bit what a fresh batch gives."""
-    import torch
+    on the same batch gives bit for bit what a fresh batch gives; under both pose sweeps
+    (per atom, and per 8-atom sub-block)."""
     ctx, ligs, ids, rec, _ = lib_sys
+    keep = ctx.tuning("pose_sweep")
+    ctx.tune(pose_sweep=pose_sweep)
+    try:
+        _gradient_rows_fully_written(ctx, ligs, ids)
+    finally:
+        ctx.tune(pose_sweep=keep)
+
+
+def _gradient_rows_fully_written(ctx, ligs, ids):
+    import torch
     rng = np.random.default_rng(12)
     pose_lig = np.asarray([ids[k % len(ids)] for k in range(37)])
     b = ctx.batch(pose_lig)
