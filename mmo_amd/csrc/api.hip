@@ -111,9 +111,9 @@ struct mmo_ctx {
   DevBuf<RecAB> rab;
   DevBuf<RecRes> rres;
   // receptor cell list (culled shifted kernel)
-  float cell_size = 5.0f;
+  float cell_size = 4.0f;
   int row_clip = 4;           // scan/grid sweeps: clip rows longer than this to the x chord
-  int cell_xsub = 4;          // pose/MC sweeps: cells cut n-fold along x (rows clipped finer)
+  int cell_xsub = 3;          // pose/MC sweeps: cells cut n-fold along x (rows clipped finer)
   int pose_cull = 1;          // batches in the shifted form: one wavefront per pose, culled
   int pose_sweep = 1;         // 1: lanes over receptor atoms per ligand atom; 0: 8-atom sub-blocks
   int mc_sweep = 1;           // the same choice for the Monte Carlo direct sum
