@@ -88,6 +88,7 @@ _sig("orc_grad_inter", _d, [_i] + _mol + _mol + [_p, _p, _p])
 _sig("orc_grad_intra", _d, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p])
 _sig("orc_grad_interp", _d, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p])
 _sig("orc_scan_shifted", None, _mol + [_p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _i, _p])
+_sig("orc_scan_shifted_cells", None, _mol + [_p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _i, _p])
 _sig("orc_scan_interp", None, [_p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _i, _i, _p])
 
 
@@ -414,6 +415,21 @@ def scan_shifted(prot, centered_lig, rots, pose_rot, trans, pose_trans, nthreads
                          k.f(_np.asarray(rots).reshape(-1)), k.i(pose_rot),
                          k.f(_np.asarray(trans).reshape(-1)), k.i(pose_trans), npose,
                          int(nthreads), _ptr(out))
+    return out
+
+
+def scan_shifted_cells(prot, centered_lig, rots, pose_rot, trans, pose_trans, nthreads=1):
+    """FP64 shifted energies of rigid-scan poses through a neighbour search (the reference's
+    default Bst form), the same terms as scan_shifted in another order."""
+    k = _Keep()
+    npose = len(pose_rot)
+    out = _np.zeros(npose)
+    lig = centered_lig
+    lib.orc_scan_shifted_cells(*_molargs(k, prot), k.f(lig["xs"]), k.f(lig["ys"]), k.f(lig["zs"]),
+                               k.f(lig["q"]), k.i(lig["anum"]), len(lig["xs"]),
+                               k.f(_np.asarray(rots).reshape(-1)), k.i(pose_rot),
+                               k.f(_np.asarray(trans).reshape(-1)), k.i(pose_trans), npose,
+                               int(nthreads), _ptr(out))
     return out
 
 

@@ -683,6 +683,124 @@ void orc_scan_shifted(const double* pxs, const double* pys, const double* pzs,
   }
 }
 
+/* The reference's default form, -ff Bst (src/lds.ml:1790; ene_inter_UFF_shifted_bst_components,
+ * src/mol.ml:928-956): per ligand atom, only the receptor atoms that a neighbour query
+ * returns within 12 A are visited (the reference queries a vantage-point tree of the
+ * third-party `bst` library).  Restated with a grid of 4 A cells over the receptor: for
+ * each (y, z) row of cells that meets the atom's 12 A sphere, the cells the sphere's x
+ * chord covers.  Same FP64 terms as orc_ene_inter_shifted, summed ligand atom by ligand
+ * atom; the energies agree with orc_scan_shifted to FP64 rounding.  The CPU timing
+ * baseline of the scan. */
+void orc_scan_shifted_cells(const double* pxs, const double* pys, const double* pzs,
+                            const double* pq, const int32_t* panum, int np,
+                            const double* cxs, const double* cys, const double* czs,
+                            const double* lq, const int32_t* lanum, int nl,
+                            const double* rots, const int32_t* pose_rot,
+                            const double* trans, const int32_t* pose_trans, int npose,
+                            int nthreads, double* e_out) {
+  const double cs = 4.0;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < np; ++i) {
+    const double v[3] = {pxs[i], pys[i], pzs[i]};
+    for (int a = 0; a < 3; ++a) { if (v[a] < lo[a]) lo[a] = v[a]; if (v[a] > hi[a]) hi[a] = v[a]; }
+  }
+  int n[3];
+  for (int a = 0; a < 3; ++a) n[a] = (int)floor((hi[a] - lo[a]) / cs) + 1;
+  const long ncell = (long)n[0] * n[1] * n[2];
+  int* start = (int*)calloc((size_t)ncell + 1, sizeof(int));
+  int* cell = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+  int* order = (int*)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+  for (int i = 0; i < np; ++i) {
+    int c[3];
+    const double v[3] = {pxs[i], pys[i], pzs[i]};
+    for (int a = 0; a < 3; ++a) {
+      c[a] = (int)floor((v[a] - lo[a]) / cs);
+      if (c[a] >= n[a]) c[a] = n[a] - 1;
+    }
+    cell[i] = (c[2] * n[1] + c[1]) * n[0] + c[0];
+    start[cell[i] + 1]++;
+  }
+  for (long k = 0; k < ncell; ++k) start[k + 1] += start[k];
+  {
+    int* fill = (int*)malloc(sizeof(int) * (size_t)(ncell > 0 ? ncell : 1));
+    for (long k = 0; k < ncell; ++k) fill[k] = start[k];
+    for (int i = 0; i < np; ++i) order[fill[cell[i]]++] = i;
+    free(fill);
+  }
+  /* the receptor in cell order */
+  double* sx = (double*)malloc(sizeof(double) * 4 * (size_t)(np > 0 ? np : 1));
+  double *sy = sx + np, *sz = sy + np, *sq = sz + np;
+  int32_t* sa = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+  for (int k = 0; k < np; ++k) {
+    const int i = order[k];
+    sx[k] = pxs[i]; sy[k] = pys[i]; sz[k] = pzs[i]; sq[k] = pq[i]; sa[k] = panum[i];
+  }
+  const double ke = orc_elec_weight();
+  /* pair parameters looked up, as the reference's 119 x 119 table (UFF.vdW_xiDi,
+   * src/UFF.ml:42-51), for the element pairs present */
+  double* tx = (double*)calloc(119 * 119, sizeof(double));
+  double* td = (double*)calloc(119 * 119, sizeof(double));
+  for (int k = 0; k < np; ++k)
+    for (int j = 0; j < nl; ++j) {
+      const int a1 = sa[k], a2 = lanum[j];
+      if (a1 < 0 || a1 > 118 || a2 < 0 || a2 > 118) continue;
+      if (tx[a1 * 119 + a2] == 0.0) {
+        xd_t v; pair_params(a1, a2, &v);
+        tx[a1 * 119 + a2] = v.x; td[a1 * 119 + a2] = v.d;
+      }
+    }
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+  #pragma omp parallel num_threads(nthreads)
+#endif
+  {
+#ifdef _OPENMP
+    #pragma omp for schedule(dynamic, 4)
+#endif
+    for (int p = 0; p < npose; ++p) {
+      const double* R = rots + 9 * pose_rot[p];
+      const double* t = trans + 3 * pose_trans[p];
+      double sum_elec = 0.0, sum_vdw = 0.0;
+      for (int j = 0; j < nl; ++j) {
+        double o[3];
+        rotate_v(R, cxs[j], cys[j], czs[j], o);
+        const double x = o[0] + t[0], y = o[1] + t[1], z = o[2] + t[2];
+        const int z0 = (int)floor((z - 12.0 - lo[2]) / cs), z1 = (int)floor((z + 12.0 - lo[2]) / cs);
+        const int y0 = (int)floor((y - 12.0 - lo[1]) / cs), y1 = (int)floor((y + 12.0 - lo[1]) / cs);
+        for (int cz = (z0 < 0 ? 0 : z0); cz <= z1 && cz < n[2]; ++cz) {
+          const double zl = lo[2] + cz * cs;
+          const double dz = z < zl ? zl - z : (z > zl + cs ? z - (zl + cs) : 0.0);
+          for (int cy = (y0 < 0 ? 0 : y0); cy <= y1 && cy < n[1]; ++cy) {
+            const double yl = lo[1] + cy * cs;
+            const double dy = y < yl ? yl - y : (y > yl + cs ? y - (yl + cs) : 0.0);
+            const double rx2 = 144.0 - dz * dz - dy * dy;
+            if (rx2 < 0.0) continue;
+            const double rx = sqrt(rx2);
+            int x0 = (int)floor((x - rx - lo[0]) / cs), x1 = (int)floor((x + rx - lo[0]) / cs);
+            if (x0 < 0) x0 = 0;
+            if (x1 >= n[0]) x1 = n[0] - 1;
+            if (x0 > x1) continue;
+            const long rowb = ((long)cz * n[1] + cy) * n[0];
+            for (int k = start[rowb + x0]; k < start[rowb + x1 + 1]; ++k) {
+              const double ddx = sx[k] - x, ddy = sy[k] - y, ddz = sz[k] - z;
+              const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+              if (!(r2 < CUTOFF2)) continue;
+              const double r = orc_non_zero_dist(sqrt(r2));
+              const double w = orc_shift_12A(r);
+              const int pi = sa[k] * 119 + lanum[j];
+              const double p6 = orc_pow6(tx[pi] / r);
+              sum_elec = sum_elec + w * ((sq[k] * lq[j]) / r);
+              sum_vdw = sum_vdw + w * (td[pi] * ((-2.0 * p6) + (p6 * p6)));
+            }
+          }
+        }
+      }
+      e_out[p] = (ke * sum_elec) + sum_vdw;
+    }
+  }
+  free(start); free(cell); free(order); free(sx); free(sa); free(tx); free(td);
+}
+
 void orc_scan_interp(const orc_grid_t* g, const float* const* grids,
                      const double* cxs, const double* cys, const double* czs,
                      const int32_t* ltype, int nl,

@@ -148,6 +148,14 @@ void orc_scan_shifted(const double* pxs, const double* pys, const double* pzs,
                       const double* rots, const int32_t* pose_rot,
                       const double* trans, const int32_t* pose_trans, int npose,
                       int nthreads, double* e_out);
+/* -ff Bst restated with a cell grid (the CPU timing baseline; same terms) */
+void orc_scan_shifted_cells(const double* pxs, const double* pys, const double* pzs,
+                            const double* pq, const int32_t* panum, int np,
+                            const double* cxs, const double* cys, const double* czs,
+                            const double* lq, const int32_t* lanum, int nl,
+                            const double* rots, const int32_t* pose_rot,
+                            const double* trans, const int32_t* pose_trans, int npose,
+                            int nthreads, double* e_out);
 void orc_scan_interp(const orc_grid_t* g, const float* const* grids,
                      const double* cxs, const double* cys, const double* czs,
                      const int32_t* ltype, int nl,

@@ -607,3 +607,26 @@ def test_desolvation_penalty_brute_force_and_invariants():
     assert p2 == pp and l2 == pytest.approx(4.0 * ll, rel=1e-14)
     far = oracle.desolvation_penalty(g, contribs, shell, lxyz + [4.5, 4.0, 0.0], lq, lrad)
     assert far == (0.0, 0.0)
+
+
+def test_neighbour_search_scan_equals_brute_force():
+    """The neighbour-search restatement of the default (Bst) form sums the same terms as
+    the brute-force one: clashing, contact and far poses agree to FP64 rounding, and a
+    pose wholly beyond 12 A of the receptor is exactly 0 in both."""
+    from mmo_amd import synth
+    lig, rec = synth.config0()
+    rots = synth.so3_rotations(4608)
+    # the cavity lattice (clashing poses), the outer surface (contacts) and far away
+    surface = [[34.0, 0.0, 0.0], [33.0, 2.0, -1.0], [0.0, 34.0, 0.0]]
+    trans = np.concatenate([synth.lattice((0.0, 0.0, 0.0), 20, 0.5), surface,
+                            [[200.0, 0.0, 0.0]]])
+    rng = np.random.default_rng(7)
+    pr = rng.integers(0, len(rots), 300).astype(np.int32)
+    pt = rng.integers(0, 8000, 300).astype(np.int32)
+    pt[-25:-1] = np.repeat(np.arange(8000, 8003), 8)
+    pt[-1] = len(trans) - 1
+    a = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=4)
+    b = oracle.scan_shifted_cells(rec, lig, rots, pr, trans, pt, nthreads=4)
+    assert np.all(np.abs(a - b) <= 1e-12 * np.maximum(np.abs(a), 1.0))
+    assert a[-1] == 0.0 and b[-1] == 0.0
+    assert (a > 1e3).any() and (a[-25:-1] < 0).any()
