@@ -149,6 +149,8 @@ struct mmo_ctx {
   int ntypes = 0;
   GridDev g{};
   DevBuf<float> grid;
+  DevBuf<float> grid_cells;   // the grids packed by cell (GridDev::cells)
+  int grid_pack = 1;          // pack the grids by cell when they are set (up to 4 GiB)
   // ligand registry
   std::vector<LigHost> ligs;
   bool lig_dirty = true;
@@ -653,6 +655,7 @@ void mmo_ctx_destroy(mmo_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->rpos.release(); c->rab.release(); c->rres.release(); c->grid.release();
+    c->grid_cells.release();
     c->rpos_s.release(); c->rab_s.release(); c->rres_s.release(); c->d_cells.release();
     c->d_ligs.release(); c->d_latoms.release(); c->d_lig_xyz.release();
     c->d_pair_ij.release(); c->d_pair_par.release(); c->d_sweep.release();
@@ -867,6 +870,26 @@ int mmo_set_receptor(mmo_ctx* c, const double* xs, const double* ys, const doubl
   return MMO_OK;
 }
 
+namespace {
+// The cell-packed copy of the context's grids (8 corners per cell contiguous, 32 B): the
+// interpolating kernels then fetch a cell with two 16-byte loads instead of 8 gathers.
+// Skipped (the kernels gather from the node arrays) when grid_pack is 0 or the copy
+// would pass 4 GiB.
+int pack_grid_cells(mmo_ctx* c, GridDev& g, int ntypes, hipStream_t s) {
+  g.cells = nullptr;
+  const size_t n = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1) * ntypes * 8;
+  if (!c->grid_pack || n * sizeof(float) > ((size_t)4 << 30)) {
+    c->grid_cells.release();
+    return MMO_OK;
+  }
+  HIPCHK(c, c->grid_cells.ensure(n));
+  HIPCHK(c, launch_pack_cells(g, ntypes, c->grid_cells.p, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  g.cells = c->grid_cells.p;
+  return MMO_OK;
+}
+}  // namespace
+
 int mmo_set_grids(mmo_ctx* c, const float* const* grids, int ntypes, double step, int nx, int ny,
                   int nz, const double* origin) {
   int r = check_ctx(c);
@@ -890,6 +913,7 @@ int mmo_set_grids(mmo_ctx* c, const float* const* grids, int ntypes, double step
   g.ox = origin ? origin[0] : 0.0;
   g.oy = origin ? origin[1] : 0.0;
   g.oz = origin ? origin[2] : 0.0;
+  if ((r = pack_grid_cells(c, g, ntypes, c->stream))) return r;
   c->g = g;
   c->ntypes = ntypes;
   c->has_grid = true;
@@ -1518,6 +1542,7 @@ int mmo_grid_compute(mmo_ctx* c, const int32_t* probe_anum, const double* probe_
   g.step = step;
   g.ods = 1.0 / step;
   g.ox = origin[0]; g.oy = origin[1]; g.oz = origin[2];
+  if ((r = pack_grid_cells(c, g, ntypes, s))) return r;
   c->g = g;
   c->ntypes = ntypes;
   c->has_grid = true;
@@ -2565,6 +2590,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "scan_cell" && v >= 1.0 && v <= 24.0) c->scan_cell = (float)v;
   else if (k == "scan_xsub" && iv >= 1 && iv <= 16) c->scan_xsub = iv;
   else if (k == "scan_chunk" && (iv == 4 || iv == 8)) c->scan_chunk = iv;
+  else if (k == "grid_pack" && (iv == 0 || iv == 1)) c->grid_pack = iv;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "mc_wg_waves" && iv >= 1 && iv <= 16) c->mc_wg_waves = iv;
   else if (k == "pose_cull" && (iv == 0 || iv == 1)) c->pose_cull = iv;
@@ -2592,6 +2618,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "scan_cell") return c->scan_cell;
   if (k == "scan_xsub") return c->scan_xsub;
   if (k == "scan_chunk") return c->scan_chunk;
+  if (k == "grid_pack") return c->grid_pack;
   if (k == "pose_waves") return c->pose_waves;
   if (k == "scan_cull") return c->scan_cull;
   if (k == "cell_size") return c->cell_size;

@@ -40,11 +40,34 @@ __device__ __forceinline__ void rot_xyz(const double* sn, const double* cs, doub
   }
 }
 
-// G3D.trilin (src/G3D.ml:97-157) of the grid `arr` at p - origin = (px, py, pz), FP64
+// The 8 corner values of cell (i0, j0, k0) of type ty's grid in the corner order a..h of
+// src/G3D.ml:144-151: two 16-byte loads from the cell-packed copy when there is one,
+// else 8 gathers from the node array.
+// CELLS = false: the node arrays only (callers whose lanes share nodes).
+template <bool CELLS = true>
+__device__ __forceinline__ void grid_corners(const GridDev& g, int ty, int i0, int j0, int k0,
+                                             float (&c)[8]) {
+  if (CELLS && g.cells) {
+    const size_t cell = (((size_t)ty * (g.nz - 1) + k0) * (g.ny - 1) + j0) * (g.nx - 1) + i0;
+    const float4* q = reinterpret_cast<const float4*>(g.cells + 8 * cell);
+    const float4 lo = q[0], hi = q[1];
+    c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w;
+    c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
+    return;
+  }
+  const float* arr = g.data + (size_t)ty * g.nvox;
+  const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
+  const size_t b = i0 + j0 * nx + k0 * nxy;
+  c[0] = arr[b]; c[1] = arr[b + 1]; c[2] = arr[b + 1 + nx]; c[3] = arr[b + nx];
+  c[4] = arr[b + nxy]; c[5] = arr[b + 1 + nxy]; c[6] = arr[b + 1 + nx + nxy];
+  c[7] = arr[b + nx + nxy];
+}
+
+// G3D.trilin (src/G3D.ml:97-157) of type ty's grid at p - origin = (px, py, pz), FP64
 // weights, corner order a..h; with GRAD also its gradient dg (A^-1).  ok = false (and
 // 0) when the cell lies outside the grid.
-template <bool GRAD>
-__device__ __forceinline__ double trilin(const GridDev& g, const float* arr, double px, double py,
+template <bool GRAD, bool CELLS = true>
+__device__ __forceinline__ double trilin(const GridDev& g, int ty, double px, double py,
                                          double pz, bool& ok, double* dg) {
   // src/G3D.ml:97-157: i0 = int_of_float (x * one_div_step) (truncation)
   const double fx = px * g.ods, fy = py * g.ods, fz = pz * g.ods;
@@ -57,11 +80,10 @@ __device__ __forceinline__ double trilin(const GridDev& g, const float* arr, dou
     return 0.0;
   }
   const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-  const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
-  const size_t b = i0 + j0 * nx + k0 * nxy;
-  const double A = arr[b], B = arr[b + 1], C = arr[b + 1 + nx], D = arr[b + nx];
-  const double E = arr[b + nxy], F = arr[b + 1 + nxy], G = arr[b + 1 + nx + nxy],
-               H = arr[b + nx + nxy];
+  float cv[8];
+  grid_corners<CELLS>(g, ty, i0, j0, k0, cv);
+  const double A = cv[0], B = cv[1], C = cv[2], D = cv[3];
+  const double E = cv[4], F = cv[5], G = cv[6], H = cv[7];
   const double wlx = (px - i0 * g.step) * g.ods, wly = (py - j0 * g.step) * g.ods,
                wlz = (pz - k0 * g.step) * g.ods;
   const double whx = 1.0 - wlx, why = 1.0 - wly, whz = 1.0 - wlz;

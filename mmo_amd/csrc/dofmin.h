@@ -410,10 +410,10 @@ __device__ double dof_eval(const DofEval& ev, const double* base, double x, doub
     for (int j = lane; j < L; j += kWave) {
       double px, py, pz;
       posed(j, px, py, pz);
-      const float* arr = ev.g.data + (size_t)ev.latoms[li.atom_base + j].type * ev.g.nvox;
       bool ok = true;
       double dg[3] = {0.0, 0.0, 0.0};
-      e += trilin<true>(ev.g, arr, px - ev.g.ox, py - ev.g.oy, pz - ev.g.oz, ok, dg);
+      e += trilin<true>(ev.g, ev.latoms[li.atom_base + j].type, px - ev.g.ox, py - ev.g.oy,
+                        pz - ev.g.oz, ok, dg);
       if (!ok) bad = 1;
       hx[j] = dg[0]; hy[j] = dg[1]; hz[j] = dg[2];
     }

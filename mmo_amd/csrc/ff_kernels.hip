@@ -503,6 +503,32 @@ k_scan_rows(InterArgs<Src> a, CellGrid cg) {
   }
 }
 
+// The cell-packed copy of the grids (GridDev::cells): one thread per (type, cell) gathers
+// the cell's 8 corners and writes them as two 16-byte stores.
+__global__ void __launch_bounds__(256) k_pack_cells(GridDev g, int ntypes, float* cells) {
+  const size_t cx = g.nx - 1, cy = g.ny - 1, cz = g.nz - 1;
+  const size_t ncell = cx * cy * cz;
+  const size_t n = ncell * ntypes;
+  for (size_t u = blockIdx.x * (size_t)blockDim.x + threadIdx.x; u < n;
+       u += (size_t)gridDim.x * blockDim.x) {
+    const size_t t = u / ncell, c = u - t * ncell;
+    const int k = (int)(c / (cx * cy)), j = (int)((c / cx) % cy), i = (int)(c % cx);
+    float v[8];
+    grid_corners<false>(g, (int)t, i, j, k, v);
+    float4* q = reinterpret_cast<float4*>(cells + 8 * u);
+    q[0] = make_float4(v[0], v[1], v[2], v[3]);
+    q[1] = make_float4(v[4], v[5], v[6], v[7]);
+  }
+}
+
+hipError_t launch_pack_cells(const GridDev& g, int ntypes, float* cells, hipStream_t s) {
+  const size_t n = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1) * ntypes;
+  if (n == 0) return hipSuccess;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_pack_cells, dim3(blocks), dim3(256), 0, s, g, ntypes, cells);
+  return hipGetLastError();
+}
+
 // ----------------------------------------------------- inter kernel, pose-culled
 // ligand atoms per pass of the per-atom sweep
 constexpr int kPoseSweepAtoms = 4;
@@ -755,7 +781,11 @@ struct InterpArgs {
   int grad_accumulate;
 };
 
-template <bool GRAD, class Src>
+// CELLS: read the cell-packed grids when there are any (batches: a pose's atoms across the
+// lanes); the scans' lanes hold one atom of 64 neighbouring poses, which share grid nodes,
+// and read the node arrays (the packed copy measured 15.1 against 10.4 ms on the
+// 36.9M-pose scan)
+template <bool GRAD, class Src, bool CELLS>
 __global__ void __launch_bounds__(256) k_interp(InterpArgs<Src> a) {
   const int lane = threadIdx.x & (kWave - 1);
   const int tidx = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -773,10 +803,9 @@ __global__ void __launch_bounds__(256) k_interp(InterpArgs<Src> a) {
       double x, y, z;
       a.src.pos(pose, pslot, j, li, a.latoms, a.lig_xyz, x, y, z);
       const int ty = a.type_override ? a.type_override[j] : a.latoms[li.atom_base + j].type;
-      const float* arr = a.g.data + (size_t)ty * a.g.nvox;
       bool ok = true;
       double dg[3] = {0.0, 0.0, 0.0};
-      e += trilin<GRAD>(a.g, arr, x - a.g.ox, y - a.g.oy, z - a.g.oz, ok, dg);
+      e += trilin<GRAD, CELLS>(a.g, ty, x - a.g.ox, y - a.g.oy, z - a.g.oz, ok, dg);
       if (!ok) bad = 1;
       if (GRAD) {
         double* gr = a.grad + 3 * (size_t)a.src.row(pose, j);
@@ -1004,8 +1033,8 @@ hipError_t launch_interp_coords(const InterpLaunch& L, const Tile* tiles, const 
   const int waves = 4;
   const int blocks = (ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
-  if (grad) hipLaunchKernelGGL((k_interp<true, CoordSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
-  else hipLaunchKernelGGL((k_interp<false, CoordSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  if (grad) hipLaunchKernelGGL((k_interp<true, CoordSrc, true>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  else hipLaunchKernelGGL((k_interp<false, CoordSrc, true>), dim3(blocks), dim3(waves * kWave), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1021,7 +1050,7 @@ hipError_t launch_interp_scan(const InterpLaunch& L, int lig, const double* rots
   const int waves = 4;
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_interp<false, ScanSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  hipLaunchKernelGGL((k_interp<false, ScanSrc, false>), dim3(blocks), dim3(waves * kWave), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1037,7 +1066,7 @@ hipError_t launch_interp_scan_list(const InterpLaunch& L, int lig, const double*
   const int waves = 4;
   const int blocks = (a.ntiles + waves - 1) / waves;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_interp<false, ScanListSrc>), dim3(blocks), dim3(waves * kWave), 0, s, a);
+  hipLaunchKernelGGL((k_interp<false, ScanListSrc, false>), dim3(blocks), dim3(waves * kWave), 0, s, a);
   return hipGetLastError();
 }
 

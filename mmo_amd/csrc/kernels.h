@@ -36,7 +36,12 @@ struct GridDev {
   size_t nvox;
   double step, ods;    // step and 1/step
   double ox, oy, oz;   // position of node (0,0,0)
+  // nullable: the same values by cell, the 8 corners of cell (i, j, k) (i < nx - 1, ...)
+  // contiguous in the trilinear corner order a..h, 32 B per cell and type:
+  // cells[((t * (nz-1) + k) * (ny-1) + j) * (nx-1) + i) * 8 + corner]
+  const float* cells;
 };
+hipError_t launch_pack_cells(const GridDev& g, int ntypes, float* cells, hipStream_t s);
 
 struct InterpLaunch {
   GridDev g;

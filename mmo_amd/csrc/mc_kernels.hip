@@ -127,18 +127,16 @@ __device__ double mc_inter_interp(const McArgs& a, const LigInfo& li, const Pose
       continue;
     }
     const int i0 = (int)fx, j0 = (int)fy, k0 = (int)fz;
-    const float* arr = g.data + (size_t)a.latoms[li.atom_base + j].type * g.nvox;
-    const size_t nx = g.nx, nxy = (size_t)g.nx * g.ny;
-    const size_t b = i0 + j0 * nx + k0 * nxy;
+    float cv[8];
+    grid_corners(g, a.latoms[li.atom_base + j].type, i0, j0, k0, cv);
     const double wlx = (px - i0 * g.step) * g.ods, wly = (py - j0 * g.step) * g.ods,
                  wlz = (pz - k0 * g.step) * g.ods;
     const double whx = 1.0 - wlx, why = 1.0 - wly, whz = 1.0 - wlz;
     // corner order of G3D.trilin, src/G3D.ml:144-151
-    e += (double)arr[b] * (whx * why * whz) + (double)arr[b + 1] * (wlx * why * whz) +
-         (double)arr[b + 1 + nx] * (wlx * wly * whz) + (double)arr[b + nx] * (whx * wly * whz) +
-         (double)arr[b + nxy] * (whx * why * wlz) + (double)arr[b + 1 + nxy] * (wlx * why * wlz) +
-         (double)arr[b + 1 + nx + nxy] * (wlx * wly * wlz) +
-         (double)arr[b + nx + nxy] * (whx * wly * wlz);
+    e += (double)cv[0] * (whx * why * whz) + (double)cv[1] * (wlx * why * whz) +
+         (double)cv[2] * (wlx * wly * whz) + (double)cv[3] * (whx * wly * whz) +
+         (double)cv[4] * (whx * why * wlz) + (double)cv[5] * (wlx * why * wlz) +
+         (double)cv[6] * (wlx * wly * wlz) + (double)cv[7] * (whx * wly * wlz);
   }
   e = wave_sum(e);
   for (int off = 1; off < kWave; off <<= 1) bad |= __shfl_xor(bad, off, kWave);

@@ -1344,3 +1344,56 @@ def test_mol2_ligands_score_like_the_oracle(sys0, tmp_path):
         refs.append(oracle.ene_inter(rec, m) + oracle.ene_intra(m))
         sums.append(inter_abs_sum(rec, xyz, m) + intra_abs_sum(m, xyz))
     assert_energy(e[:, 0] + e[:, 1], refs, sums, what="mol2 ligands")
+
+
+def test_cell_packed_grids_bit_identical():
+    """The cell-packed copy of the grids (8 corners per cell, two 16-byte loads per lookup)
+    holds the node arrays' values: grid-form Monte Carlo chains, with and without the
+    minimiser, and batched interpolated energies and gradients are bit for bit those of
+    the node arrays."""
+    import torch
+    mcmod, ctx, ligs, ids, rec, chain_lig, rot0, pos0 = _mc_setup(seed=31)
+    types, probe_a, probe_q = [], [], []
+    for l in ligs:
+        t = []
+        for a, q in zip(l["anum"], l["q"]):
+            probe_a.append(int(a))
+            probe_q.append(float(q))
+            t.append(len(probe_a) - 1)
+        types.append(np.array(t))
+    origin, dims, step = synth.grid_spec(center=SURFACE, n=64, step=0.5)
+    ids2 = [ctx.add_ligand(l, types=t) for l, t in zip(ligs, types)]
+    cl = np.array([ids2[ids.index(l)] for l in chain_lig])
+    roi = (SURFACE, 3.0)
+    rng = np.random.default_rng(2)
+    pose_lig = np.repeat(ids2, 2)
+    xyz = []
+    for l in pose_lig:
+        lig = ligs[ids2.index(l)]
+        _, x = _place(lig, synth.random_rotations(rng, 1)[0], SURFACE + rng.uniform(-1, 1, 3))
+        xyz.append(x)
+    xyz = np.concatenate(xyz)
+    out = {}
+    for pack in (1, 0):
+        ctx.tune(grid_pack=pack)
+        ctx.compute_grids(probe_a, probe_q, step, dims, origin, copy_out=False)
+        print("grid_pack", pack, flush=True)
+        res = []
+        for iters in (0, 8):
+            m = mcmod.MonteCarlo(ctx, cl, rot0, pos0, ff.INTERP, total_frames=40,
+                                 flags=mcmod.TWEAK_RBONDS | mcmod.INTRA_UFF | mcmod.HARD_ROI,
+                                 roi_center=roi[0], roi_r=roi[1], seed=8)
+            if iters:
+                m.set_minimizer(iters, history=8, max_step=0.2)
+            m.run(40)
+            st = m.state()
+            res.append((st["best_e"].copy(), st["accepted"].copy(), st["e"].copy()))
+        b = ctx.batch(pose_lig)
+        e, g = b.score(xyz, ff.F_INTER_INTERP | ff.F_GRAD)
+        res.append((e, g))
+        out[pack] = res
+    ctx.tune(grid_pack=1)
+    for a, b in zip(out[1], out[0]):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v, equal_nan=True)
+    torch.cuda.synchronize()
