@@ -1397,3 +1397,35 @@ def test_cell_packed_grids_bit_identical():
         for u, v in zip(a, b):
             assert np.array_equal(u, v, equal_nan=True)
     torch.cuda.synchronize()
+
+
+def test_row_list_scan_grids_agree():
+    """The row-list scan under several scan grids (1.5 A rows: several passes of 64
+    candidate rows per atom; 6 A rows; x cuts 1, 2, 16) and both receptor paths gives the
+    energies of the default grid to FP32 rounding, and the oracle's on a sample: a row
+    grid only changes which pairs beyond the cutoff are visited."""
+    import torch
+    lig, rec = synth.config0()
+    ctx = ff.Context(0)
+    ctx.set_receptor(rec)
+    lid = ctx.add_ligand(lig)
+    rots = synth.so3_rotations(4608)[::288]
+    trans = synth.lattice((0.0, 0.0, 0.0), 20, 0.5)
+    ctx.tune(scan_cull=5)
+    ref = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+    rng = np.random.default_rng(11)
+    sample = rng.choice(len(ref), 64, replace=False)
+    pr, pt = (sample // len(trans)).astype(np.int32), (sample % len(trans)).astype(np.int32)
+    want = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=8)
+    sums = [inter_abs_sum(rec, _place(lig, rots[a], trans[b])[1], lig, True) for a, b in zip(pr, pt)]
+    for cell, xsub, chunk in ((1.5, 1, 4), (6.0, 2, 8), (3.5, 16, 4)):
+        ctx.tune(scan_cell=cell, scan_xsub=xsub, scan_chunk=chunk)
+        ctx.set_receptor(rec)
+        for sc in (5, 6):
+            ctx.tune(scan_cull=sc)
+            print("row grid", cell, xsub, chunk, sc, flush=True)
+            e = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+            rel = np.abs(e - ref) / np.maximum(np.abs(ref), 1e-3)
+            assert rel.max() < 1e-6, (cell, xsub, sc, rel.max())
+            assert_energy(e[sample], want, sums, what="row grid %g/%d" % (cell, xsub))
+    torch.cuda.synchronize()
