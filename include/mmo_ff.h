@@ -405,7 +405,8 @@ int mmo_fixed_point_bits(const mmo_ctx* ctx);
  * in LDS; 5 row lists of the scan grid through the scalar cache, 6 the same with the
  * receptor in LDS), "scan_cell" and "scan_xsub" (the row lists' cell edge in y and z and
  * its x cut; next mmo_set_receptor), "scan_chunk" (receptor atoms the row lists fetch at
- * once: 4 or 8), "cell_size" (applies to the next mmo_set_receptor), "mc_waves_per_cu",
+ * once: 4 or 8), "row_filter" (1: the row lists test each far row's atoms against the
+ * wavefront's box grown by the cutoff and sweep only those within it; same energies), "cell_size" (applies to the next mmo_set_receptor), "mc_waves_per_cu",
  * "pose_waves" (batches: wavefronts per workgroup), "pose_cull" (1: batches in
  * the shifted form run one wavefront per pose with a culled receptor sweep; 0: tiles of
  * poses over the whole receptor), "pose_sweep" (that sweep: 1 lanes over the receptor

@@ -138,6 +138,7 @@ struct mmo_ctx {
   float scan_cell = 3.5f;
   int scan_xsub = 8;
   int scan_chunk = 4;
+  int row_filter = 1;
   CellGrid cgs{};
   DevBuf<int> d_cells;
   DevBuf<RecPos> rpos_s;
@@ -452,6 +453,7 @@ InterLaunch inter_launch(mmo_ctx* c, int mode, const LigInfo* ligs, const LigAto
   L.cull_waves = c->cull_waves;
   L.cull_k = c->cull_k;
   L.row_chunk = c->scan_chunk;
+  L.row_filter = c->row_filter;
   L.atom_order = nullptr;
   L.cg = c->cg;
   L.fr = c->fr;
@@ -551,6 +553,7 @@ InterLaunch scan_launch(mmo_ctx* c, int mode, int scan_cull) {
     L.cg = c->cgs;
     L.rpos = c->rpos_s.p; L.rab = c->rab_s.p; L.rres = c->rres_s.p;
     L.row_chunk = c->scan_chunk;
+    L.row_filter = c->row_filter;
     L.atom_order = c->d_sweep.p;
     return L;
   }
@@ -2590,6 +2593,7 @@ int mmo_tune_set(mmo_ctx* c, const char* key, double v) {
   else if (k == "scan_cell" && v >= 1.0 && v <= 24.0) c->scan_cell = (float)v;
   else if (k == "scan_xsub" && iv >= 1 && iv <= 16) c->scan_xsub = iv;
   else if (k == "scan_chunk" && (iv == 4 || iv == 8)) c->scan_chunk = iv;
+  else if (k == "row_filter" && (iv == 0 || iv == 1)) c->row_filter = iv;
   else if (k == "grid_pack" && (iv == 0 || iv == 1)) c->grid_pack = iv;
   else if (k == "mc_waves_per_cu" && iv >= 1 && iv <= 32) c->mc_waves_per_cu = iv;
   else if (k == "mc_wg_waves" && iv >= 1 && iv <= 16) c->mc_wg_waves = iv;
@@ -2618,6 +2622,7 @@ double mmo_tune_get(mmo_ctx* c, const char* key) {
   if (k == "scan_cell") return c->scan_cell;
   if (k == "scan_xsub") return c->scan_xsub;
   if (k == "scan_chunk") return c->scan_chunk;
+  if (k == "row_filter") return c->row_filter;
   if (k == "grid_pack") return c->grid_pack;
   if (k == "pose_waves") return c->pose_waves;
   if (k == "scan_cull") return c->scan_cull;

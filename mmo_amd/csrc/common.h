@@ -224,6 +224,13 @@ __device__ __forceinline__ LigPt lig_pt(double x, double y, double z, const Fram
   return l;
 }
 __device__ __forceinline__ LigPt lig_pt_zero() { return LigPt{0, 0, 0, 0.0f, 0.0f, 0.0f}; }
+// an inert lane's atom for the sweeps that skip the 0.01 A clamp on far pairs: at the
+// saturation limit on every axis, beyond the cutoff of every receptor atom (the frame
+// covers the receptor plus 16 A), so its pairs have finite r and weight exactly 0
+__device__ __forceinline__ LigPt lig_pt_far() {
+  constexpr int f = (1 << 30) - 1;
+  return LigPt{f, f, f, 0.0f, 0.0f, 0.0f};
+}
 
 // r^2 (A^2) of receptor atom i and ligand atom l from fixed-point differences d; for
 // pairs closer than sqrt(near2) the residuals of both atoms are added (a wavefront

@@ -7,8 +7,9 @@
 namespace mmo {
 
 // --------------------------------------------------------------- pair arithmetic
-// NEAR = false: the caller knows the pair is at least 0.5 A apart (the residual check is
-// skipped, which keeps the loop branch-free)
+// NEAR = false: the caller knows the pair is at least 0.5 A apart for every lane whose
+// sums it keeps, and that an inert lane's atom lies beyond the cutoff (lig_pt_far): the
+// residual check and the 0.01 A clamp are skipped, which keeps the loop branch-free
 // shift t = 1 - r^2/144 in FP64 from the fixed-point differences and both atoms'
 // quantisation residuals.  The FP32 r^2 carries ~1e-7 relative error, so the FP32 t has
 // an absolute error of ~1e-7 and the weight w = t^2 a relative error of ~2e-7 / t:
@@ -32,7 +33,7 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, co
                                             float& s6) {
   const float dx = (float)(p.x - l.x), dy = (float)(p.y - l.y), dz = (float)(p.z - l.z);
   const float r2 = NEAR ? pair_r2(dx, dy, dz, l, i, pk) : (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
-  const float r2c = max_r2(r2, pk.min_r2);
+  const float r2c = NEAR ? max_r2(r2, pk.min_r2) : r2;
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = inv_r2(ir, r2c);
   const float u3 = u * (u * u);
@@ -216,7 +217,13 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
 // the box it was swept with.  fold(sq, s12, s6) receives the K slab sums.
 // SMEM: the receptor arrays are read with wavefront-uniform addresses from global memory
 // (scalar loads into SGPRs); else from LDS into VGPRs.
-template <bool SMEM, int CHUNK, int K, class Fold>
+// FILT: before the sweep, the lane of each far candidate row tests the row's first 64
+// atoms against the box grown by 12.1 A (one gathered position per atom) and keeps a
+// bitmask; the sweep then visits only the kept atoms (chunks start at the next kept
+// atom).  A dropped atom is more than 12.09 A from every lane's atom, so its pairs would
+// add exact zeros: the sums keep their bits and their order.  The row chords alone visit
+// about 2x the pairs inside the cutoff; the per-atom test leaves about 1.3x.
+template <bool SMEM, int CHUNK, int K, bool FILT, class Fold>
 __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __restrict__ s_pos,
                                           const RecAB* __restrict__ s_ab,
                                           const RecRes* __restrict__ rres, float inv_s,
@@ -266,6 +273,31 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
       // any lane's atom: its pairs skip the near-pair residual check
       tag = 2 * cz + ((dz < 0.55f && dy < 0.55f) ? 1 : 0);
     }
+    uint32_t keep_lo = ~0u, keep_hi = ~0u;   // FILT: kept atoms of this lane's far row
+    if constexpr (FILT) {
+      constexpr float RF2 = 12.1f * 12.1f;
+      const int flen = (a1 > a0 && !(tag & 1)) ? min(a1 - a0, 64) : 0;
+      const int maxlen = (int)wave_maxf((float)flen);
+      uint64_t m = 0ull;
+      for (int it = 0; it < maxlen; it += 4) {
+        // four gathers in flight (a lane past its row reads atom 0, and drops it)
+        int px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const RecPos p = s_pos[it + u < flen ? a0 + it + u : 0];
+          px[u] = p.x; py[u] = p.y; pz[u] = p.z;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float ex = fmaxf(fabsf(fmaf((float)px[u], inv_s, -qx)) - hx, 0.0f);
+          const float ey = fmaxf(fabsf(fmaf((float)py[u], inv_s, -qy)) - hy, 0.0f);
+          const float ez = fmaxf(fabsf(fmaf((float)pz[u], inv_s, -qz)) - hz, 0.0f);
+          const bool in = it + u < flen && fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= RF2;
+          m |= (uint64_t)in << (it + u);
+        }
+      }
+      if (flen > 0) { keep_lo = (uint32_t)m; keep_hi = (uint32_t)(m >> 32); }
+    }
     uint64_t rows = __ballot(a1 > a0);
     while (rows) {
       const int r = __builtin_ctzll(rows);
@@ -290,7 +322,43 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
       } else {
         // CHUNK (<= kRowChunk) atoms fetched together (one memory latency per chunk of a
         // short row; the arrays are padded so that a chunk may run past the last atom)
-        for (int c0 = ra0; c0 < ra1; c0 += CHUNK) {
+        int tail0 = ra0;   // first atom swept without the filter
+        if constexpr (FILT) {
+          uint64_t m = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)keep_hi, r) << 32) |
+                       (uint32_t)__builtin_amdgcn_readlane((int)keep_lo, r);
+          while (m) {
+            const int k0 = __builtin_ctzll(m);
+            const int c0 = ra0 + k0;
+            RecPos pp[CHUNK];
+            RecAB aa[CHUNK];
+#pragma unroll
+            for (int k = 0; k < CHUNK; ++k) {
+              pp[k] = s_pos[c0 + k];
+              aa[k] = s_ab[c0 + k];
+            }
+#pragma unroll
+            for (int k = 0; k < CHUNK; ++k) {
+              if constexpr (SMEM)
+                asm volatile("" ::"s"(pp[k].x), "s"(pp[k].y), "s"(pp[k].z), "s"(pp[k].q),
+                             "s"(aa[k].a), "s"(aa[k].b));
+              else
+                asm volatile("" ::"v"(pp[k].x), "v"(pp[k].y), "v"(pp[k].z), "v"(pp[k].q),
+                             "v"(aa[k].a), "v"(aa[k].b));
+            }
+            const uint32_t bits = (uint32_t)(m >> k0);   // bit 0 (atom c0) is set
+#pragma unroll
+            for (int k = 0; k < CHUNK; ++k)
+              if (k == 0 || ((bits >> k) & 1u)) {
+#pragma unroll
+                for (int mm = 0; mm < K; ++mm)
+                  pair_energy<MMO_SHIFTED, false>(pp[k], aa[k], l[mm], c0 + k, pk, sq[mm], s12[mm],
+                                                  s6[mm]);
+              }
+            m &= ~(((1ull << CHUNK) - 1ull) << k0);   // the bits below k0 are clear
+          }
+          tail0 = ra0 + 64;   // atoms past the first 64 of a long row
+        }
+        for (int c0 = tail0; c0 < ra1; c0 += CHUNK) {
           RecPos pp[CHUNK];
           RecAB aa[CHUNK];
 #pragma unroll
@@ -605,7 +673,8 @@ __device__ __forceinline__ void pose_sweep(const CellGrid& cg, const int* __rest
 // wl: kAtomSweepInts ints of this wavefront's LDS.  atom(j, l, cq, ca, cb) gives
 // atom j (LigPt); with PRELOAD it is called by lane j % 64 for its chunk's atoms (so it
 // may depend on the lane's j), else by all lanes for one j at a time.  NEAR = false
-// skips the residual correction of pairs closer than 0.5 A (energy form only); with
+// (energy form only) is for callers whose pairs are all at least 0.5 A apart: it skips
+// the residual correction and the 0.01 A clamp (pair_energy); with
 // GRAD, gout(j, gx, gy, gz) receives dE/dx_j (A^-1, FP64) from the lane holding atom j
 // (PRELOAD) or from lane 0.  Sums: FP32 per lane and atom, FP64 across atoms.
 constexpr int kAtomSweepInts = 3 * kWave;

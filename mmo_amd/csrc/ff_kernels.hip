@@ -394,7 +394,7 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
         const LigAtom la = a.latoms[li.atom_base + j];
         cq[k] = la.keq; ca[k] = la.a; cb[k] = la.b;
       } else {   // inert: zero parameters
-        l[k] = lig_pt_zero();
+        l[k] = lig_pt_far();
         cq[k] = ca[k] = cb[k] = 0.0;
       }
     }
@@ -425,17 +425,18 @@ k_inter_cull(InterArgs<Src> a, CellGrid cg) {
 
 // Rigid scan over the row lists of a finer cell grid (row_sweep): one pose per lane, K
 // ligand atoms at a time (consecutive atoms of the ligand's sweep order, a walk of its
-// bond graph, so the K atoms are close and the wavefront's box grows little).  RMODE 0:
-// receptor read in place (uniform addresses, scalar cache); 1: receptor positions and
-// LJ factors staged in LDS.  The cell table stays in global memory (it is read per lane,
-// once per candidate row).
+// bond graph, so the K atoms are close and the wavefront's box grows little).  RMODE bit
+// 0 clear: receptor read in place (uniform addresses, scalar cache); set: receptor
+// positions and LJ factors staged in LDS.  Bit 1: the far rows' atoms are tested against
+// the box grown by the cutoff before the sweep (row_sweep's FILT).  The cell table stays
+// in global memory (it is read per lane, once per candidate row).
 template <int RMODE, int CHUNK, int K, class Src>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 k_scan_rows(InterArgs<Src> a, CellGrid cg) {
   extern __shared__ __align__(16) char smem[];
   const RecPos* s_pos = a.rpos;
   const RecAB* s_ab = a.rab;
-  if constexpr (RMODE == 1) {
+  if constexpr ((RMODE & 1) == 1) {
     // with the padding atoms (the row sweep's chunks may read past the last atom)
     const int nr = a.nrec + kRowChunk;
     RecPos* p = reinterpret_cast<RecPos*>(smem);
@@ -466,7 +467,7 @@ k_scan_rows(InterArgs<Src> a, CellGrid cg) {
     for (int k = 0; k < K; ++k) {
       valid[k] = pose_ok && j0 + k < li.n_atoms;
       cq[k] = ca[k] = cb[k] = 0.0;
-      l[k] = lig_pt_zero();
+      l[k] = lig_pt_far();
       if (j0 + k < li.n_atoms) {
         const int j = (K > 1 && a.atom_order) ? a.atom_order[li.atom_base + j0 + k] : j0 + k;
         const LigAtom la = a.latoms[li.atom_base + j];
@@ -480,7 +481,7 @@ k_scan_rows(InterArgs<Src> a, CellGrid cg) {
     }
     float qx, qy, qz, hx, hy, hz;
     if (!wave_box<K>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) continue;
-    row_sweep<RMODE == 0, CHUNK, K>(cg, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx,
+    row_sweep<(RMODE & 1) == 0, CHUNK, K, (RMODE & 2) != 0>(cg, s_pos, s_ab, a.rres, fr.inv_s, fr.inv_s2, qx, qy, qz, hx,
                                     hy, hz, l,
                                     [&](const float (&sq)[K], const float (&s12)[K],
                                         const float (&s6)[K]) {
@@ -659,7 +660,7 @@ __global__ void __launch_bounds__(1024) k_grid(GridArgs a, CellGrid cg) {
   const Frame fr = a.fr;
   int sat = 0;
   LigPt l[1];
-  l[0] = valid[0] ? lig_pt(x, y, z, fr, &sat) : lig_pt_zero();
+  l[0] = valid[0] ? lig_pt(x, y, z, fr, &sat) : lig_pt_far();
   double Sq = 0.0, S12 = 0.0, S6 = 0.0;
   float qx, qy, qz, hx, hy, hz;
   if (wave_box<1>(l, valid, fr.inv_s, qx, qy, qz, hx, hy, hz)) {
@@ -926,10 +927,15 @@ static hipError_t launch_rows(const InterLaunch& L, const InterArgs<Src>& a, hip
   }
     // one ligand atom per lane: two (bonded neighbours of the sweep order) were measured
     // 15-20% slower on the 36.9M-pose scan, whichever receptor path
-    if (L.cull == 5) {
-      if (L.row_chunk == 4) MMO_ROWS(1, 4, 1) else MMO_ROWS(1, 8, 1)
-    } else {
+    const int rm = (L.cull == 5 ? 1 : 0) | (L.row_filter ? 2 : 0);
+    if (rm == 0) {
       if (L.row_chunk == 4) MMO_ROWS(0, 4, 1) else MMO_ROWS(0, 8, 1)
+    } else if (rm == 1) {
+      if (L.row_chunk == 4) MMO_ROWS(1, 4, 1) else MMO_ROWS(1, 8, 1)
+    } else if (rm == 2) {
+      if (L.row_chunk == 4) MMO_ROWS(2, 4, 1) else MMO_ROWS(2, 8, 1)
+    } else {
+      if (L.row_chunk == 4) MMO_ROWS(3, 4, 1) else MMO_ROWS(3, 8, 1)
     }
 #undef MMO_ROWS
     return hipGetLastError();

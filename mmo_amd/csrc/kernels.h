@@ -18,6 +18,7 @@ struct InterLaunch {
   int cull_waves;      // wavefronts per block of the culled kernel
   int cull_k;          // ligand atoms per lane in the culled kernel (1, 2, 4)
   int row_chunk;       // k_scan_rows: receptor atoms fetched at once (4 or 8)
+  int row_filter;      // k_scan_rows: far rows' atoms tested against the box first (0 / 1)
   const int* atom_order;   // nullable: per ligand at atom_base, the sweeps' atom order
   CellGrid cg;
   Frame fr;

@@ -89,13 +89,13 @@ def _time(fn, reps):
 
 def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0), ks=(1,),
               waves=(4, 8, 16), scan_culls=(1, 2, 3),
-              row_grids=((3.0, 8), (3.5, 8), (4.0, 8), (3.5, 4)), row_chunks=(4, 8),
-              row_waves=(4, 8), reps=2, store=True):
+              row_grids=((3.5, 8), (4.0, 8), (5.0, 8), (6.0, 8), (5.0, 4)), row_chunks=(4, 8),
+              row_waves=(4, 8), row_filters=(1,), reps=2, store=True):
     """Culled rigid scan: sweeps cell size x receptor path (LDS or scalar cache) x atoms
     per lane x wavefronts per workgroup of the cell-sweep kernel, and row-list grid (cell
     edge in y and z, x cut) x receptor atoms fetched at once x wavefronts per workgroup of
-    the row-list kernel (scan_cull 5), on a slice of the caller's scan (rots, trans), and
-    leaves the best setting applied.  The pose sweeps' x cut (cell_xsub) stays as set: it
+    the row-list kernel (scan_cull 5) x its far-row atom test (row_filter), on a slice of
+    the caller's scan (rots, trans), and leaves the best setting applied.  The pose sweeps' x cut (cell_xsub) stays as set: it
     is the Monte Carlo's choice."""
     from . import ff
     import torch
@@ -125,10 +125,10 @@ def tune_scan(ctx, rec, lig_id, rots, trans, cell_sizes=(4.0, 5.0), ks=(1,),
     for (cell, xsub) in row_grids:
         ctx.tune(scan_cell=cell, scan_xsub=xsub)
         ctx.set_receptor(rec)
-        for ch, w in itertools.product(row_chunks, row_waves):
-            ctx.tune(scan_cull=5, scan_chunk=ch, cull_waves=w, cull_k=1)
+        for ch, w, f in itertools.product(row_chunks, row_waves, row_filters):
+            ctx.tune(scan_cull=5, scan_chunk=ch, cull_waves=w, cull_k=1, row_filter=f)
             measure({"cell_size": keep_cell, "scan_cull": 5, "cull_k": 1, "cull_waves": w,
-                     "scan_cell": cell, "scan_xsub": xsub, "scan_chunk": ch})
+                     "scan_cell": cell, "scan_xsub": xsub, "scan_chunk": ch, "row_filter": f})
     good = [r for r in results if r["ok"]]
     best = min(good, key=lambda r: r["ms"])
     chosen = {k: v for k, v in best.items() if k not in ("ms", "ok")}

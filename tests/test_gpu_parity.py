@@ -1403,7 +1403,9 @@ def test_row_list_scan_grids_agree():
     """The row-list scan under several scan grids (1.5 A rows: several passes of 64
     candidate rows per atom; 6 A rows; x cuts 1, 2, 16) and both receptor paths gives the
     energies of the default grid to FP32 rounding, and the oracle's on a sample: a row
-    grid only changes which pairs beyond the cutoff are visited."""
+    grid only changes which pairs beyond the cutoff are visited.  With and without the
+    per-atom box test of the far rows (row_filter) the energies are bit-identical: the
+    atoms it drops would add exact zeros."""
     import torch
     lig, rec = synth.config0()
     ctx = ff.Context(0)
@@ -1428,4 +1430,8 @@ def test_row_list_scan_grids_agree():
             rel = np.abs(e - ref) / np.maximum(np.abs(ref), 1e-3)
             assert rel.max() < 1e-6, (cell, xsub, sc, rel.max())
             assert_energy(e[sample], want, sums, what="row grid %g/%d" % (cell, xsub))
+            ctx.tune(row_filter=0)
+            e0 = ctx.scan(lid, ff.SHIFTED, rots, trans).cpu().numpy()
+            ctx.tune(row_filter=1)
+            assert np.array_equal(e0, e), (cell, xsub, sc, np.abs(e0 - e).max())
     torch.cuda.synchronize()
