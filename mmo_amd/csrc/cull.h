@@ -219,8 +219,9 @@ __device__ __forceinline__ void cull_sweep(const CellGrid& cg, const int* __rest
 // (scalar loads into SGPRs); else from LDS into VGPRs.
 // FILT: before the sweep, the lane of each far candidate row tests the row's first 64
 // atoms against the box grown by 12.1 A (one gathered position per atom) and keeps a
-// bitmask; the sweep then visits only the kept atoms (chunks start at the next kept
-// atom).  A dropped atom is more than 12.09 A from every lane's atom, so its pairs would
+// bitmask; the atoms of the x cells inside the chord at the row's farthest (y, z) point
+// are kept untested.  The sweep then visits only the kept atoms (chunks start at the
+// next kept atom).  A dropped atom is more than 12.09 A from every lane's atom, so its pairs would
 // add exact zeros: the sums keep their bits and their order.  The row chords alone visit
 // about 2x the pairs inside the cutoff; the per-atom test leaves about 1.3x.
 template <bool SMEM, int CHUNK, int K, bool FILT, class Fold>
@@ -250,6 +251,7 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
   for (int base = 0; base < nc; base += kWave) {
     const int cidx = base + lane;
     int a0 = 0, a1 = 0, tag = 0;
+    int b0 = 0, b1 = 0;   // FILT: atoms of the row's inner x cells, kept without a test
     if (cidx < nc) {
       int dzi = (int)(((float)cidx + 0.5f) * inv_ncy);
       dzi -= dzi * ncy > cidx ? 1 : 0;
@@ -267,6 +269,23 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
           const int rowb = (cz * cg.ny + cy) * cg.nx;
           a0 = cg.cell_start[rowb + cx0];
           a1 = cg.cell_start[rowb + cx1 + 1];
+          b0 = b1 = a0;
+          if constexpr (FILT) {
+            // the chord at the row's farthest (y, z) point: x cells within it hold only
+            // atoms within 12.05 A of the box (keeping an atom beyond only visits zeros)
+            const float dzf = fmaxf(0.0f, fmaxf((qz - hz) - zlo, (zlo + cg.c) - (qz + hz)));
+            const float dyf = fmaxf(0.0f, fmaxf((qy - hy) - ylo, (ylo + cg.c) - (qy + hy)));
+            const float ri2 = R2 - dzf * dzf - dyf * dyf;
+            if (ri2 > 0.0f) {
+              const float ri = hx + bound_sqrt(ri2);
+              const int ix0 = max(cx0, (int)ceilf((qx - ri - cg.x0) * cg.inv_cx));
+              const int ix1 = min(cx1, (int)floorf((qx + ri - cg.x0) * cg.inv_cx) - 1);
+              if (ix0 <= ix1) {
+                b0 = cg.cell_start[rowb + ix0];
+                b1 = cg.cell_start[rowb + ix1 + 1];
+              }
+            }
+          }
         }
       }
       // a row 0.5 A or more from the box in y or z holds no atom closer than 0.5 A to
@@ -276,15 +295,22 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
     uint32_t keep_lo = ~0u, keep_hi = ~0u;   // FILT: kept atoms of this lane's far row
     if constexpr (FILT) {
       constexpr float RF2 = 12.1f * 12.1f;
-      const int flen = (a1 > a0 && !(tag & 1)) ? min(a1 - a0, 64) : 0;
+      // bits (atom - a0) of the first 64 atoms: [0, n0) and [n1, n2) are tested, the
+      // inner cells' atoms [n0, n1) kept
+      const bool far = a1 > a0 && !(tag & 1);
+      const int n2 = min(a1 - a0, 64);
+      const int n0 = min(b0 - a0, n2), n1 = min(b1 - a0, n2);
+      const int flen = far ? n0 + (n2 - n1) : 0;
       const int maxlen = (int)wave_maxf((float)flen);
-      uint64_t m = 0ull;
+      const uint64_t lo_n1 = n1 >= 64 ? ~0ull : ((1ull << n1) - 1ull);
+      uint64_t m = n1 > n0 ? (lo_n1 & ~((1ull << n0) - 1ull)) : 0ull;   // then n0 < 64
       for (int it = 0; it < maxlen; it += 4) {
-        // four gathers in flight (a lane past its row reads atom 0, and drops it)
-        int px[4], py[4], pz[4];
+        // four gathers in flight (a lane past its tested atoms reads atom 0, and drops it)
+        int px[4], py[4], pz[4], bit[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const RecPos p = s_pos[it + u < flen ? a0 + it + u : 0];
+          bit[u] = it + u < n0 ? it + u : n1 + (it + u - n0);
+          const RecPos p = s_pos[it + u < flen ? a0 + bit[u] : 0];
           px[u] = p.x; py[u] = p.y; pz[u] = p.z;
         }
 #pragma unroll
@@ -293,10 +319,10 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
           const float ey = fmaxf(fabsf(fmaf((float)py[u], inv_s, -qy)) - hy, 0.0f);
           const float ez = fmaxf(fabsf(fmaf((float)pz[u], inv_s, -qz)) - hz, 0.0f);
           const bool in = it + u < flen && fmaf(ex, ex, fmaf(ey, ey, ez * ez)) <= RF2;
-          m |= (uint64_t)in << (it + u);
+          m |= (uint64_t)in << (bit[u] & 63);
         }
       }
-      if (flen > 0) { keep_lo = (uint32_t)m; keep_hi = (uint32_t)(m >> 32); }
+      if (far) { keep_lo = (uint32_t)m; keep_hi = (uint32_t)(m >> 32); }
     }
     uint64_t rows = __ballot(a1 > a0);
     while (rows) {
