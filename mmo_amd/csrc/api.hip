@@ -135,7 +135,7 @@ struct mmo_ctx {
   DevBuf<int> d_cellp;
   // rigid scan row lists (scan_cull 5, 6): cells of scan_cell A in y and z, cut scan_xsub
   // times along x, with a copy of the receptor sorted by that grid
-  float scan_cell = 3.5f;
+  float scan_cell = 5.0f;
   int scan_xsub = 8;
   int scan_chunk = 4;
   int row_filter = 1;
