@@ -304,17 +304,18 @@ __device__ __forceinline__ void row_sweep(const CellGrid& cg, const RecPos* __re
       const int maxlen = (int)wave_maxf((float)flen);
       const uint64_t lo_n1 = n1 >= 64 ? ~0ull : ((1ull << n1) - 1ull);
       uint64_t m = n1 > n0 ? (lo_n1 & ~((1ull << n0) - 1ull)) : 0ull;   // then n0 < 64
-      for (int it = 0; it < maxlen; it += 4) {
-        // four gathers in flight (a lane past its tested atoms reads atom 0, and drops it)
-        int px[4], py[4], pz[4], bit[4];
+      constexpr int U = 2;   // gathers in flight (more would spill at 64 VGPRs)
+      for (int it = 0; it < maxlen; it += U) {
+        // a lane past its tested atoms reads atom 0, and drops it
+        int px[U], py[U], pz[U], bit[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
           bit[u] = it + u < n0 ? it + u : n1 + (it + u - n0);
           const RecPos p = s_pos[it + u < flen ? a0 + bit[u] : 0];
           px[u] = p.x; py[u] = p.y; pz[u] = p.z;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
           const float ex = fmaxf(fabsf(fmaf((float)px[u], inv_s, -qx)) - hx, 0.0f);
           const float ey = fmaxf(fabsf(fmaf((float)py[u], inv_s, -qy)) - hy, 0.0f);
           const float ez = fmaxf(fabsf(fmaf((float)pz[u], inv_s, -qz)) - hz, 0.0f);
