@@ -172,22 +172,36 @@ __device__ __forceinline__ double intra_pairs_wave(const uint32_t* pij, const do
                                                    double* gz, bool grad) {
   const int lane = threadIdx.x & (kWave - 1);
   double e = 0.0;
-  for (int p = lane; p < n; p += kWave) {
-    const uint32_t ij = pij[p];
-    const double4 pp = ppar[p];   // ke qi qj, Ai Aj, 2 Bi Bj
-    const int i = ij & 0xffff, j = ij >> 16;
-    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const bool apart = r2 >= 1.0e-4;              // r >= 0.01 (Math.non_zero_dist)
-    const double ir = apart ? rsqrt(r2) : 100.0;  // 1 / max(r, 0.01)
-    const double u = ir * ir;
-    const double u3 = u * u * u;
-    const double u6 = u3 * u3;
-    e += pp.x * ir + pp.y * u6 - pp.z * u3;
-    if (grad && apart) {
-      const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
-      atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
-      atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
+  // four pairs per lane per pass, their loads issued together (one memory latency per
+  // four pairs); each lane still takes its pairs lane, lane + 64, ... in that order
+  for (int p0 = lane; p0 < n; p0 += 4 * kWave) {
+    uint32_t ijs[4];
+    double4 pps[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int p = p0 + q * kWave;
+      ijs[q] = p < n ? pij[p] : 0u;
+      pps[q] = p < n ? ppar[p] : make_double4(0.0, 0.0, 0.0, 0.0);   // ke qi qj, Ai Aj, 2 Bi Bj
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (p0 + q * kWave >= n) break;
+      const double4 pp = pps[q];
+      const int i = ijs[q] & 0xffff, j = ijs[q] >> 16;
+      const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool apart = r2 >= 1.0e-4;              // r >= 0.01 (Math.non_zero_dist)
+      const double irr = rsqrt(fmax(r2, 1.0e-4));   // no branch around the refinement
+      const double ir = apart ? irr : 100.0;        // 1 / max(r, 0.01)
+      const double u = ir * ir;
+      const double u3 = u * u * u;
+      const double u6 = u3 * u3;
+      e += pp.x * ir + pp.y * u6 - pp.z * u3;
+      if (grad && apart) {
+        const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
+        atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
+        atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
+      }
     }
   }
   return wave_sum(e);

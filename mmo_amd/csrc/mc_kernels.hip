@@ -211,16 +211,32 @@ __device__ double mc_inter_direct(const McArgs& a, const LigInfo& li, const Pose
 __device__ double mc_intra(const McArgs& a, const LigInfo& li, const double* sx, const double* sy,
                            const double* sz) {
   const int lane = threadIdx.x & (kWave - 1);
+  const int n = li.n_pairs;
+  const uint32_t* pij = a.pair_ij + li.pair_base;
+  const double4* ppar = a.pair_par + li.pair_base;
   double e = 0.0;
-  for (int p = lane; p < li.n_pairs; p += kWave) {
-    const uint32_t ij = a.pair_ij[li.pair_base + p];
-    const int i = ij & 0xffff, j = ij >> 16;
-    const double4 pp = a.pair_par[li.pair_base + p];
-    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double ir = r2 >= 1.0e-4 ? rsqrt(r2) : 100.0;   // 1 / max(r, 0.01)
-    const double u = ir * ir, u3 = u * u * u, u6 = u3 * u3;
-    e += pp.x * ir + pp.y * u6 - pp.z * u3;
+  // four pairs per lane per pass, their loads issued together (one memory latency per
+  // four pairs); each lane still sums its pairs lane, lane + 64, ... in that order
+  for (int p0 = lane; p0 < n; p0 += 4 * kWave) {
+    uint32_t ij[4];
+    double4 pp[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int p = p0 + u * kWave;
+      ij[u] = p < n ? pij[p] : 0u;
+      pp[u] = p < n ? ppar[p] : make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (p0 + u * kWave >= n) break;
+      const int i = ij[u] & 0xffff, j = ij[u] >> 16;
+      const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double irr = rsqrt(fmax(r2, 1.0e-4));   // no branch around the refinement
+      const double ir = r2 >= 1.0e-4 ? irr : 100.0;   // 1 / max(r, 0.01)
+      const double uu = ir * ir, u3 = uu * uu * uu, u6 = u3 * u3;
+      e += pp[u].x * ir + pp[u].y * u6 - pp[u].z * u3;
+    }
   }
   return wave_sum(e);
 }
