@@ -320,6 +320,35 @@ def test_so3_super_fibonacci():
     assert np.allclose(synth.so3_rotations(n), R, atol=1e-13)
 
 
+def test_so3_super_fibonacci_pinned_by_its_definition():
+    """Alexa's super-Fibonacci spiral (CVPR 2022, Algorithm 1, the algorithm src/SO3.ml
+    cites): q_i = (r sin a, r cos a, R sin b, R cos b) with s = i + 1/2, r = sqrt(s/n),
+    R = sqrt(1 - s/n), a = 2 pi s / phi, b = 2 pi s / psi, phi^2 = 2 and psi the real root
+    of psi^4 = psi + 4.  The constants are recovered from the oracle's output and checked
+    against their defining equations; a swapped component pair, a swapped sin/cos, a
+    missing half offset or a wrong constant each fails one of the checks."""
+    n = 4608
+    q0 = oracle.so3_quat(n, 0)
+    # i = 0: a = pi / phi and b = pi / psi, both in (0, pi), so atan2 returns them unwrapped
+    phi = math.pi / math.atan2(q0[0], q0[1])
+    psi = math.pi / math.atan2(q0[2], q0[3])
+    assert abs(phi * phi - 2.0) < 1e-12
+    assert abs(psi ** 4 - psi - 4.0) < 1e-11
+    assert 1.5 < psi < 1.6
+    for i in (0, 1, 7, 1000, 4607):
+        q = oracle.so3_quat(n, i)
+        s = i + 0.5
+        assert abs(q[0] ** 2 + q[1] ** 2 - s / n) < 1e-14
+        assert abs(q[2] ** 2 + q[3] ** 2 - (1.0 - s / n)) < 1e-14
+        two_pi = 2.0 * math.pi
+        a = math.atan2(q[0], q[1]) % two_pi
+        b = math.atan2(q[2], q[3]) % two_pi
+        da = (a - (two_pi * s / math.sqrt(2.0)) % two_pi + math.pi) % two_pi - math.pi
+        assert abs(da) < 1e-9, (i, da)
+        db = (b - (two_pi * s / psi) % two_pi + math.pi) % two_pi - math.pi
+        assert abs(db) < 1e-9, (i, db)
+
+
 # the 8 known answers of the reference's own test, src/tests.ml:47-54
 def test_dihedral_known_answers_from_reference_tests():
     c4 = math.cos(0.25 * math.pi)
