@@ -103,36 +103,42 @@ __device__ void forward_torsions(const LigInfo& li, const TorsionLds& t, double*
 
 // src/mol.ml:881-903 over the ligand's pair list (topo distance >= 3) for the conformer
 // in LDS, FP64 as in k_intra, every thread of the workgroup taking pairs p = tid,
-// tid + blockDim.x, ... with the next pair loaded ahead; gradient terms go to this
-// wavefront's LDS copy g*.  Returns the wavefront's energy sum (uniform).
+// tid + blockDim.x, ... (four per pass, their loads issued together); gradient terms go
+// to this wavefront's LDS copy g*.  Returns the wavefront's energy sum (uniform).
 __device__ double intra_pairs(const uint32_t* pij, const double4* ppar, int n, const double* sx,
                               const double* sy, const double* sz, double* gx, double* gy,
                               double* gz) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  uint32_t ij_next = tid < n ? pij[tid] : 0u;
-  double4 pp_next = tid < n ? ppar[tid] : make_double4(0.0, 0.0, 0.0, 0.0);
   double e = 0.0;
-  for (int p = tid; p < n; p += nt) {
-    const uint32_t ij = ij_next;
-    const double4 pp = pp_next;   // ke qi qj, Ai Aj, 2 Bi Bj
-    if (p + nt < n) {
-      ij_next = pij[p + nt];
-      pp_next = ppar[p + nt];
+  for (int p0 = tid; p0 < n; p0 += 4 * nt) {
+    uint32_t ijs[4];
+    double4 pps[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int p = p0 + q * nt;
+      ijs[q] = p < n ? pij[p] : 0u;
+      pps[q] = p < n ? ppar[p] : make_double4(0.0, 0.0, 0.0, 0.0);   // ke qi qj, Ai Aj, 2 Bi Bj
     }
-    const int i = ij & 0xffff, j = ij >> 16;
-    const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const bool apart = r2 >= 1.0e-4;              // r >= 0.01 (Math.non_zero_dist)
-    const double ir = apart ? rsqrt(r2) : 100.0;  // 1 / max(r, 0.01)
-    const double u = ir * ir;
-    const double u3 = u * u * u;
-    const double u6 = u3 * u3;
-    e += pp.x * ir + pp.y * u6 - pp.z * u3;
-    if (apart) {
-      // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
-      const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
-      atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
-      atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (p0 + q * nt >= n) break;
+      const double4 pp = pps[q];
+      const int i = ijs[q] & 0xffff, j = ijs[q] >> 16;
+      const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool apart = r2 >= 1.0e-4;              // r >= 0.01 (Math.non_zero_dist)
+      const double irr = rsqrt(fmax(r2, 1.0e-4));   // no branch around the refinement
+      const double ir = apart ? irr : 100.0;        // 1 / max(r, 0.01)
+      const double u = ir * ir;
+      const double u3 = u * u * u;
+      const double u6 = u3 * u3;
+      e += pp.x * ir + pp.y * u6 - pp.z * u3;
+      if (apart) {
+        // dE/dr / r = -(ke qq / r^3) - 12 AA u^7 + 6 BB2 u^4
+        const double c = (-pp.x * ir * u - 12.0 * pp.y * u6 * u + 6.0 * pp.z * u3 * u);
+        atomicAdd(&gx[i], c * dx); atomicAdd(&gy[i], c * dy); atomicAdd(&gz[i], c * dz);
+        atomicAdd(&gx[j], -c * dx); atomicAdd(&gy[j], -c * dy); atomicAdd(&gz[j], -c * dz);
+      }
     }
   }
   return wave_sum(e);
