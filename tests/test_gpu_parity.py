@@ -1401,7 +1401,8 @@ def test_cell_packed_grids_bit_identical():
 
 def test_row_list_scan_grids_agree():
     """The row-list scan under several scan grids (1.5 A rows: several passes of 64
-    candidate rows per atom; 6 A rows; x cuts 1, 2, 16) and both receptor paths gives the
+    candidate rows per atom; 6 A rows; x cuts 1, 2, 16; 10 A rows longer than 64 atoms)
+    and both receptor paths gives the
     energies of the default grid to FP32 rounding, and the oracle's on a sample: a row
     grid only changes which pairs beyond the cutoff are visited.  With and without the
     per-atom box test of the far rows (row_filter) the energies are bit-identical: the
@@ -1420,7 +1421,8 @@ def test_row_list_scan_grids_agree():
     pr, pt = (sample // len(trans)).astype(np.int32), (sample % len(trans)).astype(np.int32)
     want = oracle.scan_shifted(rec, lig, rots, pr, trans, pt, nthreads=8)
     sums = [inter_abs_sum(rec, _place(lig, rots[a], trans[b])[1], lig, True) for a, b in zip(pr, pt)]
-    for cell, xsub, chunk in ((1.5, 1, 4), (6.0, 2, 8), (3.5, 16, 4)):
+    # 10 A rows uncut along x hold more than 64 atoms: the atom test covers the first 64
+    for cell, xsub, chunk in ((1.5, 1, 4), (6.0, 2, 8), (3.5, 16, 4), (10.0, 1, 4)):
         ctx.tune(scan_cell=cell, scan_xsub=xsub, scan_chunk=chunk)
         ctx.set_receptor(rec)
         for sc in (5, 6):
