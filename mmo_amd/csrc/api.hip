@@ -2489,7 +2489,7 @@ int mmo_min_run_dev(mmo_min* z, double* dofs, int iters, double* e_out, void* st
       return fail(c, MMO_ERR_BAD_ARG, "ligands of %d atoms exceed the minimiser's LDS", b->max_atoms);
     bool rec_lds = false;
     if (kind != kDofInterGrid && c->min_rec_lds) {
-      int wl = waves;
+      int wl = std::min(waves, 8);   // the receptor-in-LDS form is compiled for 8 wavefronts
       while (wl > 1 && min_persist_lds_bytes(p, wl, true) > kMaxLds) --wl;
       if (wl >= 8 && min_persist_lds_bytes(p, wl, true) <= kMaxLds) {
         rec_lds = true;

@@ -78,9 +78,15 @@ __host__ __device__ inline size_t min_wave_lds_bytes(int max_atoms) {
 }
 
 constexpr int kMinPersistWaves = 16;
+// Wavefronts per workgroup each form is compiled for.  With the receptor in LDS at most
+// 8-9 wavefronts fit in a workgroup's LDS anyway, so that form is compiled for 8 (up to
+// 256 VGPRs, 2 wavefronts per SIMD): at 16 (128 VGPRs) it spilled 236 B per lane.
+__host__ __device__ constexpr int min_persist_waves(bool rec_lds) {
+  return rec_lds ? 8 : kMinPersistWaves;
+}
 
 template <bool REC_LDS, int INTER>
-__global__ void __launch_bounds__(kMinPersistWaves * 64) k_min_persist(MinPersistArgs a) {
+__global__ void __launch_bounds__(min_persist_waves(REC_LDS) * 64) k_min_persist(MinPersistArgs a) {
   extern __shared__ __align__(16) char smem[];
   const RecPos* s_pos;
   const RecAB* s_ab;
@@ -161,7 +167,7 @@ static hipError_t min_persist_go(const MinPersistArgs& a, int nblocks, int waves
 hipError_t launch_min_persist(const MinPersistArgs& a, int inter, int nblocks, int waves,
                               bool rec_lds, hipStream_t s) {
   if (a.npose == 0) return hipSuccess;
-  if (waves < 1 || waves > kMinPersistWaves) return hipErrorInvalidValue;
+  if (waves < 1 || waves > min_persist_waves(rec_lds)) return hipErrorInvalidValue;
   switch (inter + (rec_lds ? 8 : 0)) {
     case kDofInterAtom + 8: return min_persist_go<true, kDofInterAtom>(a, nblocks, waves, s);
     case kDofInterF64 + 8: return min_persist_go<true, kDofInterF64>(a, nblocks, waves, s);
