@@ -251,7 +251,7 @@ def bench_select(ctx, lig, rec, lid, rots, trans, out, k=100):
     the protein vdW bitmask) and the top-k of the filtered energies, on the energies of
     the direct scan still in `out`."""
     import torch
-    from mmo_amd import mol2
+    from mmo_amd import mol as molmod
     rxyz = np.stack([rec["xs"], rec["ys"], rec["zs"]], 1)
     step = 0.375
     origin = rxyz.min(axis=0) - 2.0
@@ -259,7 +259,7 @@ def bench_select(ctx, lig, rec, lid, rots, trans, out, k=100):
     t0 = time.perf_counter()
     ctx.set_vdw_mask(step, dims, origin)
     mask_s = time.perf_counter() - t0
-    radii = [mol2.VDW_RADII[int(a)] for a in lig["anum"]]
+    radii = [molmod.VDW_RADII[int(a)] for a in lig["anum"]]
     roi = (np.zeros(3), 4.75 + 1e-9)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -26,6 +26,11 @@ ANUM_XI_DI = {0: (0.0, 0.0), 1: (2.886, 0.044), 6: (3.851, 0.105), 7: (3.660, 0.
               8: (3.500, 0.060), 9: (3.364, 0.050), 12: (3.021, 0.111), 15: (4.147, 0.305),
               16: (4.035, 0.274), 17: (3.947, 0.227), 35: (4.189, 0.251), 53: (4.500, 0.339)}
 ELEC_WEIGHT_PROTEIN = 332.0637 / 4.0      # src/UFF.ml:25
+
+# Ptable.vdW_radii (src/ptable.ml:43-53), by atomic number: the ligand atom radii of the
+# scan's clash filters
+VDW_RADII = {1: 1.2, 6: 1.7, 7: 1.6, 8: 1.55, 9: 1.5, 12: 2.2, 15: 1.95, 16: 1.8, 17: 1.8,
+             35: 1.9, 53: 2.1}
 SYM2ANUM = {"C": 6, "H": 1, "N": 7, "O": 8, "P": 15, "S": 16, "F": 9, "Cl": 17, "Br": 35,
             "I": 53, "Mg": 12}            # src/ptable.ml:86-100
 ANUM2SYM = {v: k for k, v in SYM2ANUM.items()}
@@ -177,6 +182,22 @@ def _molecule(lib, h, i):
     groups = [ga[off[k]:off[k + 1]] for k in range(nrb.value)]
     return Mol(name, xs, ys, zs, q, an, r_a=r, dists_a=dists,
                rbonds_a=list(zip(rl.tolist(), rr.tolist())), rgroups_a=groups)
+
+
+# .mol2 input (the reference's mol2pqrs step): `mmo_molset_from_mol2` reads the Tripos
+# blocks (lone pairs removed, src/mol2.ml:99-323), derives the topology (topological
+# distances, rotatable bonds, rotation groups, src/mol_graph.ml:53-192), writes the .pqrs
+# block of Mol2pqrs.process_ligand (src/mol2pqrs.ml:9-43) and reads it back, so ligands
+# from .mol2 and from the .pqrs file the reference would write are identical records.
+def ligands_of_mol2_file(fn):
+    with open(fn) as f:
+        return molecules_of_text(f.read(), "mol2")[0]
+
+
+def pqrs_text_of_mol2(fn):
+    """The .pqrs text mol2pqrs writes for every molecule of the file."""
+    with open(fn) as f:
+        return molecules_of_text(f.read(), "mol2")[1]
 
 
 def ligands_of_pqrs_file(fn):

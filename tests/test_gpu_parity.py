@@ -1305,14 +1305,14 @@ def test_mol2_ligands_score_like_the_oracle(sys0, tmp_path):
     """Ligands read from .mol2 (topology, rotatable bonds and groups computed here)
     flow into the DOF kernels: coordinates and energies agree with the oracle."""
     import torch
-    from mmo_amd import mol2
+    from mmo_amd import mol as molmod
     from tests.test_mol2 import BUTANE, RING
     ctx, _, rec = sys0
     fn = str(tmp_path / "ligs.mol2")
     with open(fn, "w") as f:
         f.write(BUTANE + RING)
     ligs = []
-    for m in mol2.ligands_of_mol2_file(fn):
+    for m in molmod.ligands_of_mol2_file(fn):
         m = m.with_coords(m.coords() - m.coords().mean(axis=0))   # Lds.preprocess_ligand
         n = m.num_atoms()
         d = {"xs": m.xs, "ys": m.ys, "zs": m.zs, "q": m.q_a, "anum": m.elt_a,

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from mmo_amd import mol, mol2
+from mmo_amd import mol
 from oracle import mol2_ref as ref
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -110,7 +110,7 @@ def test_ring_bonds_lone_pairs_and_bond_orders(tmp_path):
 
 def test_pqrs_text_round_trip(tmp_path):
     fn = _write(tmp_path, BUTANE + RING, "two.mol2")
-    ligs = mol2.ligands_of_mol2_file(fn)
+    ligs = mol.ligands_of_mol2_file(fn)
     assert [l.name for l in ligs] == ["butane", "ring_lp"]
     b = ligs[0]
     assert b.num_rbonds() == 3
@@ -121,7 +121,7 @@ def test_pqrs_text_round_trip(tmp_path):
     d = np.asarray(b.dists_a).reshape(n, n)
     assert d[0, 3] == 3 and d[3, 0] == 3
     assert np.allclose(b.r_a[:4], 1.7) and np.allclose(b.r_a[4:], 1.2)
-    text = mol2.pqrs_text_of_mol2(fn)
+    text = mol.pqrs_text_of_mol2(fn)
     assert text.splitlines()[0] == "14:3:butane"
     assert text.splitlines()[1] == "0 0 0 -0.18 1.7 C"
 
@@ -142,18 +142,18 @@ def _same_molecules(got, want):
 @pytest.mark.parametrize("name", REF_MOL2)
 def test_library_reader_matches_oracle_on_reference_files(name):
     fn = os.path.join(GOLDEN, name)
-    text = mol2.pqrs_text_of_mol2(fn)
+    text = mol.pqrs_text_of_mol2(fn)
     assert text == ref.mol2_to_pqrs_text(fn)
-    _same_molecules(mol2.ligands_of_mol2_file(fn), ref.read_pqrs_text(text))
+    _same_molecules(mol.ligands_of_mol2_file(fn), ref.read_pqrs_text(text))
     # the .pqrs route gives the same molecules
     _same_molecules(mol.ligands_of_pqrs_text(text), ref.read_pqrs_text(text))
 
 
 def test_library_reader_matches_oracle_on_known_topologies(tmp_path):
     fn = _write(tmp_path, BUTANE + RING, "two.mol2")
-    text = mol2.pqrs_text_of_mol2(fn)
+    text = mol.pqrs_text_of_mol2(fn)
     assert text == ref.mol2_to_pqrs_text(fn)
-    _same_molecules(mol2.ligands_of_mol2_file(fn), ref.read_pqrs_text(text))
+    _same_molecules(mol.ligands_of_mol2_file(fn), ref.read_pqrs_text(text))
 
 
 def test_reference_mol2_files_topology_invariants():
@@ -187,6 +187,6 @@ def test_receptor_pqrs_and_errors(tmp_path):
     with pytest.raises(ValueError, match="rot-bond"):
         mol.ligands_of_pqrs_text("\n".join(lines) + "\n")
     with pytest.raises(ValueError, match="unsupported mol2 atom type"):
-        mol2.ligands_of_mol2_text(BUTANE.replace(" C.3 ", " Q.9 ", 1))
+        mol.molecules_of_text(BUTANE.replace(" C.3 ", " Q.9 ", 1), "mol2")
     with pytest.raises(ValueError, match="disconnected atom"):
-        mol2.ligands_of_mol2_text(BUTANE.replace("    13     4    14 1\n", "").replace("   14    13 ", "   14    12 "))
+        mol.molecules_of_text(BUTANE.replace("    13     4    14 1\n", "").replace("   14    13 ", "   14    12 "), "mol2")
