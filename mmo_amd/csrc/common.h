@@ -240,10 +240,12 @@ __device__ __forceinline__ float pair_r2(float dx, float dy, float dz, const Lig
   float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
   if (__builtin_expect(__ballot(r2 < pk.near2) != 0ull, 0)) {
     if (r2 < pk.near2) {
-      // 32-bit byte offset from the (wave-uniform) base: a scalar-base load, with no
-      // 64-bit address arithmetic on the pair path
+      // 32-bit byte offset from the (wave-uniform) base (a 24-bit multiply, full rate),
+      // formed here: the empty asm keeps the compiler from hoisting it onto every pair
+      int ii = i;
+      asm volatile("" : "+v"(ii));
       const RecRes rr = *reinterpret_cast<const RecRes*>(
-          reinterpret_cast<const char*>(pk.rres) + (uint32_t)i * (uint32_t)sizeof(RecRes));
+          reinterpret_cast<const char*>(pk.rres) + __umul24((uint32_t)ii, (uint32_t)sizeof(RecRes)));
       const float ex = fmaf(dx, pk.inv_s, rr.x - l.rx);
       const float ey = fmaf(dy, pk.inv_s, rr.y - l.ry);
       const float ez = fmaf(dz, pk.inv_s, rr.z - l.rz);
