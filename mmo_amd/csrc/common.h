@@ -260,6 +260,28 @@ __device__ __forceinline__ float max_r2(float r2, float min_r2) {
   return __int_as_float(max(__float_as_int(r2), __float_as_int(min_r2)));
 }
 
+// pair_r2 followed by the 0.01 A clamp (max_r2).  A wavefront none of whose pairs is
+// closer than sqrt(near2) skips both: its r2 are above the clamp already, so the result
+// is the same and the clamp costs nothing outside the rare branch.
+__device__ __forceinline__ float pair_r2_clamped(float dx, float dy, float dz, const LigPt& l,
+                                                 int i, const PairK& pk) {
+  float r2 = (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
+  if (__builtin_expect(__ballot(r2 < pk.near2) != 0ull, 0)) {
+    if (r2 < pk.near2) {
+      int ii = i;
+      asm volatile("" : "+v"(ii));
+      const RecRes rr = *reinterpret_cast<const RecRes*>(
+          reinterpret_cast<const char*>(pk.rres) + __umul24((uint32_t)ii, (uint32_t)sizeof(RecRes)));
+      const float ex = fmaf(dx, pk.inv_s, rr.x - l.rx);
+      const float ey = fmaf(dy, pk.inv_s, rr.y - l.ry);
+      const float ez = fmaf(dz, pk.inv_s, rr.z - l.rz);
+      r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+    }
+    r2 = max_r2(r2, pk.min_r2);
+  }
+  return r2;
+}
+
 // 1/r^2 for the LJ powers.  A Newton step on u = ir^2 was measured to change nothing:
 // over 8,192 poses of the rigid scan the largest error against FP64 stays 1.9e-6 with or
 // without it (the error of clashing pairs came from the coordinate grid, see LigPt)

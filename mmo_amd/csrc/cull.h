@@ -32,8 +32,9 @@ __device__ __forceinline__ void pair_energy(const RecPos& p, const RecAB& ab, co
                                             int i, const PairK& pk, float& sq, float& s12,
                                             float& s6) {
   const float dx = (float)(p.x - l.x), dy = (float)(p.y - l.y), dz = (float)(p.z - l.z);
-  const float r2 = NEAR ? pair_r2(dx, dy, dz, l, i, pk) : (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
-  const float r2c = NEAR ? max_r2(r2, pk.min_r2) : r2;
+  // NEAR: the clamp is applied in the near-pair branch (pair_r2_clamped); else not needed
+  const float r2c = NEAR ? pair_r2_clamped(dx, dy, dz, l, i, pk)
+                         : (dx * dx + dy * dy + dz * dz) * pk.inv_s2;
   const float ir = __builtin_amdgcn_rsqf(r2c);
   const float u = inv_r2(ir, r2c);
   const float u3 = u * (u * u);
